@@ -1,0 +1,131 @@
+"""Runs the reference's own CPU SP2 path (built by oracle/ref/Makefile into
+oracle/_ref/modem_ref) on one grid/model and reads back what it computed.
+
+Test infrastructure only: tests use it as a checker for the oracle
+restatement and, through the golden fixtures, for the HIP library.  The
+driver program is oracle/ref/ref_driver.f90; its input layout is documented
+there.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_BIN = os.path.join(HERE, "_ref", "modem_ref")
+
+
+def available() -> bool:
+    return os.path.isfile(REF_BIN) and os.access(REF_BIN, os.X_OK)
+
+
+def _rd_csr(path, cplx=False):
+    with open(path, "rb") as f:
+        nrow, ncol, nnz = np.fromfile(f, np.int32, 3)
+        row = np.fromfile(f, np.int32, nrow + 1)
+        col = np.fromfile(f, np.int32, nnz)
+        val = np.fromfile(f, np.complex128 if cplx else np.float64, nnz)
+    return {"nrow": int(nrow), "ncol": int(ncol), "row": row, "col": col, "val": val}
+
+
+def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
+        e2m=None, workdir=None, timeout=600):
+    """grid: dict(dx, dy, dz, nzAir); logcond: (nx, ny, nzEarth) natural-log
+    conductivities.  solves: list of (iomega0, maxit, tol, b, x0).  applies:
+    list of (iomega0, x).  divcor: (maxit, tol, rhs) or None.  e2m: real
+    all-edge vector or None.  Returns a dict of numpy arrays."""
+    dx = np.asarray(grid["dx"], np.float64)
+    dy = np.asarray(grid["dy"], np.float64)
+    dz = np.asarray(grid["dz"], np.float64)
+    nx, ny, nz = len(dx), len(dy), len(dz)
+    nza = int(grid["nzAir"])
+    nze = nz - nza
+    lc = np.asarray(logcond, np.float64)
+    assert lc.shape == (nx, ny, nze)
+    own = workdir is None
+    wd = tempfile.mkdtemp(prefix="mtref_") if own else workdir
+    inp = os.path.join(wd, "input.bin")
+    with open(inp, "wb") as f:
+        np.array([nx, ny, nza, nze], np.int32).tofile(f)
+        dx.tofile(f); dy.tofile(f); dz.tofile(f)
+        lc.ravel(order="F").tofile(f)
+        np.array([air_log], np.float64).tofile(f)
+        np.array([len(omegas)], np.int32).tofile(f)
+        np.asarray(omegas, np.float64).tofile(f)
+        np.array([len(solves)], np.int32).tofile(f)
+        for (iom, maxit, tol, b, x0) in solves:
+            np.array([iom + 1, maxit], np.int32).tofile(f)
+            np.array([tol], np.float64).tofile(f)
+            np.asarray(b, np.complex128).tofile(f)
+            np.asarray(x0, np.complex128).tofile(f)
+        np.array([len(applies)], np.int32).tofile(f)
+        for (iom, x) in applies:
+            np.array([iom + 1], np.int32).tofile(f)
+            np.asarray(x, np.complex128).tofile(f)
+        if divcor is None:
+            np.array([0], np.int32).tofile(f)
+        else:
+            maxit, tol, rhs = divcor
+            np.array([1, maxit], np.int32).tofile(f)
+            np.array([tol], np.float64).tofile(f)
+            np.asarray(rhs, np.complex128).tofile(f)
+        if e2m is None:
+            np.array([0], np.int32).tofile(f)
+        else:
+            np.array([1], np.int32).tofile(f)
+            np.asarray(e2m, np.float64).tofile(f)
+    subprocess.run([REF_BIN, inp, wd], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    out = {}
+    with open(os.path.join(wd, "index.bin"), "rb") as f:
+        nall, nint, nb, nnall, nni, nnb = np.fromfile(f, np.int32, 6)
+        out["EDGEi"] = np.fromfile(f, np.int32, nint)
+        out["EDGEb"] = np.fromfile(f, np.int32, nb)
+        out["NODEi"] = np.fromfile(f, np.int32, nni)
+        out["NODEb"] = np.fromfile(f, np.int32, nnb)
+    nfx, nfy, nfz = (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
+    nface = nfx + nfy + nfz
+    with open(os.path.join(wd, "metric.bin"), "rb") as f:
+        out["FaceA"] = np.fromfile(f, np.float64, nface)
+        out["EdgeL"] = np.fromfile(f, np.float64, nall)
+        out["DualFaceA"] = np.fromfile(f, np.float64, nall)
+        out["DualEdgeL"] = np.fromfile(f, np.float64, nface)
+        out["Vnode"] = np.fromfile(f, np.float64, nnall)
+        out["Vedge"] = np.fromfile(f, np.float64, nall)
+    with open(os.path.join(wd, "sigma.bin"), "rb") as f:
+        out["SigEdge"] = np.fromfile(f, np.float64, nall)
+        out["SigNode"] = np.fromfile(f, np.float64, nnall)
+    for name in ("AAii", "CCii", "CCib", "GDii", "T", "G"):
+        out[name] = _rd_csr(os.path.join(wd, name + ".csr"))
+    for k in range(len(omegas)):
+        out[f"vms_{k}"] = np.fromfile(os.path.join(wd, f"vms_{k+1}.bin"), np.float64)
+        out[f"L_{k}"] = _rd_csr(os.path.join(wd, f"L_{k+1}.csr"), cplx=True)
+        out[f"U_{k}"] = _rd_csr(os.path.join(wd, f"U_{k+1}.csr"), cplx=True)
+    for k in range(len(solves)):
+        with open(os.path.join(wd, f"bicg_{k+1}.bin"), "rb") as f:
+            niter, failed, maxit = np.fromfile(f, np.int32, 3)
+            rerr = np.fromfile(f, np.float64, maxit)
+            x = np.fromfile(f, np.complex128, nint)
+        out[f"bicg_{k}"] = {"niter": int(niter), "failed": bool(failed), "rerr": rerr, "x": x}
+    for k in range(len(applies)):
+        a = np.fromfile(os.path.join(wd, f"apply_{k+1}.bin"), np.complex128)
+        out[f"apply_{k}"] = {"Ax": a[:nint], "Lsolve": a[nint:2 * nint], "LUsolve": a[2 * nint:]}
+    if divcor is not None:
+        for name in ("VDiv", "VDs", "Gai", "VDsG", "VDsG_L", "VDsG_U"):
+            out[name] = _rd_csr(os.path.join(wd, name + ".csr"))
+        with open(os.path.join(wd, "pcg.bin"), "rb") as f:
+            niter, maxit = np.fromfile(f, np.int32, 2)
+            rerr = np.fromfile(f, np.float64, maxit)
+            phi = np.fromfile(f, np.complex128, nni)
+        out["pcg"] = {"niter": int(niter), "rerr": rerr, "phi": phi}
+    if e2m is not None:
+        out["e2m"] = np.fromfile(os.path.join(wd, "e2m.bin"), np.float64).reshape(
+            (nx, ny, nze), order="F")
+    if own:
+        for fn in os.listdir(wd):
+            os.remove(os.path.join(wd, fn))
+        os.rmdir(wd)
+    return out

@@ -1,0 +1,1422 @@
+/*
+ * sp2_oracle.c -- CPU restatement of ModEM's SP2 3D-MT path (TEST INFRASTRUCTURE).
+ *
+ * The oracle the HIP library is checked against.  Only tests/, the smoke check
+ * and the benchmark's CPU-baseline leg load it; the product never does.
+ *
+ * Each function names the reference routine it restates (paths relative to
+ * the reference's f90/ tree).  Loop and summation order follow the reference
+ * so that assembled operators agree bit for bit with the flang build of the
+ * reference (tests/test_oracle_vs_ref.py).  Build: oracle/Makefile, with
+ * -ffp-contract=off (the reference's flang build uses no FMA) and C99 complex
+ * arithmetic (gcc and flang both divide through __divdc3).
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   grid/metric/topology/index sets/sigma maps/AAii/CCii/CCib/GDii/L/U/VDsG:
+ *     bit-exact against the flang reference build; plus T*G = 0 exactly and
+ *     CC*G = 0 to rounding.
+ *   Mult_Aii, PC_Lsolve/PC_Usolve: agreement with the reference to 1e-15 rel;
+ *     L*U reproduces the block pattern of A (DILU invariant).
+ *   BiCGStab (both semantics): converged solution vs the reference CPU BICG and
+ *     vs a dense direct solve on small grids.
+ *   PCG: vs the reference PCG and a dense solve.
+ *   RHS/post-processing: closed forms against the reference's definitions.
+ *   edge2model / pmultT: against the reference dEdgeToModelParam dump and
+ *     the adjoint identity <Cell2Edge m, e> = <m, Edge2Cell e> in V-weights.
+ */
+#include "sp2_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* math_constants.f90:12-19 (note the truncated PI the reference uses) */
+#define O_PI 3.14159265357898
+#define O_MU0 (O_PI * 4e-7)
+#define O_ISIGN (-1)
+#define O_SIGMA_AIR 1e-10
+
+
+/* Complex division as the reference's flang build performs it: flang lowers
+ * every complex quotient (complex/complex and complex/real) to __divdc3, whose
+ * algorithm is the C99 Annex G example (G.5.1, _Cdivd): scale the divisor by
+ * 2^-logb(max(|c|,|d|)), divide, scale back.  Restated here from the standard
+ * so that quotients round exactly as in the reference. */
+static zdouble zdiv(zdouble num, zdouble den) {
+  double a = creal(num), b = cimag(num), c = creal(den), d = cimag(den);
+  int ilogbw = 0;
+  double logbw = logb(fmax(fabs(c), fabs(d)));
+  if (isfinite(logbw)) {
+    ilogbw = (int)logbw;
+    c = scalbn(c, -ilogbw);
+    d = scalbn(d, -ilogbw);
+  }
+  double denom = c * c + d * d;
+  double x = scalbn((a * c + b * d) / denom, -ilogbw);
+  double y = scalbn((b * c - a * d) / denom, -ilogbw);
+  if (isnan(x) && isnan(y)) {
+    if (denom == 0.0 && (!isnan(a) || !isnan(b))) {
+      x = copysign(INFINITY, c) * a;
+      y = copysign(INFINITY, c) * b;
+    } else if ((isinf(a) || isinf(b)) && isfinite(c) && isfinite(d)) {
+      a = copysign(isinf(a) ? 1.0 : 0.0, a);
+      b = copysign(isinf(b) ? 1.0 : 0.0, b);
+      x = INFINITY * (a * c + b * d);
+      y = INFINITY * (b * c - a * d);
+    } else if (isinf(logbw) && logbw > 0.0 && isfinite(a) && isfinite(b)) {
+      c = copysign(isinf(c) ? 1.0 : 0.0, c);
+      d = copysign(isinf(d) ? 1.0 : 0.0, d);
+      x = 0.0 * (a * c + b * d);
+      y = 0.0 * (b * c - a * d);
+    }
+  }
+  return CMPLX(x, y);
+}
+static zdouble zdivr(zdouble num, double den) { return zdiv(num, CMPLX(den, 0.0)); }
+
+#define XALLOC(T, n) ((T *)calloc((size_t)((n) > 0 ? (n) : 1), sizeof(T)))
+
+/* ------------------------------------------------------------------ grid */
+typedef struct {
+  int nx, ny, nz, nzAir, nzEarth;
+  double *dx, *dy, *dz, *delX, *delY, *delZ;
+} o_grid;
+
+/* GridDef.f90:301-399 setup_grid (delX etc.) */
+static void grid_setup(o_grid *g) {
+  int i;
+  g->delX = XALLOC(double, g->nx + 1);
+  g->delY = XALLOC(double, g->ny + 1);
+  g->delZ = XALLOC(double, g->nz + 1);
+  g->delX[0] = g->dx[0];
+  for (i = 1; i < g->nx; i++) g->delX[i] = g->dx[i - 1] + g->dx[i];
+  g->delX[g->nx] = g->dx[g->nx - 1];
+  for (i = 0; i <= g->nx; i++) g->delX[i] = g->delX[i] / 2.0;
+  g->delY[0] = g->dy[0];
+  for (i = 1; i < g->ny; i++) g->delY[i] = g->dy[i - 1] + g->dy[i];
+  g->delY[g->ny] = g->dy[g->ny - 1];
+  for (i = 0; i <= g->ny; i++) g->delY[i] = g->delY[i] / 2.0;
+  g->delZ[0] = g->dz[0];
+  for (i = 1; i < g->nz; i++) g->delZ[i] = g->dz[i - 1] + g->dz[i];
+  g->delZ[g->nz] = g->dz[g->nz - 1];
+  for (i = 0; i <= g->nz; i++) g->delZ[i] = g->delZ[i] / 2.0;
+}
+
+/* component dims, vecTranslate.f90:275-325 setLimits */
+enum { XEDGE, YEDGE, ZEDGE, XFACE, YFACE, ZFACE, CORNER, CENTER };
+static void limits(const o_grid *g, int t, int *a, int *b, int *c) {
+  int nx = g->nx, ny = g->ny, nz = g->nz;
+  switch (t) {
+    case XEDGE: *a = nx; *b = ny + 1; *c = nz + 1; break;
+    case YEDGE: *a = nx + 1; *b = ny; *c = nz + 1; break;
+    case ZEDGE: *a = nx + 1; *b = ny + 1; *c = nz; break;
+    case XFACE: *a = nx + 1; *b = ny; *c = nz; break;
+    case YFACE: *a = nx; *b = ny + 1; *c = nz; break;
+    case ZFACE: *a = nx; *b = ny; *c = nz + 1; break;
+    case CORNER: *a = nx + 1; *b = ny + 1; *c = nz + 1; break;
+    default: *a = nx; *b = ny; *c = nz; break;
+  }
+}
+static int ncomp(const o_grid *g, int t) {
+  int a, b, c;
+  limits(g, t, &a, &b, &c);
+  return a * b * c;
+}
+/* vectorIndex (vecTranslate.f90:367-390), 0-based i,j,k -> 0-based index */
+static int vidx(const o_grid *g, int t, int i, int j, int k) {
+  int a, b, c;
+  limits(g, t, &a, &b, &c);
+  return (k * b + j) * a + i;
+}
+
+/* ------------------------------------------------------------ CSR toolkit */
+static o_csr csr_new(int m, int n, int nz) {
+  o_csr A;
+  A.nrow = m;
+  A.ncol = n;
+  A.row = XALLOC(int, m + 1);
+  A.col = XALLOC(int, nz);
+  A.val = XALLOC(double, nz);
+  return A;
+}
+static void csr_free(o_csr *A) {
+  free(A->row); free(A->col); free(A->val);
+  A->row = A->col = NULL; A->val = NULL;
+}
+static o_csrz csrz_new(int m, int n, int nz) {
+  o_csrz A;
+  A.nrow = m;
+  A.ncol = n;
+  A.row = XALLOC(int, m + 1);
+  A.col = XALLOC(int, nz);
+  A.val = XALLOC(zdouble, nz);
+  return A;
+}
+static void csrz_free(o_csrz *A) {
+  free(A->row); free(A->col); free(A->val);
+  A->row = A->col = NULL; A->val = NULL;
+}
+static int nnz_of(const o_csr *A) { return A->row[A->nrow]; }
+
+/* drop exact zeros keeping order (the compaction step shared by
+ * DIAGxRMAT/RMATxDIAG/RMATxRMAT/RMATplusRMAT, spOpTools.f90:680-844, 477-596) */
+static o_csr csr_compact(o_csr *T) {
+  int m = T->nrow, i, j, k = 0, nz = 0;
+  for (j = 0; j < nnz_of(T); j++) nz += (fabs(T->val[j]) > 0);
+  o_csr B = csr_new(m, T->ncol, nz);
+  for (i = 0; i < m; i++) {
+    B.row[i] = k;
+    for (j = T->row[i]; j < T->row[i + 1]; j++)
+      if (fabs(T->val[j]) > 0) { B.col[k] = T->col[j]; B.val[k] = T->val[j]; k++; }
+  }
+  B.row[m] = k;
+  csr_free(T);
+  return B;
+}
+
+/* DIAGxRMAT spOpTools.f90:680-744: B = diag(D) A */
+static o_csr diag_x_rmat(const double *D, const o_csr *A) {
+  o_csr T = csr_new(A->nrow, A->ncol, nnz_of(A));
+  memcpy(T.row, A->row, sizeof(int) * (A->nrow + 1));
+  for (int i = 0; i < A->nrow; i++)
+    for (int j = A->row[i]; j < A->row[i + 1]; j++) {
+      T.val[j] = D[i] * A->val[j];
+      T.col[j] = A->col[j];
+    }
+  return csr_compact(&T);
+}
+/* RMATxDIAG spOpTools.f90:779-844: B = A diag(D) */
+static o_csr rmat_x_diag(const o_csr *A, const double *D) {
+  o_csr T = csr_new(A->nrow, A->ncol, nnz_of(A));
+  memcpy(T.row, A->row, sizeof(int) * (A->nrow + 1));
+  for (int i = 0; i < A->nrow; i++)
+    for (int j = A->row[i]; j < A->row[i + 1]; j++) {
+      T.val[j] = A->val[j] * D[A->col[j]];
+      T.col[j] = A->col[j];
+    }
+  return csr_compact(&T);
+}
+/* RMATtrans spOpTools.f90:917-948 via CSR2IJS/IJS2CSR (:285-361): a stable
+ * counting sort, so transposed rows list their columns in ascending order. */
+static o_csr rmat_trans(const o_csr *A) {
+  int nz = nnz_of(A), i, j;
+  o_csr B = csr_new(A->ncol, A->nrow, nz);
+  int *cnt = XALLOC(int, A->ncol + 1);
+  for (j = 0; j < nz; j++) cnt[A->col[j]]++;
+  B.row[0] = 0;
+  for (i = 0; i < A->ncol; i++) B.row[i + 1] = B.row[i] + cnt[i];
+  memset(cnt, 0, sizeof(int) * (A->ncol + 1));
+  for (i = 0; i < A->nrow; i++)
+    for (j = A->row[i]; j < A->row[i + 1]; j++) {
+      int r = A->col[j], pos = B.row[r] + cnt[r]++;
+      B.col[pos] = i;
+      B.val[pos] = A->val[j];
+    }
+  free(cnt);
+  return B;
+}
+/* RMATxRMAT spOpTools.f90:477-596: C = A B; columns of a row in first-encounter
+ * order, partial sums in encounter order, exact zeros dropped. */
+static o_csr rmat_x_rmat(const o_csr *A, const o_csr *B) {
+  int m = A->nrow, i, j, k, l;
+  int maxA = 0, maxB = 0;
+  for (i = 0; i < A->nrow; i++) maxA = (A->row[i + 1] - A->row[i]) > maxA ? (A->row[i + 1] - A->row[i]) : maxA;
+  for (i = 0; i < B->nrow; i++) maxB = (B->row[i + 1] - B->row[i]) > maxB ? (B->row[i + 1] - B->row[i]) : maxB;
+  int cap = maxA * maxB;
+  int *colT = XALLOC(int, cap);
+  int *rowT = XALLOC(int, m + 1);
+  for (i = 0; i < m; i++) {
+    int nc = 0;
+    for (j = A->row[i]; j < A->row[i + 1]; j++) {
+      int jj = A->col[j];
+      for (k = B->row[jj]; k < B->row[jj + 1]; k++) {
+        int nw = 1;
+        for (l = 0; l < nc; l++) if (colT[l] == B->col[k]) { nw = 0; break; }
+        if (nw) colT[nc++] = B->col[k];
+      }
+    }
+    rowT[i + 1] = rowT[i] + nc;
+  }
+  o_csr T = csr_new(m, B->ncol, rowT[m]);
+  memcpy(T.row, rowT, sizeof(int) * (m + 1));
+  for (i = 0; i < m; i++) {
+    int nc = 0, i1 = T.row[i];
+    for (j = A->row[i]; j < A->row[i + 1]; j++) {
+      int jj = A->col[j];
+      for (k = B->row[jj]; k < B->row[jj + 1]; k++) {
+        int pos = -1;
+        for (l = 0; l < nc; l++) if (colT[l] == B->col[k]) { pos = l; break; }
+        if (pos < 0) {
+          colT[nc] = B->col[k];
+          T.val[i1 + nc] = A->val[j] * B->val[k];
+          T.col[i1 + nc] = B->col[k];
+          nc++;
+        } else {
+          T.val[i1 + pos] = T.val[i1 + pos] + A->val[j] * B->val[k];
+        }
+      }
+    }
+  }
+  free(colT);
+  free(rowT);
+  return csr_compact(&T);
+}
+/* subMatrix_Real spOpTools.f90:1334-1383 */
+static o_csr submatrix(const o_csr *A, const int *r, int m, const int *c, int n) {
+  int *colT = XALLOC(int, A->ncol);
+  int i, j, nz = 0;
+  for (i = 0; i < A->ncol; i++) colT[i] = -1;
+  for (i = 0; i < n; i++) colT[c[i]] = i;
+  for (i = 0; i < m; i++)
+    for (j = A->row[r[i]]; j < A->row[r[i] + 1]; j++) nz += (colT[A->col[j]] >= 0);
+  o_csr B = csr_new(m, n, nz);
+  int k = 0;
+  for (i = 0; i < m; i++) {
+    B.row[i] = k;
+    for (j = A->row[r[i]]; j < A->row[r[i] + 1]; j++)
+      if (colT[A->col[j]] >= 0) { B.col[k] = colT[A->col[j]]; B.val[k] = A->val[j]; k++; }
+  }
+  B.row[m] = k;
+  free(colT);
+  return B;
+}
+/* RMATplusRMAT spOpTools.f90:2197-2297 (droptol = 0): A's entries first, then
+ * B's new columns appended; sums that are exactly zero are dropped. */
+static o_csr rmat_plus_rmat(const o_csr *A, const o_csr *B) {
+  int m = A->nrow, i, j, k;
+  o_csr T = csr_new(m, A->ncol, nnz_of(A) + nnz_of(B));
+  int w = 0;
+  for (i = 0; i < m; i++) {
+    T.row[i] = w;
+    int a0 = A->row[i], a1 = A->row[i + 1], base = w;
+    for (j = a0; j < a1; j++) { T.col[w] = A->col[j]; T.val[w] = A->val[j]; w++; }
+    for (k = B->row[i]; k < B->row[i + 1]; k++) {
+      int found = -1;
+      for (j = a0; j < a1; j++) if (B->col[k] == A->col[j]) { found = j; break; }
+      if (found >= 0) {
+        double t = B->val[k] + A->val[found];
+        double test = (A->val[found] != 0.0) ? fabs(t) / fabs(A->val[found]) : fabs(t) / fabs(B->val[k]);
+        T.val[base + (found - a0)] = (test > 0.0) ? t : 0.0;
+      } else {
+        T.col[w] = B->col[k];
+        T.val[w] = B->val[k];
+        w++;
+      }
+    }
+  }
+  T.row[m] = w;
+  return csr_compact(&T);
+}
+/* RMATxCVEC spOpTools.f90:404-427 */
+static void rmat_x_cvec(const o_csr *A, const zdouble *x, zdouble *y) {
+  for (int i = 0; i < A->nrow; i++) {
+    zdouble s = 0;
+    for (int j = A->row[i]; j < A->row[i + 1]; j++) s = s + A->val[j] * x[A->col[j]];
+    y[i] = s;
+  }
+}
+/* RMATxRVEC spOpTools.f90:429-451 */
+static void rmat_x_rvec(const o_csr *A, const double *x, double *y) {
+  for (int i = 0; i < A->nrow; i++) {
+    double s = 0.0;
+    for (int j = A->row[i]; j < A->row[i + 1]; j++) s = s + A->val[j] * x[A->col[j]];
+    y[i] = s;
+  }
+}
+
+/* --------------------------------------------------------------- problem */
+struct sp2o_problem {
+  o_grid g;
+  int nXe, nYe, nZe, nE, nNode;
+  int nInt, nBdry, nNi, nNb;
+  int *EDGEi, *EDGEb, *NODEi, *NODEb;
+  int nxInt, nyInt, nzInt; /* interior edges per component */
+  double *FaceA, *EdgeL, *DualFaceA, *DualEdgeL, *Vnode, *Vedge;
+  double *sigEdge, *sigNode; /* as formed by UpdateCond (boundary edges zeroed) */
+  double *logcond, airLog;
+  o_csr T, G, CCii, CCib, AAii, GDii;
+  double *vms; /* VomegaMuSig */
+  double omega;
+  o_csrz L, U;
+  int have_lu;
+  o_csr VDiv, VDs, Gai, VDsG, VDsG_L, VDsG_U;
+  int have_divcor;
+};
+
+/* spOpTopology_SG.f90:24-171 setCurlTopology; faces x 4 edges */
+static o_csr curl_topology(const o_grid *g) {
+  int nXf = ncomp(g, XFACE), nYf = ncomp(g, YFACE), nZf = ncomp(g, ZFACE);
+  int nXe = ncomp(g, XEDGE), nYe = ncomp(g, YEDGE);
+  int m = nXf + nYf + nZf, n = nXe + nYe + ncomp(g, ZEDGE);
+  o_csr T = csr_new(m, n, 4 * m);
+  for (int r = 0; r <= m; r++) T.row[r] = 4 * r;
+  int a, b, c, i, j, k, r;
+  limits(g, XFACE, &a, &b, &c);
+  for (k = 0; k < c; k++) for (j = 0; j < b; j++) for (i = 0; i < a; i++) {
+    r = (k * b + j) * a + i;
+    T.col[4 * r + 0] = vidx(g, YEDGE, i, j, k) + nXe;       T.val[4 * r + 0] = 1;
+    T.col[4 * r + 2] = vidx(g, ZEDGE, i, j, k) + nXe + nYe; T.val[4 * r + 2] = -1;
+    T.col[4 * r + 1] = vidx(g, YEDGE, i, j, k + 1) + nXe;   T.val[4 * r + 1] = -1;
+    T.col[4 * r + 3] = vidx(g, ZEDGE, i, j + 1, k) + nXe + nYe; T.val[4 * r + 3] = 1;
+  }
+  limits(g, YFACE, &a, &b, &c);
+  for (k = 0; k < c; k++) for (j = 0; j < b; j++) for (i = 0; i < a; i++) {
+    r = nXf + (k * b + j) * a + i;
+    T.col[4 * r + 0] = vidx(g, XEDGE, i, j, k);             T.val[4 * r + 0] = -1;
+    T.col[4 * r + 2] = vidx(g, ZEDGE, i, j, k) + nXe + nYe; T.val[4 * r + 2] = 1;
+    T.col[4 * r + 1] = vidx(g, XEDGE, i, j, k + 1);         T.val[4 * r + 1] = 1;
+    T.col[4 * r + 3] = vidx(g, ZEDGE, i + 1, j, k) + nXe + nYe; T.val[4 * r + 3] = -1;
+  }
+  limits(g, ZFACE, &a, &b, &c);
+  for (k = 0; k < c; k++) for (j = 0; j < b; j++) for (i = 0; i < a; i++) {
+    r = nXf + nYf + (k * b + j) * a + i;
+    T.col[4 * r + 0] = vidx(g, XEDGE, i, j, k);             T.val[4 * r + 0] = 1;
+    T.col[4 * r + 2] = vidx(g, YEDGE, i, j, k) + nXe;       T.val[4 * r + 2] = -1;
+    T.col[4 * r + 1] = vidx(g, XEDGE, i, j + 1, k);         T.val[4 * r + 1] = -1;
+    T.col[4 * r + 3] = vidx(g, YEDGE, i + 1, j, k) + nXe;   T.val[4 * r + 3] = 1;
+  }
+  return T;
+}
+
+/* spOpTopology_SG.f90:173-266 setGradTopology; edges x 2 nodes (-1, +1) */
+static o_csr grad_topology(const o_grid *g) {
+  int nXe = ncomp(g, XEDGE), nYe = ncomp(g, YEDGE), nZe = ncomp(g, ZEDGE);
+  int m = nXe + nYe + nZe, n = ncomp(g, CORNER);
+  o_csr G = csr_new(m, n, 2 * m);
+  int a, b, c, i, j, k, r;
+  for (r = 0; r <= m; r++) G.row[r] = 2 * r;
+  for (r = 0; r < m; r++) { G.val[2 * r] = -1; G.val[2 * r + 1] = 1; }
+  limits(g, XEDGE, &a, &b, &c);
+  for (k = 0; k < c; k++) for (j = 0; j < b; j++) for (i = 0; i < a; i++) {
+    r = (k * b + j) * a + i;
+    G.col[2 * r] = vidx(g, CORNER, i, j, k);
+    G.col[2 * r + 1] = vidx(g, CORNER, i + 1, j, k);
+  }
+  limits(g, YEDGE, &a, &b, &c);
+  for (k = 0; k < c; k++) for (j = 0; j < b; j++) for (i = 0; i < a; i++) {
+    r = nXe + (k * b + j) * a + i;
+    G.col[2 * r] = vidx(g, CORNER, i, j, k);
+    G.col[2 * r + 1] = vidx(g, CORNER, i, j + 1, k);
+  }
+  limits(g, ZEDGE, &a, &b, &c);
+  for (k = 0; k < c; k++) for (j = 0; j < b; j++) for (i = 0; i < a; i++) {
+    r = nXe + nYe + (k * b + j) * a + i;
+    G.col[2 * r] = vidx(g, CORNER, i, j, k);
+    G.col[2 * r + 1] = vidx(g, CORNER, i, j, k + 1);
+  }
+  return G;
+}
+
+/* vecTranslate.f90:436-527 boundaryIndex (EDGE and CORNER) */
+static void boundary_index(sp2o_problem *p) {
+  const o_grid *g = &p->g;
+  int nx = g->nx, ny = g->ny, nz = g->nz, i, j, k, t = 0;
+  char *flag = XALLOC(char, p->nE);
+  for (k = 0; k <= nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i < nx; i++)
+    flag[t++] = (j == 0 || j == ny || k == 0 || k == nz);
+  for (k = 0; k <= nz; k++) for (j = 0; j < ny; j++) for (i = 0; i <= nx; i++)
+    flag[t++] = (i == 0 || i == nx || k == 0 || k == nz);
+  for (k = 0; k < nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i <= nx; i++)
+    flag[t++] = (i == 0 || i == nx || j == 0 || j == ny);
+  int nb = 0;
+  for (i = 0; i < p->nE; i++) nb += flag[i];
+  p->nBdry = nb;
+  p->nInt = p->nE - nb;
+  p->EDGEi = XALLOC(int, p->nInt);
+  p->EDGEb = XALLOC(int, p->nBdry);
+  int ib = 0, ii = 0;
+  for (i = 0; i < p->nE; i++) { if (flag[i]) p->EDGEb[ib++] = i; else p->EDGEi[ii++] = i; }
+  free(flag);
+  flag = XALLOC(char, p->nNode);
+  t = 0;
+  for (k = 0; k <= nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i <= nx; i++)
+    flag[t++] = (i == 0 || i == nx || j == 0 || j == ny || k == 0 || k == nz);
+  nb = 0;
+  for (i = 0; i < p->nNode; i++) nb += flag[i];
+  p->nNb = nb;
+  p->nNi = p->nNode - nb;
+  p->NODEi = XALLOC(int, p->nNi);
+  p->NODEb = XALLOC(int, p->nNb);
+  ib = ii = 0;
+  for (i = 0; i < p->nNode; i++) { if (flag[i]) p->NODEb[ib++] = i; else p->NODEi[ii++] = i; }
+  free(flag);
+  p->nxInt = nx * (ny - 1) * (nz - 1);
+  p->nyInt = (nx - 1) * ny * (nz - 1);
+  p->nzInt = (nx - 1) * (ny - 1) * nz;
+}
+
+/* GridCalc.f90:230-418 EdgeLength, FaceLength, EdgeArea, FaceArea;
+ * :152-186 NodeVolume; :44-90 EdgeVolume = EdgeLength * EdgeArea */
+static void metrics(sp2o_problem *p) {
+  const o_grid *g = &p->g;
+  int nx = g->nx, ny = g->ny, nz = g->nz, i, j, k, t;
+  int nF = ncomp(g, XFACE) + ncomp(g, YFACE) + ncomp(g, ZFACE);
+  p->FaceA = XALLOC(double, nF);
+  p->DualEdgeL = XALLOC(double, nF);
+  p->EdgeL = XALLOC(double, p->nE);
+  p->DualFaceA = XALLOC(double, p->nE);
+  p->Vedge = XALLOC(double, p->nE);
+  p->Vnode = XALLOC(double, p->nNode);
+  /* edges */
+  t = 0;
+  for (k = 0; k <= nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i < nx; i++, t++) {
+    p->EdgeL[t] = g->dx[i]; p->DualFaceA[t] = g->delY[j] * g->delZ[k];
+  }
+  for (k = 0; k <= nz; k++) for (j = 0; j < ny; j++) for (i = 0; i <= nx; i++, t++) {
+    p->EdgeL[t] = g->dy[j]; p->DualFaceA[t] = g->delX[i] * g->delZ[k];
+  }
+  for (k = 0; k < nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i <= nx; i++, t++) {
+    p->EdgeL[t] = g->dz[k]; p->DualFaceA[t] = g->delX[i] * g->delY[j];
+  }
+  for (t = 0; t < p->nE; t++) p->Vedge[t] = p->EdgeL[t] * p->DualFaceA[t];
+  /* faces; FaceLength uses (d(i-1)+d(i))/2.0d0 with half-cells at the ends */
+  t = 0;
+  for (k = 0; k < nz; k++) for (j = 0; j < ny; j++) for (i = 0; i <= nx; i++, t++) {
+    double l = (i == 0) ? g->dx[0] : (i == nx ? g->dx[nx - 1] : g->dx[i - 1] + g->dx[i]);
+    p->DualEdgeL[t] = l / 2.0;
+    p->FaceA[t] = g->dy[j] * g->dz[k];
+  }
+  for (k = 0; k < nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i < nx; i++, t++) {
+    double l = (j == 0) ? g->dy[0] : (j == ny ? g->dy[ny - 1] : g->dy[j - 1] + g->dy[j]);
+    p->DualEdgeL[t] = l / 2.0;
+    p->FaceA[t] = g->dx[i] * g->dz[k];
+  }
+  for (k = 0; k <= nz; k++) for (j = 0; j < ny; j++) for (i = 0; i < nx; i++, t++) {
+    double l = (k == 0) ? g->dz[0] : (k == nz ? g->dz[nz - 1] : g->dz[k - 1] + g->dz[k]);
+    p->DualEdgeL[t] = l / 2.0;
+    p->FaceA[t] = g->dx[i] * g->dy[j];
+  }
+  t = 0;
+  for (k = 0; k <= nz; k++) for (j = 0; j <= ny; j++) for (i = 0; i <= nx; i++, t++)
+    p->Vnode[t] = g->delX[i] * g->delY[j] * g->delZ[k];
+}
+
+/* ----------------------------------------------- cell <-> edge/node maps */
+#define CI(i, j, k) ((((k) * g->ny) + (j)) * g->nx + (i))
+/* GridCalc.f90:430-590 Cell2Edge (1-based loops restated 0-based) */
+static void cell2edge(const o_grid *g, const double *C, double *E) {
+  int nx = g->nx, ny = g->ny, nz = g->nz, ix, iy, iz;
+  int nXe = nx * (ny + 1) * (nz + 1), nYe = (nx + 1) * ny * (nz + 1);
+  double *Ex = E, *Ey = E + nXe, *Ez = E + nXe + nYe;
+#define EX(i, j, k) Ex[((k) * (ny + 1) + (j)) * nx + (i)]
+#define EY(i, j, k) Ey[((k) * ny + (j)) * (nx + 1) + (i)]
+#define EZ(i, j, k) Ez[((k) * (ny + 1) + (j)) * (nx + 1) + (i)]
+#define C1(i, j, k) C[CI((i) - 1, (j) - 1, (k) - 1)]
+  memset(E, 0, sizeof(double) * (nXe + nYe + (nx + 1) * (ny + 1) * nz));
+  for (ix = 1; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++)
+    EX(ix - 1, iy - 1, iz - 1) = (C1(ix, iy - 1, iz - 1) + C1(ix, iy, iz - 1) + C1(ix, iy - 1, iz) + C1(ix, iy, iz)) / 4.0;
+  for (ix = 2; ix <= nx; ix++) for (iy = 1; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++)
+    EY(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy, iz - 1) + C1(ix, iy, iz - 1) + C1(ix - 1, iy, iz) + C1(ix, iy, iz)) / 4.0;
+  for (ix = 2; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) for (iz = 1; iz <= nz; iz++)
+    EZ(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy - 1, iz) + C1(ix - 1, iy, iz) + C1(ix, iy - 1, iz) + C1(ix, iy, iz)) / 4.0;
+  iz = 1;
+  for (iy = 1; iy <= ny; iy++) for (ix = 2; ix <= nx; ix++) EY(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy, iz) + C1(ix, iy, iz)) / 2.0;
+  for (ix = 1; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) EX(ix - 1, iy - 1, iz - 1) = (C1(ix, iy - 1, iz) + C1(ix, iy, iz)) / 2.0;
+  iz = nz + 1;
+  for (iy = 1; iy <= ny; iy++) for (ix = 2; ix <= nx; ix++) EY(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy, iz - 1) + C1(ix, iy, iz - 1)) / 2.0;
+  for (ix = 1; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) EX(ix - 1, iy - 1, iz - 1) = (C1(ix, iy - 1, iz - 1) + C1(ix, iy, iz - 1)) / 2.0;
+  ix = 1;
+  for (iy = 1; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) EY(ix - 1, iy - 1, iz - 1) = (C1(ix, iy, iz - 1) + C1(ix, iy, iz)) / 2.0;
+  for (iz = 1; iz <= nz; iz++) for (iy = 2; iy <= ny; iy++) EZ(ix - 1, iy - 1, iz - 1) = (C1(ix, iy - 1, iz) + C1(ix, iy, iz)) / 2.0;
+  ix = nx + 1;
+  for (iy = 1; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) EY(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy, iz - 1) + C1(ix - 1, iy, iz)) / 2.0;
+  for (iz = 1; iz <= nz; iz++) for (iy = 2; iy <= ny; iy++) EZ(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy - 1, iz) + C1(ix - 1, iy, iz)) / 2.0;
+  iy = 1;
+  for (ix = 1; ix <= nx; ix++) for (iz = 2; iz <= nz; iz++) EX(ix - 1, iy - 1, iz - 1) = (C1(ix, iy, iz - 1) + C1(ix, iy, iz)) / 2.0;
+  for (iz = 1; iz <= nz; iz++) for (ix = 2; ix <= nx; ix++) EZ(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy, iz) + C1(ix, iy, iz)) / 2.0;
+  iy = ny + 1;
+  for (ix = 1; ix <= nx; ix++) for (iz = 2; iz <= nz; iz++) EX(ix - 1, iy - 1, iz - 1) = (C1(ix, iy - 1, iz - 1) + C1(ix, iy - 1, iz)) / 2.0;
+  for (iz = 1; iz <= nz; iz++) for (ix = 2; ix <= nx; ix++) EZ(ix - 1, iy - 1, iz - 1) = (C1(ix - 1, iy - 1, iz) + C1(ix, iy - 1, iz)) / 2.0;
+  for (iz = 1; iz <= nz; iz++) {
+    EZ(0, 0, iz - 1) = C1(1, 1, iz);
+    EZ(0, ny, iz - 1) = C1(1, ny, iz);
+    EZ(nx, 0, iz - 1) = C1(nx, 1, iz);
+    EZ(nx, ny, iz - 1) = C1(nx, ny, iz);
+  }
+  for (iy = 1; iy <= ny; iy++) {
+    EY(0, iy - 1, 0) = C1(1, iy, 1);
+    EY(0, iy - 1, nz) = C1(1, iy, nz);
+    EY(nx, iy - 1, 0) = C1(nx, iy, 1);
+    EY(nx, iy - 1, nz) = C1(nx, iy, nz);
+  }
+  for (ix = 1; ix <= nx; ix++) {
+    EX(ix - 1, 0, 0) = C1(ix, 1, 1);
+    EX(ix - 1, 0, nz) = C1(ix, 1, nz);
+    EX(ix - 1, ny, 0) = C1(ix, ny, 1);
+    EX(ix - 1, ny, nz) = C1(ix, ny, nz);
+  }
+}
+
+/* GridCalc.f90:1047-1226 Edge2Cell: the scatter transpose of Cell2Edge */
+static void edge2cell(const o_grid *g, const double *E, double *C) {
+  int nx = g->nx, ny = g->ny, nz = g->nz, ix, iy, iz;
+  int nXe = nx * (ny + 1) * (nz + 1), nYe = (nx + 1) * ny * (nz + 1);
+  const double *Ex = E, *Ey = E + nXe, *Ez = E + nXe + nYe;
+  memset(C, 0, sizeof(double) * nx * ny * nz);
+#define CA(i, j, k, v) C1(i, j, k) = C1(i, j, k) + (v)
+  for (ix = 1; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) {
+    double e = EX(ix - 1, iy - 1, iz - 1);
+    CA(ix, iy - 1, iz - 1, e / 4.0); CA(ix, iy, iz - 1, e / 4.0); CA(ix, iy - 1, iz, e / 4.0); CA(ix, iy, iz, e / 4.0);
+  }
+  for (ix = 2; ix <= nx; ix++) for (iy = 1; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) {
+    double e = EY(ix - 1, iy - 1, iz - 1);
+    CA(ix - 1, iy, iz - 1, e / 4.0); CA(ix, iy, iz - 1, e / 4.0); CA(ix - 1, iy, iz, e / 4.0); CA(ix, iy, iz, e / 4.0);
+  }
+  for (ix = 2; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) for (iz = 1; iz <= nz; iz++) {
+    double e = EZ(ix - 1, iy - 1, iz - 1);
+    CA(ix - 1, iy - 1, iz, e / 4.0); CA(ix - 1, iy, iz, e / 4.0); CA(ix, iy - 1, iz, e / 4.0); CA(ix, iy, iz, e / 4.0);
+  }
+  iz = 1;
+  for (iy = 1; iy <= ny; iy++) for (ix = 2; ix <= nx; ix++) { double e = EY(ix - 1, iy - 1, iz - 1); CA(ix - 1, iy, iz, e / 2.0); CA(ix, iy, iz, e / 2.0); }
+  for (ix = 1; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) { double e = EX(ix - 1, iy - 1, iz - 1); CA(ix, iy - 1, iz, e / 2.0); CA(ix, iy, iz, e / 2.0); }
+  iz = nz + 1;
+  for (iy = 1; iy <= ny; iy++) for (ix = 2; ix <= nx; ix++) { double e = EY(ix - 1, iy - 1, iz - 1); CA(ix - 1, iy, iz - 1, e / 2.0); CA(ix, iy, iz - 1, e / 2.0); }
+  for (ix = 1; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) { double e = EX(ix - 1, iy - 1, iz - 1); CA(ix, iy - 1, iz - 1, e / 2.0); CA(ix, iy, iz - 1, e / 2.0); }
+  ix = 1;
+  for (iy = 1; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) { double e = EY(ix - 1, iy - 1, iz - 1); CA(ix, iy, iz - 1, e / 2.0); CA(ix, iy, iz, e / 2.0); }
+  for (iz = 1; iz <= nz; iz++) for (iy = 2; iy <= ny; iy++) { double e = EZ(ix - 1, iy - 1, iz - 1); CA(ix, iy - 1, iz, e / 2.0); CA(ix, iy, iz, e / 2.0); }
+  ix = nx + 1;
+  for (iy = 1; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) { double e = EY(ix - 1, iy - 1, iz - 1); CA(ix - 1, iy, iz - 1, e / 2.0); CA(ix - 1, iy, iz, e / 2.0); }
+  for (iz = 1; iz <= nz; iz++) for (iy = 2; iy <= ny; iy++) { double e = EZ(ix - 1, iy - 1, iz - 1); CA(ix - 1, iy - 1, iz, e / 2.0); CA(ix - 1, iy, iz, e / 2.0); }
+  iy = 1;
+  for (ix = 1; ix <= nx; ix++) for (iz = 2; iz <= nz; iz++) { double e = EX(ix - 1, iy - 1, iz - 1); CA(ix, iy, iz - 1, e / 2.0); CA(ix, iy, iz, e / 2.0); }
+  for (iz = 1; iz <= nz; iz++) for (ix = 2; ix <= nx; ix++) { double e = EZ(ix - 1, iy - 1, iz - 1); CA(ix - 1, iy, iz, e / 2.0); CA(ix, iy, iz, e / 2.0); }
+  iy = ny + 1;
+  for (ix = 1; ix <= nx; ix++) for (iz = 2; iz <= nz; iz++) { double e = EX(ix - 1, iy - 1, iz - 1); CA(ix, iy - 1, iz - 1, e / 2.0); CA(ix, iy - 1, iz, e / 2.0); }
+  for (iz = 1; iz <= nz; iz++) for (ix = 2; ix <= nx; ix++) { double e = EZ(ix - 1, iy - 1, iz - 1); CA(ix - 1, iy - 1, iz, e / 2.0); CA(ix, iy - 1, iz, e / 2.0); }
+  for (iz = 1; iz <= nz; iz++) {
+    CA(1, 1, iz, EZ(0, 0, iz - 1));
+    CA(1, ny, iz, EZ(0, ny, iz - 1));
+    CA(nx, 1, iz, EZ(nx, 0, iz - 1));
+    CA(nx, ny, iz, EZ(nx, ny, iz - 1));
+  }
+  for (iy = 1; iy <= ny; iy++) {
+    CA(1, iy, 1, EY(0, iy - 1, 0));
+    CA(1, iy, nz, EY(0, iy - 1, nz));
+    CA(nx, iy, 1, EY(nx, iy - 1, 0));
+    CA(nx, iy, nz, EY(nx, iy - 1, nz));
+  }
+  for (ix = 1; ix <= nx; ix++) {
+    CA(ix, 1, 1, EX(ix - 1, 0, 0));
+    CA(ix, 1, nz, EX(ix - 1, 0, nz));
+    CA(ix, ny, 1, EX(ix - 1, ny, 0));
+    CA(ix, ny, nz, EX(ix - 1, ny, nz));
+  }
+#undef CA
+}
+
+/* GridCalc.f90:864-1043 Cell2Node (accumulated term by term, /8, /4, /2) */
+static void cell2node(const o_grid *g, const double *C, double *N) {
+  int nx = g->nx, ny = g->ny, nz = g->nz, ix, iy, iz;
+#define NV(i, j, k) N[(((k) - 1) * (ny + 1) + ((j) - 1)) * (nx + 1) + ((i) - 1)]
+#define NA(i, j, k, v) NV(i, j, k) = NV(i, j, k) + (v)
+  memset(N, 0, sizeof(double) * (nx + 1) * (ny + 1) * (nz + 1));
+  for (ix = 2; ix <= nx; ix++) for (iy = 2; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) {
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 8.0); NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 8.0);
+    NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 8.0);     NA(ix, iy, iz, C1(ix - 1, iy, iz) / 8.0);
+    NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 8.0);     NA(ix, iy, iz, C1(ix, iy - 1, iz) / 8.0);
+    NA(ix, iy, iz, C1(ix, iy, iz - 1) / 8.0);         NA(ix, iy, iz, C1(ix, iy, iz) / 8.0);
+  }
+  iy = 1;
+  for (ix = 2; ix <= nx; ix++) for (iz = 2; iz <= nz; iz++) {
+    NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 4.0); NA(ix, iy, iz, C1(ix, iy, iz - 1) / 4.0);
+    NA(ix, iy, iz, C1(ix - 1, iy, iz) / 4.0);     NA(ix, iy, iz, C1(ix, iy, iz) / 4.0);
+  }
+  iy = ny + 1;
+  for (ix = 2; ix <= nx; ix++) for (iz = 2; iz <= nz; iz++) {
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 4.0); NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 4.0);
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 4.0);     NA(ix, iy, iz, C1(ix, iy - 1, iz) / 4.0);
+  }
+  ix = 1;
+  for (iy = 2; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) {
+    NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 4.0); NA(ix, iy, iz, C1(ix, iy, iz - 1) / 4.0);
+    NA(ix, iy, iz, C1(ix, iy - 1, iz) / 4.0);     NA(ix, iy, iz, C1(ix, iy, iz) / 4.0);
+  }
+  ix = nx + 1;
+  for (iy = 2; iy <= ny; iy++) for (iz = 2; iz <= nz; iz++) {
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 4.0); NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 4.0);
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 4.0);     NA(ix, iy, iz, C1(ix - 1, iy, iz) / 4.0);
+  }
+  iz = 1;
+  for (iy = 2; iy <= ny; iy++) for (ix = 2; ix <= nx; ix++) {
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 4.0); NA(ix, iy, iz, C1(ix, iy - 1, iz) / 4.0);
+    NA(ix, iy, iz, C1(ix - 1, iy, iz) / 4.0);     NA(ix, iy, iz, C1(ix, iy, iz) / 4.0);
+  }
+  iz = nz + 1;
+  for (iy = 2; iy <= ny; iy++) for (ix = 2; ix <= nx; ix++) {
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 4.0); NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 4.0);
+    NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 4.0);     NA(ix, iy, iz, C1(ix, iy, iz - 1) / 4.0);
+  }
+  for (ix = 2; ix <= nx; ix++) {
+    iy = 1; iz = 1;
+    NA(ix, iy, iz, C1(ix - 1, iy, iz) / 2.0); NA(ix, iy, iz, C1(ix, iy, iz) / 2.0);
+    iz = nz + 1;
+    NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix, iy, iz - 1) / 2.0);
+    iy = ny + 1; iz = 1;
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 2.0); NA(ix, iy, iz, C1(ix, iy - 1, iz) / 2.0);
+    iz = nz + 1;
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 2.0);
+  }
+  for (iy = 2; iy <= ny; iy++) {
+    ix = 1; iz = 1;
+    NA(ix, iy, iz, C1(ix, iy - 1, iz) / 2.0); NA(ix, iy, iz, C1(ix, iy, iz) / 2.0);
+    iz = nz + 1;
+    NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix, iy, iz - 1) / 2.0);
+    ix = nx + 1; iz = 1;
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 2.0); NA(ix, iy, iz, C1(ix - 1, iy, iz) / 2.0);
+    iz = nz + 1;
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 2.0);
+  }
+  for (iz = 2; iz <= nz; iz++) {
+    ix = 1; iy = 1;
+    NA(ix, iy, iz, C1(ix, iy, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix, iy, iz) / 2.0);
+    iy = ny + 1;
+    NA(ix, iy, iz, C1(ix, iy - 1, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix, iy - 1, iz) / 2.0);
+    ix = nx + 1; iy = 1;
+    NA(ix, iy, iz, C1(ix - 1, iy, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix - 1, iy, iz) / 2.0);
+    iy = ny + 1;
+    NA(ix, iy, iz, C1(ix - 1, iy - 1, iz - 1) / 2.0); NA(ix, iy, iz, C1(ix - 1, iy - 1, iz) / 2.0);
+  }
+  NV(1, 1, 1) = C1(1, 1, 1);
+  NV(nx + 1, 1, 1) = C1(nx, 1, 1);
+  NV(1, ny + 1, 1) = C1(1, ny, 1);
+  NV(nx + 1, ny + 1, 1) = C1(nx, ny, 1);
+  NV(1, 1, nz + 1) = C1(1, 1, nz);
+  NV(nx + 1, 1, nz + 1) = C1(nx, 1, nz);
+  NV(1, ny + 1, nz + 1) = C1(1, ny, nz);
+  NV(nx + 1, ny + 1, nz + 1) = C1(nx, ny, nz);
+#undef NA
+#undef NV
+}
+
+/* full-grid cell conductivity from LOGE parameters (ModelMap.inc:108-120) */
+static double *cell_sigma(const sp2o_problem *p, const double *logcond) {
+  const o_grid *g = &p->g;
+  double *c = XALLOC(double, g->nx * g->ny * g->nz);
+  double air = exp(p->airLog);
+  for (int k = 0; k < g->nz; k++)
+    for (int j = 0; j < g->ny; j++)
+      for (int i = 0; i < g->nx; i++)
+        c[CI(i, j, k)] = (k < g->nzAir) ? air
+                          : exp(logcond[((k - g->nzAir) * g->ny + j) * g->nx + i]);
+  return c;
+}
+static double *cell_volume(const o_grid *g) {
+  double *v = XALLOC(double, g->nx * g->ny * g->nz);
+  for (int k = 0; k < g->nz; k++) for (int j = 0; j < g->ny; j++) for (int i = 0; i < g->nx; i++)
+    v[CI(i, j, k)] = g->dx[i] * g->dy[j] * g->dz[k];
+  return v;
+}
+/* ModelParamToEdge ModelMap.inc:88-185: Econd = Cell2Edge(V_C*sigma)/Cell2Edge(V_C) */
+static void model_to_edge(const sp2o_problem *p, const double *logcond, double *E) {
+  const o_grid *g = &p->g;
+  int nc = g->nx * g->ny * g->nz;
+  double *s = cell_sigma(p, logcond), *vc = cell_volume(g), *ct = XALLOC(double, nc);
+  double *ghost = XALLOC(double, p->nE), *et = XALLOC(double, p->nE);
+  cell2edge(g, vc, ghost);
+  for (int i = 0; i < nc; i++) ct[i] = vc[i] * s[i];
+  cell2edge(g, ct, et);
+  for (int i = 0; i < p->nE; i++) E[i] = et[i] / ghost[i];
+  free(s); free(vc); free(ct); free(ghost); free(et);
+}
+/* ModelParamToNode ModelMap.inc:187-277 */
+static void model_to_node(const sp2o_problem *p, const double *logcond, double *N) {
+  const o_grid *g = &p->g;
+  int nc = g->nx * g->ny * g->nz;
+  double *s = cell_sigma(p, logcond), *vc = cell_volume(g), *ct = XALLOC(double, nc);
+  double *ghost = XALLOC(double, p->nNode), *nt = XALLOC(double, p->nNode);
+  cell2node(g, vc, ghost);
+  for (int i = 0; i < nc; i++) ct[i] = vc[i] * s[i];
+  cell2node(g, ct, nt);
+  for (int i = 0; i < p->nNode; i++) N[i] = nt[i] / ghost[i];
+  free(s); free(vc); free(ct); free(ghost); free(nt);
+}
+#undef C1
+#undef EX
+#undef EY
+#undef EZ
+
+/* ------------------------------------------------------ operator set-up */
+/* modelOperator3D.f90:371-405 CurlCurlSetUp */
+static void curlcurl_setup(sp2o_problem *p) {
+  int nF = p->T.nrow;
+  double *Dt = XALLOC(double, nF);
+  o_csr Temp = rmat_x_diag(&p->T, p->EdgeL);
+  for (int i = 0; i < nF; i++) Dt[i] = p->DualEdgeL[i] / p->FaceA[i];
+  o_csr CC = diag_x_rmat(Dt, &Temp);
+  csr_free(&Temp);
+  o_csr Tt = rmat_trans(&p->T);
+  Temp = rmat_x_rmat(&Tt, &CC);
+  csr_free(&CC);
+  CC = diag_x_rmat(p->EdgeL, &Temp);
+  csr_free(&Temp);
+  csr_free(&Tt);
+  p->CCii = submatrix(&CC, p->EDGEi, p->nInt, p->EDGEi, p->nInt);
+  p->CCib = submatrix(&CC, p->EDGEi, p->nInt, p->EDGEb, p->nBdry);
+  csr_free(&CC);
+  free(Dt);
+}
+
+/* modelOperator3D.f90:1086-1130 airNIndex */
+static void air_nindex(const sp2o_problem *p, const double *sigEdge, double *Nair, double *Nearth) {
+  o_csr Dt = rmat_trans(&p->G);
+  int j, i;
+  for (j = 0; j < nnz_of(&Dt); j++) Dt.val[j] = fabs(Dt.val[j]);
+  double *Eearth = XALLOC(double, p->nE);
+  const double thr = (double)1.1f * O_SIGMA_AIR; /* default-real 1.1 in the reference */
+  for (i = 0; i < p->nE; i++) {
+    double ea = 0.0, ee = 0.0;
+    if (sigEdge[i] < thr) ea = 1.0; else ee = 1.0;
+    if (sigEdge[i] == 0.0) { ea = 0.0; ee = 0.0; }
+    (void)ea;
+    Eearth[i] = ee;
+  }
+  rmat_x_rvec(&Dt, Eearth, Nair);
+  for (i = 0; i < p->nNode; i++) {
+    Nearth[i] = 0.0;
+    if (Nair[i] > 0.0) { Nearth[i] = 1.0; Nair[i] = 0.0; } else { Nair[i] = 1.0; }
+  }
+  for (i = 0; i < p->nNb; i++) { Nearth[p->NODEb[i]] = 0.0; Nair[p->NODEb[i]] = 0.0; }
+  free(Eearth);
+  csr_free(&Dt);
+}
+
+/* modelOperator3D.f90:916-1008 GradDivSetup2 */
+static void graddiv_setup2(sp2o_problem *p) {
+  int Ne = p->nE, Nn = p->nNode, i;
+  double *M1air = XALLOC(double, Ne), *M2air = XALLOC(double, Nn);
+  double *M0e = XALLOC(double, Ne), *M1e = XALLOC(double, Ne), *M2e = XALLOC(double, Nn);
+  double *M3e = XALLOC(double, Ne), *M4e = XALLOC(double, Ne);
+  double *Nair = XALLOC(double, Nn), *Nearth = XALLOC(double, Nn);
+  air_nindex(p, p->sigEdge, Nair, Nearth);
+  for (i = 0; i < Ne; i++) {
+    M1air[i] = p->DualFaceA[i];
+    M0e[i] = p->DualFaceA[i];
+    M1e[i] = p->sigEdge[i] * p->DualFaceA[i];
+    M3e[i] = 1.0 / p->EdgeL[i];
+    M4e[i] = p->DualFaceA[i];
+  }
+  for (i = 0; i < Nn; i++) {
+    M2air[i] = Nair[i] / p->Vnode[i];
+    M2e[i] = 1.0 / p->sigNode[i] * Nearth[i] / p->Vnode[i];
+  }
+  o_csr Dt = rmat_trans(&p->G);
+  /* air part */
+  o_csr GD = diag_x_rmat(M1air, &p->G);
+  o_csr GDa = rmat_x_diag(&GD, M2air);
+  csr_free(&GD);
+  GD = rmat_x_rmat(&GDa, &Dt);
+  csr_free(&GDa);
+  GDa = rmat_x_diag(&GD, M1air);
+  csr_free(&GD);
+  /* earth part */
+  GD = diag_x_rmat(M0e, &p->G);
+  o_csr GDe = rmat_x_diag(&GD, M2e);
+  csr_free(&GD);
+  GD = rmat_x_rmat(&GDe, &Dt);
+  csr_free(&GDe);
+  GDe = rmat_x_diag(&GD, M1e);
+  csr_free(&GD);
+  GD = rmat_plus_rmat(&GDa, &GDe);
+  csr_free(&GDa);
+  csr_free(&GDe);
+  o_csr GDii = submatrix(&GD, p->EDGEi, p->nInt, p->EDGEi, p->nInt);
+  csr_free(&GD);
+  p->AAii = rmat_plus_rmat(&p->CCii, &GDii);
+  csr_free(&GDii);
+  /* GD for the RHS correction terms */
+  GDe = diag_x_rmat(M3e, &p->G);
+  GD = rmat_x_diag(&GDe, M2e);
+  csr_free(&GDe);
+  GDe = rmat_x_rmat(&GD, &Dt);
+  csr_free(&GD);
+  GD = rmat_x_diag(&GDe, M4e);
+  csr_free(&GDe);
+  p->GDii = submatrix(&GD, p->EDGEi, p->nInt, p->EDGEi, p->nInt);
+  csr_free(&GD);
+  csr_free(&Dt);
+  free(M1air); free(M2air); free(M0e); free(M1e); free(M2e); free(M3e); free(M4e);
+  free(Nair); free(Nearth);
+}
+
+sp2o_problem *sp2o_create(int nx, int ny, int nzAir, int nzEarth, const double *dx,
+                          const double *dy, const double *dz, const double *logcond,
+                          double airLogCond) {
+  sp2o_problem *p = XALLOC(sp2o_problem, 1);
+  o_grid *g = &p->g;
+  g->nx = nx; g->ny = ny; g->nzAir = nzAir; g->nzEarth = nzEarth; g->nz = nzAir + nzEarth;
+  g->dx = XALLOC(double, nx); memcpy(g->dx, dx, sizeof(double) * nx);
+  g->dy = XALLOC(double, ny); memcpy(g->dy, dy, sizeof(double) * ny);
+  g->dz = XALLOC(double, g->nz); memcpy(g->dz, dz, sizeof(double) * g->nz);
+  grid_setup(g);
+  p->nXe = ncomp(g, XEDGE); p->nYe = ncomp(g, YEDGE); p->nZe = ncomp(g, ZEDGE);
+  p->nE = p->nXe + p->nYe + p->nZe;
+  p->nNode = ncomp(g, CORNER);
+  p->logcond = XALLOC(double, nx * ny * nzEarth);
+  memcpy(p->logcond, logcond, sizeof(double) * nx * ny * nzEarth);
+  p->airLog = airLogCond;
+  /* ModelDataInit modelOperator3D.f90:69-117 */
+  p->T = curl_topology(g);
+  p->G = grad_topology(g);
+  boundary_index(p);
+  metrics(p);
+  curlcurl_setup(p);
+  /* UpdateCond modelOperator3D.f90:167-213 */
+  p->sigEdge = XALLOC(double, p->nE);
+  p->sigNode = XALLOC(double, p->nNode);
+  model_to_edge(p, logcond, p->sigEdge);
+  model_to_node(p, logcond, p->sigNode);
+  for (int i = 0; i < p->nBdry; i++) p->sigEdge[p->EDGEb[i]] = 0.0;
+  graddiv_setup2(p);
+  p->omega = 1.0;
+  p->vms = XALLOC(double, p->nInt);
+  for (int i = 0; i < p->nInt; i++)
+    p->vms[i] = O_MU0 * p->omega * p->sigEdge[p->EDGEi[i]] * p->Vedge[p->EDGEi[i]];
+  return p;
+}
+
+void sp2o_destroy(sp2o_problem *p) {
+  if (!p) return;
+  free(p->g.dx); free(p->g.dy); free(p->g.dz); free(p->g.delX); free(p->g.delY); free(p->g.delZ);
+  free(p->EDGEi); free(p->EDGEb); free(p->NODEi); free(p->NODEb);
+  free(p->FaceA); free(p->EdgeL); free(p->DualFaceA); free(p->DualEdgeL); free(p->Vnode); free(p->Vedge);
+  free(p->sigEdge); free(p->sigNode); free(p->logcond); free(p->vms);
+  csr_free(&p->T); csr_free(&p->G); csr_free(&p->CCii); csr_free(&p->CCib);
+  csr_free(&p->AAii); csr_free(&p->GDii);
+  if (p->have_lu) { csrz_free(&p->L); csrz_free(&p->U); }
+  if (p->have_divcor) {
+    csr_free(&p->VDiv); csr_free(&p->VDs); csr_free(&p->Gai);
+    csr_free(&p->VDsG); csr_free(&p->VDsG_L); csr_free(&p->VDsG_U);
+  }
+  free(p);
+}
+
+void sp2o_sizes(const sp2o_problem *p, int *o) {
+  o[0] = p->nE; o[1] = p->nInt; o[2] = p->nBdry; o[3] = p->nNode; o[4] = p->nNi; o[5] = p->nNb;
+  o[6] = p->nxInt; o[7] = p->nyInt; o[8] = p->nzInt;
+}
+void sp2o_index(const sp2o_problem *p, int *ei, int *eb, int *ni, int *nb) {
+  memcpy(ei, p->EDGEi, sizeof(int) * p->nInt);
+  memcpy(eb, p->EDGEb, sizeof(int) * p->nBdry);
+  memcpy(ni, p->NODEi, sizeof(int) * p->nNi);
+  memcpy(nb, p->NODEb, sizeof(int) * p->nNb);
+}
+void sp2o_metric(const sp2o_problem *p, double *faceA, double *edgeL, double *dualFaceA,
+                 double *dualEdgeL, double *vnode, double *vedge) {
+  int nF = p->T.nrow;
+  memcpy(faceA, p->FaceA, sizeof(double) * nF);
+  memcpy(edgeL, p->EdgeL, sizeof(double) * p->nE);
+  memcpy(dualFaceA, p->DualFaceA, sizeof(double) * p->nE);
+  memcpy(dualEdgeL, p->DualEdgeL, sizeof(double) * nF);
+  memcpy(vnode, p->Vnode, sizeof(double) * p->nNode);
+  memcpy(vedge, p->Vedge, sizeof(double) * p->nE);
+}
+void sp2o_sigma(const sp2o_problem *p, double *sE, double *sN) {
+  /* ModelParamToEdge itself, before UpdateCond zeroes the boundary */
+  model_to_edge(p, p->logcond, sE);
+  memcpy(sN, p->sigNode, sizeof(double) * p->nNode);
+}
+void sp2o_model2edge(const sp2o_problem *p, const double *logcond, double *sE) {
+  model_to_edge(p, logcond, sE);
+}
+
+static const o_csr *pick(const sp2o_problem *p, int which) {
+  switch (which) {
+    case 0: return &p->AAii;
+    case 1: return &p->CCii;
+    case 2: return &p->CCib;
+    case 3: return &p->GDii;
+    case 4: return &p->T;
+    case 5: return &p->G;
+    case 6: return p->have_divcor ? &p->VDiv : NULL;
+    case 7: return p->have_divcor ? &p->VDs : NULL;
+    case 8: return p->have_divcor ? &p->Gai : NULL;
+    case 9: return p->have_divcor ? &p->VDsG : NULL;
+    case 10: return p->have_divcor ? &p->VDsG_L : NULL;
+    case 11: return p->have_divcor ? &p->VDsG_U : NULL;
+  }
+  return NULL;
+}
+int sp2o_csr_nnz(const sp2o_problem *p, int which, int *nrow, int *ncol) {
+  const o_csr *A = pick(p, which);
+  if (!A) return -1;
+  *nrow = A->nrow;
+  *ncol = A->ncol;
+  return nnz_of(A);
+}
+void sp2o_csr_get(const sp2o_problem *p, int which, int *row, int *col, double *val) {
+  const o_csr *A = pick(p, which);
+  if (!A) return;
+  memcpy(row, A->row, sizeof(int) * (A->nrow + 1));
+  memcpy(col, A->col, sizeof(int) * nnz_of(A));
+  memcpy(val, A->val, sizeof(double) * nnz_of(A));
+}
+
+/* --------------------------------------------------- block DILU set-up */
+/* CSR_R2Cdiag spOpTools.f90:1809-1825: B = A + ISIGN*i*diag(d) */
+static o_csrz csr_r2cdiag(const o_csr *A, const double *d) {
+  o_csrz B = csrz_new(A->nrow, A->ncol, nnz_of(A));
+  memcpy(B.row, A->row, sizeof(int) * (A->nrow + 1));
+  memcpy(B.col, A->col, sizeof(int) * nnz_of(A));
+  for (int j = 0; j < nnz_of(A); j++) B.val[j] = A->val[j];
+  for (int i = 0; i < A->nrow; i++)
+    for (int j = B.row[i]; j < B.row[i + 1]; j++)
+      if (B.col[j] == i) { B.val[j] = B.val[j] + (O_ISIGN * I) * d[i]; break; }
+  return B;
+}
+/* lowerTri_Cmplx / upperTri_Cmplx spOpTools.f90:1188-1282 (diagonal kept) */
+static o_csrz tri_part(const o_csrz *A, int lower) {
+  int m = A->nrow, i, j, nz = 0;
+  for (i = 0; i < m; i++)
+    for (j = A->row[i]; j < A->row[i + 1]; j++)
+      nz += lower ? (A->col[j] <= i) : (A->col[j] >= i);
+  o_csrz B = csrz_new(m, A->ncol, nz);
+  int k = 0;
+  for (i = 0; i < m; i++) {
+    B.row[i] = k;
+    for (j = A->row[i]; j < A->row[i + 1]; j++)
+      if (lower ? (A->col[j] <= i) : (A->col[j] >= i)) { B.col[k] = A->col[j]; B.val[k] = A->val[j]; k++; }
+  }
+  B.row[m] = k;
+  return B;
+}
+/* CMATtrans spOpTools.f90:879-915 (conjugate = .true.) */
+static o_csrz cmat_trans(const o_csrz *A) {
+  int nz = A->row[A->nrow], i, j;
+  o_csrz B = csrz_new(A->ncol, A->nrow, nz);
+  int *cnt = XALLOC(int, A->ncol + 1);
+  for (j = 0; j < nz; j++) cnt[A->col[j]]++;
+  for (i = 0; i < A->ncol; i++) B.row[i + 1] = B.row[i] + cnt[i];
+  memset(cnt, 0, sizeof(int) * (A->ncol + 1));
+  for (i = 0; i < A->nrow; i++)
+    for (j = A->row[i]; j < A->row[i + 1]; j++) {
+      int r = A->col[j], pos = B.row[r] + cnt[r]++;
+      B.col[pos] = i;
+      B.val[pos] = conj(A->val[j]);
+    }
+  free(cnt);
+  return B;
+}
+/* Dilu_Cmplx_AS spOpTools.f90:1996-2055 */
+static void dilu_cmplx_as(const o_csrz *A, o_csrz *L, o_csrz *U) {
+  *L = tri_part(A, 1);
+  *U = tri_part(A, 0);
+  o_csrz AT = cmat_trans(A);
+  int m = A->nrow, i, j, k;
+  zdouble *d = XALLOC(zdouble, m);
+  for (i = 0; i < m; i++) {
+    for (j = A->row[i]; j < A->row[i + 1]; j++) {
+      if (A->col[j] == i) {
+        d[i] = d[i] + A->val[j];
+      } else if (A->col[j] < i) {
+        for (k = AT.row[i]; k < AT.row[i + 1]; k++)
+          if (AT.col[k] == A->col[j]) d[i] = d[i] - A->val[j] * AT.val[k] * d[A->col[j]];
+      }
+    }
+    d[i] = zdiv(1.0, d[i]);
+  }
+  csrz_free(&AT);
+  for (i = 0; i < m; i++)
+    for (j = L->row[i]; j < L->row[i + 1]; j++)
+      L->val[j] = (L->col[j] == i) ? (zdouble)1.0 : L->val[j] * d[L->col[j]];
+  for (i = 0; i < m; i++)
+    for (j = U->row[i]; j < U->row[i + 1]; j++)
+      if (U->col[j] == i) { U->val[j] = zdiv(1.0, d[i]); break; }
+  free(d);
+}
+/* BlkDiag_Cmplx spOpTools.f90:1474-1520 */
+static o_csrz blkdiag(o_csrz *B, int nb) {
+  int m = 0, n = 0, nz = 0, b, i, j;
+  for (b = 0; b < nb; b++) { m += B[b].nrow; n += B[b].ncol; nz += B[b].row[B[b].nrow]; }
+  o_csrz C = csrz_new(m, n, nz);
+  int r0 = 0, c0 = 0, z0 = 0;
+  for (b = 0; b < nb; b++) {
+    for (i = 0; i < B[b].nrow; i++) C.row[r0 + i] = B[b].row[i] + z0;
+    for (j = 0; j < B[b].row[B[b].nrow]; j++) { C.col[z0 + j] = B[b].col[j] + c0; C.val[z0 + j] = B[b].val[j]; }
+    r0 += B[b].nrow; c0 += B[b].ncol; z0 += B[b].row[B[b].nrow];
+  }
+  C.row[m] = z0;
+  return C;
+}
+/* PC_setup modelOperator3D.f90:512-611 */
+static void pc_setup(sp2o_problem *p) {
+  int starts[4] = {0, p->nxInt, p->nxInt + p->nyInt, p->nInt};
+  o_csrz Lb[3], Ub[3];
+  for (int b = 0; b < 3; b++) {
+    int n = starts[b + 1] - starts[b];
+    int *ix = XALLOC(int, n);
+    double *d = XALLOC(double, n);
+    for (int i = 0; i < n; i++) { ix[i] = starts[b] + i; d[i] = p->vms[starts[b] + i]; }
+    o_csr Cxx = submatrix(&p->AAii, ix, n, ix, n);
+    o_csrz Axx = csr_r2cdiag(&Cxx, d);
+    dilu_cmplx_as(&Axx, &Lb[b], &Ub[b]);
+    csr_free(&Cxx);
+    csrz_free(&Axx);
+    free(ix);
+    free(d);
+  }
+  if (p->have_lu) { csrz_free(&p->L); csrz_free(&p->U); }
+  p->L = blkdiag(Lb, 3);
+  p->U = blkdiag(Ub, 3);
+  p->have_lu = 1;
+  for (int b = 0; b < 3; b++) { csrz_free(&Lb[b]); csrz_free(&Ub[b]); }
+}
+
+/* UpdateFreq modelOperator3D.f90:155-164 -> updateOmegaMuSig (:236-282) */
+void sp2o_update_freq(sp2o_problem *p, double omega) {
+  for (int i = 0; i < p->nInt; i++) {
+    if (p->omega > 0) p->vms[i] = p->vms[i] / p->omega;
+    p->vms[i] = p->vms[i] * omega;
+  }
+  p->omega = omega;
+  pc_setup(p);
+}
+void sp2o_vms(const sp2o_problem *p, double *v) { memcpy(v, p->vms, sizeof(double) * p->nInt); }
+int sp2o_lu_nnz(const sp2o_problem *p, int which) {
+  if (!p->have_lu) return -1;
+  return which ? p->U.row[p->U.nrow] : p->L.row[p->L.nrow];
+}
+void sp2o_lu_get(const sp2o_problem *p, int which, int *row, int *col, zdouble *val) {
+  const o_csrz *A = which ? &p->U : &p->L;
+  memcpy(row, A->row, sizeof(int) * (A->nrow + 1));
+  memcpy(col, A->col, sizeof(int) * A->row[A->nrow]);
+  memcpy(val, A->val, sizeof(zdouble) * A->row[A->nrow]);
+}
+
+/* -------------------------------------------------------- operator apply */
+/* Mult_Aii modelOperator3D.f90:415-436 */
+void sp2o_mult_aii(const sp2o_problem *p, int adjt, const zdouble *x, zdouble *y) {
+  if (adjt) {
+    o_csr At = rmat_trans(&p->AAii);
+    rmat_x_cvec(&At, x, y);
+    csr_free(&At);
+    for (int i = 0; i < p->nInt; i++) y[i] = y[i] - (I * O_ISIGN) * p->vms[i] * x[i];
+  } else {
+    rmat_x_cvec(&p->AAii, x, y);
+    for (int i = 0; i < p->nInt; i++) y[i] = y[i] + (I * O_ISIGN) * p->vms[i] * x[i];
+  }
+}
+/* LTsolve_Cmplx / UTsolve_Cmplx spOpTools.f90:1664-1730 */
+static void ltsolve_z(const o_csrz *L, const zdouble *b, zdouble *x) {
+  for (int i = 0; i < L->nrow; i++) {
+    zdouble s = b[i], d = 1.0;
+    for (int j = L->row[i]; j < L->row[i + 1]; j++) {
+      if (L->col[j] < i) s = s - L->val[j] * x[L->col[j]];
+      else d = L->val[j];
+    }
+    x[i] = zdiv(s, d);
+  }
+}
+static void utsolve_z(const o_csrz *U, const zdouble *b, zdouble *x) {
+  for (int i = U->nrow - 1; i >= 0; i--) {
+    zdouble s = b[i], d = 1.0;
+    for (int j = U->row[i]; j < U->row[i + 1]; j++) {
+      if (U->col[j] > i) s = s - U->val[j] * x[U->col[j]];
+      else d = U->val[j];
+    }
+    x[i] = zdiv(s, d);
+  }
+}
+void sp2o_pc_lsolve(const sp2o_problem *p, const zdouble *x, zdouble *y) { ltsolve_z(&p->L, x, y); }
+void sp2o_pc_usolve(const sp2o_problem *p, const zdouble *x, zdouble *y) { utsolve_z(&p->U, x, y); }
+
+/* ------------------------------------------------------------- BiCGStab */
+static zdouble zdot(int n, const zdouble *a, const zdouble *b) { /* Fortran dot_product */
+  zdouble s = 0;
+  for (int i = 0; i < n; i++) s = s + conj(a[i]) * b[i];
+  return s;
+}
+static double znrm(int n, const zdouble *a) { return sqrt(creal(zdot(n, a, a))); }
+
+static void precond(const sp2o_problem *p, int kind, const zdouble *r, zdouble *tmp, zdouble *z) {
+  int n = p->nInt;
+  if (kind == 0) {
+    ltsolve_z(&p->L, r, tmp);
+    utsolve_z(&p->U, tmp, z);
+  } else if (kind == 1) {
+    for (int i = 0; i < n; i++) {
+      double a = 0.0;
+      for (int j = p->AAii.row[i]; j < p->AAii.row[i + 1]; j++)
+        if (p->AAii.col[j] == i) a = p->AAii.val[j];
+      z[i] = zdiv(r[i], a + (O_ISIGN * I) * p->vms[i]);
+    }
+  } else {
+    memcpy(z, r, sizeof(zdouble) * n);
+  }
+}
+
+/* BICG solver.f90:720-926 (gpu=0) and cuBiCG solver.f90:931-2009 (gpu=1) */
+int sp2o_bicg(const sp2o_problem *p, const zdouble *b, zdouble *x, int maxit, double tol,
+              int gpu, int pc, int *niter, double *rerr) {
+  int n = p->nInt, iter, i, status = 1;
+  for (i = 0; i < maxit; i++) rerr[i] = 0.0;
+  double bnorm = znrm(n, b);
+  if (isnan(bnorm)) { *niter = 0; return 3; }
+  if (bnorm == 0.0) { for (i = 0; i < n; i++) x[i] = 0; *niter = 1; return 0; }
+  zdouble *R = XALLOC(zdouble, n), *RT = XALLOC(zdouble, n), *P = XALLOC(zdouble, n);
+  zdouble *PT = XALLOC(zdouble, n), *PH = XALLOC(zdouble, n), *S = XALLOC(zdouble, n);
+  zdouble *SH = XALLOC(zdouble, n), *V = XALLOC(zdouble, n), *T = XALLOC(zdouble, n);
+  zdouble *X0 = XALLOC(zdouble, n);
+  sp2o_mult_aii(p, 0, x, R);
+  for (i = 0; i < n; i++) R[i] = b[i] - R[i];
+  double rnorm = znrm(n, R), btol = tol * bnorm, rnorm0 = rnorm;
+  rerr[0] = (float)(rnorm / bnorm);
+  if (rnorm <= btol) { *niter = 1; status = 0; goto done; }
+  memcpy(X0, x, sizeof(zdouble) * n);
+  zdouble RHO = 1.0, RHO1 = 1.0, ALPHA = 1.0, OMEGA = 1.0, BETA;
+  if (!gpu) memcpy(RT, R, sizeof(zdouble) * n);
+  *niter = maxit;
+  for (iter = 1; iter <= maxit; iter++) {
+    int restart = gpu ? ((iter % 120) == 1) : (iter == 1);
+    if (gpu && restart) {
+      memcpy(RT, R, sizeof(zdouble) * n);
+      RHO1 = 1.0;
+      RHO = zdot(n, RT, R);
+      memcpy(P, R, sizeof(zdouble) * n);
+      OMEGA = 1.0;
+    } else {
+      RHO1 = RHO;
+      RHO = zdot(n, RT, R);
+      if (!gpu && RHO == 0.0) { status = 2; break; }
+      if (iter == 1) {
+        memcpy(P, R, sizeof(zdouble) * n);
+      } else {
+        BETA = zdiv(RHO, RHO1) * zdiv(ALPHA, OMEGA);
+        if (BETA == 0.0) { status = 2; break; }
+        for (i = 0; i < n; i++) P[i] = R[i] + BETA * (P[i] - OMEGA * V[i]);
+      }
+    }
+    precond(p, pc, P, PT, PH);
+    sp2o_mult_aii(p, 0, PH, V);
+    zdouble RTV = zdot(n, RT, V);
+    if (!gpu && RTV == 0.0) { status = 2; break; }
+    ALPHA = zdiv(RHO, RTV);
+    if (ALPHA == 0.0) { status = 2; break; }
+    for (i = 0; i < n; i++) x[i] = x[i] + ALPHA * PH[i];
+    for (i = 0; i < n; i++) S[i] = R[i] - ALPHA * V[i];
+    precond(p, pc, S, PT, SH);
+    sp2o_mult_aii(p, 0, SH, T);
+    zdouble TT = zdot(n, T, T);
+    if (!gpu && TT == 0.0) { status = 2; break; }
+    OMEGA = zdiv(zdot(n, T, S), TT);
+    if (OMEGA == 0.0) { status = 2; break; }
+    for (i = 0; i < n; i++) x[i] = x[i] + OMEGA * SH[i];
+    for (i = 0; i < n; i++) R[i] = S[i] - OMEGA * T[i];
+    rnorm = znrm(n, R);
+    /* the CPU solver stores real(rnorm/bnorm), a default (single) real */
+    rerr[iter - 1] = gpu ? rnorm / bnorm : (float)(rnorm / bnorm);
+    if (rnorm < btol) {
+      if (!gpu) { *niter = iter; status = 0; break; }
+      /* cuBiCG re-checks with the true residual and keeps it (solver.f90:1855-1895) */
+      sp2o_mult_aii(p, 0, x, R);
+      for (i = 0; i < n; i++) R[i] = b[i] - R[i];
+      rnorm = znrm(n, R);
+      if (rnorm < btol) { rerr[iter - 1] = (float)(rnorm / bnorm); *niter = iter; status = 0; break; }
+    }
+    if (rnorm < rnorm0) { rnorm0 = rnorm; if (gpu) memcpy(X0, x, sizeof(zdouble) * n); }
+  }
+  if (gpu && status != 0) memcpy(x, X0, sizeof(zdouble) * n); /* best iterate */
+  if (status == 2) *niter = maxit;
+done:
+  free(R); free(RT); free(P); free(PT); free(PH); free(S); free(SH); free(V); free(T); free(X0);
+  return status;
+}
+
+/* ------------------------------------------------- divergence correction */
+/* lowerTri_Real/upperTri_Real + Dilu_Real spOpTools.f90:1092-1186, 1903-1948 */
+static o_csr tri_real(const o_csr *A, int lower) {
+  int m = A->nrow, i, j, nz = 0;
+  for (i = 0; i < m; i++) for (j = A->row[i]; j < A->row[i + 1]; j++) nz += lower ? (A->col[j] <= i) : (A->col[j] >= i);
+  o_csr B = csr_new(m, A->ncol, nz);
+  int k = 0;
+  for (i = 0; i < m; i++) {
+    B.row[i] = k;
+    for (j = A->row[i]; j < A->row[i + 1]; j++)
+      if (lower ? (A->col[j] <= i) : (A->col[j] >= i)) { B.col[k] = A->col[j]; B.val[k] = A->val[j]; k++; }
+  }
+  B.row[m] = k;
+  return B;
+}
+static void dilu_real(const o_csr *A, o_csr *L, o_csr *U) {
+  *L = tri_real(A, 1);
+  *U = tri_real(A, 0);
+  int m = A->nrow, i, j;
+  double *d = XALLOC(double, m);
+  for (i = 0; i < m; i++) {
+    d[i] = 0.0;
+    for (j = A->row[i]; j < A->row[i + 1]; j++) {
+      if (A->col[j] == i) d[i] = d[i] + A->val[j];
+      else if (A->col[j] < i) d[i] = d[i] - A->val[j] * A->val[j] * d[A->col[j]];
+    }
+    d[i] = 1.0 / d[i];
+  }
+  for (i = 0; i < m; i++)
+    for (j = L->row[i]; j < L->row[i + 1]; j++)
+      L->val[j] = (L->col[j] == i) ? 1.0 : L->val[j] * d[L->col[j]];
+  for (i = 0; i < m; i++)
+    for (j = U->row[i]; j < U->row[i + 1]; j++)
+      if (U->col[j] == i) { U->val[j] = 1.0 / d[i]; break; }
+  free(d);
+}
+/* DivCorInit + DivCorSetup modelOperator3D.f90:648-730 */
+void sp2o_divcor_setup(sp2o_problem *p) {
+  if (p->have_divcor) {
+    csr_free(&p->VDiv); csr_free(&p->VDs); csr_free(&p->Gai);
+    csr_free(&p->VDsG); csr_free(&p->VDsG_L); csr_free(&p->VDsG_U);
+  }
+  int i;
+  o_csr Temp = rmat_trans(&p->G);
+  o_csr Temp2 = rmat_x_diag(&Temp, p->DualFaceA);
+  p->VDiv = submatrix(&Temp2, p->NODEi, p->nNi, p->EDGEi, p->nInt);
+  csr_free(&Temp); csr_free(&Temp2);
+  double *d = XALLOC(double, p->nE);
+  for (i = 0; i < p->nE; i++) d[i] = 1. / p->EdgeL[i];
+  int *all = XALLOC(int, p->nNode);
+  for (i = 0; i < p->nNode; i++) all[i] = i;
+  Temp = diag_x_rmat(d, &p->G);
+  p->Gai = submatrix(&Temp, p->EDGEi, p->nInt, all, p->nNode);
+  csr_free(&Temp);
+  free(d);
+  free(all);
+  double *dd = XALLOC(double, p->nInt);
+  for (i = 0; i < p->nInt; i++) {
+    dd[i] = p->vms[i] / (O_MU0 * p->omega);
+    dd[i] = -dd[i] / p->Vedge[p->EDGEi[i]];
+  }
+  p->VDs = rmat_x_diag(&p->VDiv, dd);
+  free(dd);
+  int *allr = XALLOC(int, p->Gai.nrow);
+  for (i = 0; i < p->Gai.nrow; i++) allr[i] = i;
+  Temp = submatrix(&p->Gai, allr, p->Gai.nrow, p->NODEi, p->nNi);
+  p->VDsG = rmat_x_rmat(&p->VDs, &Temp);
+  dilu_real(&p->VDsG, &p->VDsG_L, &p->VDsG_U);
+  csr_free(&Temp);
+  free(allr);
+  p->have_divcor = 1;
+}
+/* LTsolve_Real / UTsolve_Real spOpTools.f90:1732-1798 */
+static void ltsolve_r(const o_csr *L, const zdouble *b, zdouble *x) {
+  for (int i = 0; i < L->nrow; i++) {
+    zdouble s = b[i];
+    double d = 1.0;
+    for (int j = L->row[i]; j < L->row[i + 1]; j++) {
+      if (L->col[j] < i) s = s - L->val[j] * x[L->col[j]];
+      else d = L->val[j];
+    }
+    x[i] = zdivr(s, d);
+  }
+}
+static void utsolve_r(const o_csr *U, const zdouble *b, zdouble *x) {
+  for (int i = U->nrow - 1; i >= 0; i--) {
+    zdouble s = b[i];
+    double d = 1.0;
+    for (int j = U->row[i]; j < U->row[i + 1]; j++) {
+      if (U->col[j] > i) s = s - U->val[j] * x[U->col[j]];
+      else d = U->val[j];
+    }
+    x[i] = zdivr(s, d);
+  }
+}
+/* PCG solver.f90:57-136 with A = DivCgrad, Minv = DivCgradILU */
+int sp2o_pcg(const sp2o_problem *p, const zdouble *b, zdouble *x, int maxit, double tol,
+             int *niter, double *rerr) {
+  int n = p->nNi, i, it;
+  zdouble *r = XALLOC(zdouble, n), *s = XALLOC(zdouble, n), *pp = XALLOC(zdouble, n);
+  zdouble *q = XALLOC(zdouble, n), *tmp = XALLOC(zdouble, n);
+  zdouble delta, deltaOld = 1, beta, alpha;
+  for (i = 0; i < maxit; i++) rerr[i] = 0.0;
+  rmat_x_cvec(&p->VDsG, x, r);
+  for (i = 0; i < n; i++) r[i] = b[i] - r[i];
+  double bnorm = znrm(n, b), rnorm = znrm(n, r);
+  it = 1;
+  rerr[0] = (float)(rnorm / bnorm);
+  while (rerr[it - 1] > tol && it < maxit) {
+    ltsolve_r(&p->VDsG_L, r, tmp);
+    utsolve_r(&p->VDsG_U, tmp, s);
+    delta = zdot(n, r, s);
+    if (it == 1) memcpy(pp, s, sizeof(zdouble) * n);
+    else {
+      beta = zdiv(delta, deltaOld);
+      for (i = 0; i < n; i++) pp[i] = s[i] + beta * pp[i];
+    }
+    rmat_x_cvec(&p->VDsG, pp, q);
+    alpha = zdiv(delta, zdot(n, pp, q));
+    for (i = 0; i < n; i++) x[i] = x[i] + pp[i] * alpha;
+    for (i = 0; i < n; i++) r[i] = r[i] - q[i] * alpha;
+    deltaOld = delta;
+    it++;
+    rnorm = znrm(n, r);
+    rerr[it - 1] = (float)(rnorm / bnorm);
+  }
+  *niter = it;
+  free(r); free(s); free(pp); free(q); free(tmp);
+  return rerr[it - 1] > tol;
+}
+void sp2o_divc(const sp2o_problem *p, const zdouble *e, zdouble *phi) { rmat_x_cvec(&p->VDs, e, phi); }
+void sp2o_grad(const sp2o_problem *p, const zdouble *phi, zdouble *e) { rmat_x_cvec(&p->Gai, phi, e); }
+
+/* --------------------------------------------------- RHS / post-process */
+/* FWDsolve3D, forward case with boundary data and no source
+ * (EMsolve3D.f90:261-274, 318): b = -CCib * e(EDGEb) */
+void sp2o_rhs_fwd(const sp2o_problem *p, const zdouble *e_all, zdouble *b) {
+  zdouble *eb = XALLOC(zdouble, p->nBdry);
+  for (int i = 0; i < p->nBdry; i++) eb[i] = e_all[p->EDGEb[i]];
+  rmat_x_cvec(&p->CCib, eb, b);
+  for (int i = 0; i < p->nInt; i++) b[i] = -b[i];
+  free(eb);
+}
+/* FWDsolve3D, transposed case with a dense source (EMsolve3D.f90:217-256):
+ * b = s_i + V_E * GDii (s_i * iOmegaMuInv / V_E) */
+void sp2o_rhs_trn(const sp2o_problem *p, double omega, const zdouble *s_all, zdouble *b) {
+  int n = p->nInt;
+  zdouble iOmegaMuInv = zdiv(1.0, CMPLX(0.0, 1.0 * O_ISIGN * omega * O_MU0));
+  zdouble *si = XALLOC(zdouble, n), *st = XALLOC(zdouble, n);
+  for (int i = 0; i < n; i++) si[i] = s_all[p->EDGEi[i]];
+  for (int i = 0; i < n; i++) si[i] = zdivr(si[i] * iOmegaMuInv, p->Vedge[p->EDGEi[i]]);
+  rmat_x_cvec(&p->GDii, si, st);
+  for (int i = 0; i < n; i++) b[i] = s_all[p->EDGEi[i]] + st[i] * p->Vedge[p->EDGEi[i]];
+  free(si); free(st);
+}
+/* FWDsolve3D post-processing for TRN without boundary data (EMsolve3D.f90:463-478):
+ * e(EDGEi) = ei; s(EDGEb) = CCib^T ei; e = V_E e; e(EDGEb) = -s(EDGEb) */
+void sp2o_post_trn(const sp2o_problem *p, const zdouble *ei, zdouble *e) {
+  o_csr Ct = rmat_trans(&p->CCib);
+  zdouble *yb = XALLOC(zdouble, p->nBdry);
+  rmat_x_cvec(&Ct, ei, yb);
+  for (int i = 0; i < p->nE; i++) e[i] = 0;
+  for (int i = 0; i < p->nInt; i++) e[p->EDGEi[i]] = ei[i];
+  for (int i = 0; i < p->nE; i++) e[i] = p->Vedge[i] * e[i];
+  for (int i = 0; i < p->nBdry; i++) e[p->EDGEb[i]] = -yb[i];
+  free(yb);
+  csr_free(&Ct);
+}
+
+/* EdgeToModelParam ModelMap.inc:302-372: m = V_C * Edge2Cell(e / V_E_ghost) * exp(m0) */
+void sp2o_edge2model(const sp2o_problem *p, const double *e, double *dm) {
+  const o_grid *g = &p->g;
+  int nc = g->nx * g->ny * g->nz;
+  double *vc = cell_volume(g), *ghost = XALLOC(double, p->nE), *et = XALLOC(double, p->nE);
+  double *ct = XALLOC(double, nc);
+  cell2edge(g, vc, ghost);
+  for (int i = 0; i < p->nE; i++) et[i] = e[i] / ghost[i];
+  edge2cell(g, et, ct);
+  for (int i = 0; i < nc; i++) ct[i] = vc[i] * ct[i];
+  int ne = g->nx * g->ny * g->nzEarth, off = g->nx * g->ny * g->nzAir;
+  for (int i = 0; i < ne; i++) dm[i] = ct[off + i] * exp(p->logcond[i]);
+  free(vc); free(ghost); free(et); free(ct);
+}
+/* PmultT SolverSens.f90:63-116 (real part): dm = dEdgeToModelParam(Re(-i*ISIGN*mu0*omega * sum_pol e0.*e)) */
+void sp2o_pmultT(const sp2o_problem *p, double omega, int npol, const zdouble *e0, const zdouble *e,
+                 double *dm) {
+  int n = p->nE;
+  zdouble c = 0.0 + I * (-O_ISIGN * O_MU0 * omega);
+  zdouble *acc = XALLOC(zdouble, n);
+  double *re = XALLOC(double, n);
+  for (int i = 0; i < n; i++) acc[i] = e0[i] * e[i];
+  for (int k = 1; k < npol; k++)
+    for (int i = 0; i < n; i++) acc[i] = acc[i] + e0[(size_t)k * n + i] * e[(size_t)k * n + i];
+  for (int i = 0; i < n; i++) re[i] = creal(c * acc[i]);
+  sp2o_edge2model(p, re, dm);
+  free(acc);
+  free(re);
+}
