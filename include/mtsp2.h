@@ -1,0 +1,202 @@
+/*
+ * mtsp2.h -- C ABI of libmtsp2, the MI355X (gfx950) SP2 3D-MT solver library.
+ *
+ * Drop-in for the GPU path of ModEM-GPU's SP2 forward/adjoint solver.  The
+ * reference's Fortran host (f90/3D_MT/FWD_SP2) reaches its GPU through
+ * ISO_C_BINDING interfaces (f90/3D_MT/FWD_SP2/cudaFortMap.f90:93-1408) inside
+ * cuBiCG (f90/3D_MT/FWD_SP2/solver.f90:931-2009).  This header is what a
+ * Fortran `bind(C)` interface block binds instead; the matching Fortran module
+ * is modem-gpu_amd/fortran/mtsp2_fortran.f90.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Complex numbers are C99 `double _Complex`
+ *    (Fortran `complex(c_double_complex)`), stored interleaved (re, im).
+ *  - Host pointers unless the name ends in `_dev` (device pointers).
+ *  - Interior-edge vectors are in the reference's EDGEi order (x-, y-, then
+ *    z-edges, each Fortran-ordered), length n_int.
+ *  - Every function returns an int status (MT_OK = 0); mt_last_error() gives
+ *    the message.  Nothing in the library calls exit()/stop, unlike the
+ *    reference, whose host code stops on any CUDA error (solver.f90:1140-1143).
+ *  - A context is bound to one device and one stream and is not thread-safe.
+ */
+#ifndef MTSP2_H
+#define MTSP2_H
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> mt_zdouble;
+extern "C" {
+#else
+#include <complex.h>
+typedef double _Complex mt_zdouble;
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define MT_OK 0
+#define MT_E_ARG 1        /* bad argument / size mismatch                     */
+#define MT_E_DEVICE 2     /* HIP runtime error                                 */
+#define MT_E_OOM 3        /* device allocation failed                          */
+#define MT_E_NAN_RHS 4    /* right-hand side contains NaN (solver.f90:1500)    */
+#define MT_E_BREAKDOWN 5  /* BiCGStab breakdown (alpha/beta/omega == 0)        */
+#define MT_E_MAXIT 6      /* not converged within maxit (best iterate returned)*/
+#define MT_E_STATE 7      /* operator / preconditioner not set                 */
+#define MT_E_STENCIL 8    /* CSR does not fit the staggered-grid stencil       */
+
+/* ---- preconditioners and precisions ---------------------------------- */
+#define MT_PC_DILU 0      /* block diagonal-ILU, the reference's PC_setup      */
+#define MT_PC_JACOBI 1
+#define MT_PC_NONE 2
+#define MT_PREC_FP64 0
+#define MT_PREC_MIXED 1   /* FP32 inner BiCGStab + FP64 defect correction      */
+
+typedef struct mt_ctx mt_ctx;
+
+/* Solver controls.  Mirrors solverControl_t (solver.f90:25-36) plus the
+ * options the reference hard-codes (restart interval 120, solver.f90:1545). */
+typedef struct {
+  int maxit;            /* max BiCGStab iterations (total, FP64 or inner)      */
+  double tol;           /* relative residual ||b-Ax||/||b|| target             */
+  int precond;          /* MT_PC_*                                             */
+  int precision;        /* MT_PREC_*                                           */
+  int restart;          /* restart interval; 0 -> 120 (the reference's)        */
+  int best_iterate;     /* 1: return the smallest-residual iterate on failure  */
+  double inner_tol;     /* mixed: inner relative reduction per refinement step */
+  int max_outer;        /* mixed: max refinement steps                         */
+  int op_transpose;     /* 1: solve with A^T (north-star adjoint); 0: A        */
+  int chunk;            /* iterations per host status check / graph replay     */
+  int use_graph;        /* 1: capture the iteration chunk in a hipGraph        */
+} mt_solve_opts;
+
+typedef struct {
+  int niter;            /* total BiCGStab iterations (inner for mixed)         */
+  int nouter;           /* refinement steps (mixed) / true-residual rechecks   */
+  int status;           /* MT_OK, MT_E_BREAKDOWN, MT_E_MAXIT                   */
+  double relres;        /* final FP64 true relative residual                   */
+} mt_sys_stats;
+
+void mt_default_opts(mt_solve_opts *opts);
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the per-call stream/handle creation of cuBiCG (solver.f90:1118-
+ * 1160) and kernelc_hookCtx (kernel_c.cu:341-460): one context, reused. */
+int mt_ctx_create(int device, mt_ctx **ctx);
+int mt_ctx_destroy(mt_ctx *ctx);
+const char *mt_last_error(const mt_ctx *ctx);
+/* Number of visible devices; replaces kernelc_getDevNum (kernel_c.cu:463-500). */
+int mt_device_count(int *count);
+
+/* ---- operator --------------------------------------------------------- */
+/* Grid in cells (nz includes the air layers), as grid_t (GridDef.f90:34-91). */
+int mt_set_grid(mt_ctx *ctx, int nx, int ny, int nz);
+
+/* Uploads AAii (real CSR, n_int x n_int, as built by GradDivSetup2,
+ * modelOperator3D.f90:916-1008) and the frequency-free diagonal
+ * vms1 = mu0*sigma_E*V_E on interior edges (VomegaMuSig / omega,
+ * modelOperator3D.f90:271).  index_base is 1 for the reference's Fortran CSR
+ * (spOpTools.f90:11-28), 0 for C.  Replaces the per-solve H2D copies of
+ * cuBiCG (solver.f90:1236-1263).  The CSR is converted to the device stencil
+ * format; MT_E_STENCIL if an entry does not fit the staggered-grid stencil. */
+int mt_operator_set_csr(mt_ctx *ctx, int n_int, int nnz, const int *rowptr, const int *col,
+                        int index_base, const double *val, const double *vms1);
+
+/* Builds AAii, vms1, V_E, CCib and the RHS-form GDii on the library side from
+ * the grid and the LOGE cell conductivities (nx*ny*nzEarth, Fortran order),
+ * restating ModelDataInit + UpdateCond (modelOperator3D.f90:69-117, 167-213).
+ * Sets the grid too.  The operator is then device-resident. */
+int mt_assemble(mt_ctx *ctx, int nx, int ny, int nzAir, int nzEarth, const double *dx,
+                const double *dy, const double *dz, const double *logcond, double air_logcond);
+
+/* Host-only assembly (no device needed): the same construction as
+ * mt_assemble, returned as interior CSR (0-based, row-sorted) plus vms1.
+ * Call with rowptr == NULL to get *n_int and *nnz, then with arrays. */
+int mt_host_assemble(int nx, int ny, int nzAir, int nzEarth, const double *dx, const double *dy,
+                     const double *dz, const double *logcond, double air_logcond, int *n_int,
+                     int *nnz, int *rowptr, int *col, double *val, double *vms1);
+
+/* Host-only block-DILU set-up (PC_setup, modelOperator3D.f90:512-611): the
+ * inverse pivots 1/p (interior order, complex) of AAii + i*ISIGN*omega*vms1
+ * given as interior CSR on grid (nx, ny, nz cells). */
+int mt_host_dilu_pivots(int nx, int ny, int nz, int n_int, int nnz, const int *rowptr,
+                        const int *col, int index_base, const double *val, const double *vms1,
+                        double omega, mt_zdouble *dinv);
+
+/* Copies of what mt_assemble built (host arrays sized by the caller):
+ * n_int, nnz of AAii as CSR (0-based, row-sorted), and the arrays. */
+int mt_operator_size(const mt_ctx *ctx, int *n_int, int *n_edge, int *nnz);
+int mt_operator_get_csr(mt_ctx *ctx, int *rowptr, int *col, double *val, double *vms1);
+
+/* ---- drop-in solve ------------------------------------------------------ */
+/* cuBiCG(b, x, KSPiter, device_idx) (solver.f90:931-965): right-preconditioned
+ * BiCGStab with conjugated inner products, restart every `restart` iterations,
+ * true-residual recheck at convergence, best iterate returned on failure.
+ * x holds x0 on entry.  rerr (length maxit, may be NULL) receives the relative
+ * residual per iteration (KSPiter%rerr).  failed mirrors KSPiter%failed. */
+int mt_bicg(mt_ctx *ctx, double omega, const mt_zdouble *b, mt_zdouble *x, int maxit,
+            double tol, int precond, int *niter, double *rerr, int *failed);
+
+/* ---- batched solve -------------------------------------------------------- */
+/* nsys systems A(omega_s) x_s = b_s sharing AAii, solved together: both
+ * polarizations and many frequencies in one launch sequence.  b and x are
+ * nsys*n_int, system-major; x holds x0 on entry.  stats may be NULL. */
+int mt_solve_batch(mt_ctx *ctx, int nsys, const double *omegas, const mt_zdouble *b,
+                   mt_zdouble *x, const mt_solve_opts *opts, mt_sys_stats *stats);
+/* Same with device-resident b/x (interior-edge order, system-major). */
+int mt_solve_batch_dev(mt_ctx *ctx, int nsys, const double *omegas, const void *b_dev,
+                       void *x_dev, const mt_solve_opts *opts, mt_sys_stats *stats);
+
+/* ---- operator applications (Mult_Aii, PC_Lsolve/PC_Usolve) -------------- */
+/* y = A(omega) x (Mult_Aii, modelOperator3D.f90:415-436).  op=0: AAii x +
+ * i*ISIGN*omega*vms1 x; op=1: the transpose A^T; op=2: the conjugate transpose
+ * A^H, which is what the reference's Mult_Aii(adjt=.true.) computes. */
+int mt_apply(mt_ctx *ctx, double omega, int op, const mt_zdouble *x, mt_zdouble *y);
+/* y = M^{-1} x for the preconditioner of frequency omega (L then U solves,
+ * modelOperator3D.f90:622-644). */
+int mt_precond_apply(mt_ctx *ctx, double omega, int precond, const mt_zdouble *x,
+                     mt_zdouble *y);
+
+/* ---- right-hand sides, post-processing (FWDsolve3D, EMsolve3D.f90:129-501) */
+/* Forward RHS b = -CCib e_b from boundary values (all-edge vector e_all, only
+ * its boundary entries are read).  Requires mt_assemble. */
+int mt_rhs_fwd(mt_ctx *ctx, const mt_zdouble *e_all, mt_zdouble *b);
+/* Transposed (adjoint) RHS b = s_i + V_E GDii(s_i (i*ISIGN*omega*mu0)^-1 / V_E). */
+int mt_rhs_trn(mt_ctx *ctx, double omega, const mt_zdouble *s_all, mt_zdouble *b);
+/* TRN post-processing: e(EDGEi)=ei, e = V_E e, e(EDGEb) = -CCib^T ei. */
+int mt_post_trn(mt_ctx *ctx, const mt_zdouble *ei, mt_zdouble *e_all);
+
+/* ---- divergence correction (SdivCorr, EMsolve3D.f90:507-613) -------------- */
+/* Node-Laplacian PCG (solver.f90:57-136) on VDsG = VDs*Gai with its DILU
+ * (modelOperator3D.f90:648-763), FP64.  Corrects ei in place:
+ * phi = VDsG^-1 (VDs ei - phi0), ei -= Gai phi.  phi0 (interior nodes) may be
+ * NULL.  niter/relres may be NULL. */
+int mt_divcor(mt_ctx *ctx, double omega, mt_zdouble *ei, const mt_zdouble *phi0, int maxit,
+              double tol, int *niter, double *relres);
+
+/* ---- gradient assembly (PmultT + dEdgeToModelParam) ----------------------- */
+/* dm = Re(-i*ISIGN*mu0*omega * sum_pol e0_pol .* lam_pol) mapped to cells by
+ * dEdgeToModelParam (SolverSens.f90:63-116, ModelMap.inc:281-372): all-edge
+ * complex inputs (npol*n_edge), output nx*ny*nzEarth (Fortran order), added
+ * into dm when accumulate != 0.  Requires mt_assemble. */
+int mt_pmultT(mt_ctx *ctx, double omega, int npol, const mt_zdouble *e0_all,
+              const mt_zdouble *lam_all, double *dm, int accumulate);
+
+/* ---- tuning / introspection ------------------------------------------------ */
+/* Runs the SpMV autotuner for nsys systems: times the solver's SpMV kernel for
+ * several row-tile group counts per system and keeps the fastest; *variant
+ * receives the chosen group count. */
+int mt_autotune(mt_ctx *ctx, int nsys, int *variant);
+/* Fixes the row-tile group count per system (0: automatic). */
+int mt_set_tuning(mt_ctx *ctx, int groups_per_system);
+/* Times the solver's fused SpMV+dot kernel (v = A x, <rt,v>) for nsys systems
+ * at MT_PREC_FP64 (FP64) or MT_PREC_MIXED (FP32) with HIP events on the
+ * context stream: average ms per launch and algorithmic bytes per launch
+ * (nnz*s_v + n*s_d + nsys*3*n*s_c). */
+int mt_bench_spmv(mt_ctx *ctx, int nsys, int precision, int reps, double *ms_per_launch,
+                  double *bytes_per_launch);
+/* Kernel timing of the last mt_solve_* call: count and summed milliseconds of
+ * the dominant SpMV kernel, measured with HIP events on the solver stream. */
+int mt_last_spmv_timing(const mt_ctx *ctx, int *launches, double *ms, double *bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

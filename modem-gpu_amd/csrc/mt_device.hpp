@@ -1,0 +1,224 @@
+// Device kernels of libmtsp2 (gfx950, wave64).
+//
+// Everything here is memory-bound integer-indexed sparse work; there is no
+// GEMM and no MFMA.  Design points:
+//  * vectors are padded to all edges (boundary entries zero) so stencil
+//    addresses are affine and always in bounds (see mt_internal.h);
+//  * batched kernels take a list of active systems and map blocks to
+//    (system, row tile) XCD-aware: the 8 XCDs work on different row tiles and
+//    consecutive blocks on one XCD cover the systems of one tile, so the
+//    shared stencil coefficients are fetched once per XCD L2;
+//  * dot products use wave64 shuffles, then LDS, then a deterministic
+//    last-block reduction over per-block partials (fixed order, no float
+//    atomics), and the same last block updates the solver scalars on the
+//    device: no host round trip per dot (the reference reads every dot back
+//    to the host, solver.f90:1098-1104).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mt {
+
+constexpr double kIsignD = -1.0;  // ISIGN of the reference (math_constants.f90:16)
+
+template <class T>
+struct alignas(2 * sizeof(T)) C2 {
+  T x, y;
+};
+using Z = C2<double>;
+
+template <class T>
+__device__ __forceinline__ C2<T> cadd(C2<T> a, C2<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <class T>
+__device__ __forceinline__ C2<T> csub(C2<T> a, C2<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <class T>
+__device__ __forceinline__ C2<T> cmul(C2<T> a, C2<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// conj(a) * b
+template <class T>
+__device__ __forceinline__ C2<T> cdotm(C2<T> a, C2<T> b) {
+  return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
+}
+template <class T>
+__device__ __forceinline__ C2<T> cscale(T s, C2<T> a) { return {s * a.x, s * a.y}; }
+__host__ __device__ __forceinline__ Z zdiv(Z a, Z b) {
+  double d = b.x * b.x + b.y * b.y;
+  return {(a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d};
+}
+__host__ __device__ __forceinline__ Z zmul(Z a, Z b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <class T, class U>
+__device__ __forceinline__ C2<T> ccast(C2<U> a) { return {T(a.x), T(a.y)}; }
+
+// ---------------------------------------------------------------- operator
+// Stencil operator view.  coef is [15][nE] (slot-major), vms1 is [nE], both
+// zero outside interior rows.  off/base select the neighbour addresses.
+template <class V>
+struct StencilView {
+  const V* coef;
+  const V* vms1;
+  int64_t nE;
+  int64_t nXi, nYi, nInt;
+  int nx, ny, nz;
+  int64_t nXe, nYe;
+  int off[3][15];  // neighbour offset relative to its base, per component/slot
+};
+
+struct RowInfo {
+  int64_t e;   // all-edge index of the row
+  int64_t b1;  // base (same lattice position) in the first other component
+  int64_t b2;  // base in the second other component
+  int c;       // component 0/1/2
+};
+
+template <class V>
+__device__ __forceinline__ RowInfo decode_row(const StencilView<V>& A, int64_t r) {
+  RowInfo ri;
+  int i, j, k;
+  const int nx = A.nx, ny = A.ny;
+  if (r < A.nXi) {
+    ri.c = 0;
+    uint32_t q = uint32_t(r), ai = uint32_t(nx), bi = uint32_t(ny - 1);
+    i = q % ai; q /= ai; j = 1 + q % bi; k = 1 + q / bi;
+  } else if (r < A.nXi + A.nYi) {
+    ri.c = 1;
+    uint32_t q = uint32_t(r - A.nXi), ai = uint32_t(nx - 1), bi = uint32_t(ny);
+    i = 1 + q % ai; q /= ai; j = q % bi; k = 1 + q / bi;
+  } else {
+    ri.c = 2;
+    uint32_t q = uint32_t(r - A.nXi - A.nYi), ai = uint32_t(nx - 1), bi = uint32_t(ny - 1);
+    i = 1 + q % ai; q /= ai; j = 1 + q % bi; k = q / bi;
+  }
+  const int64_t ex = i + int64_t(nx) * (j + int64_t(ny + 1) * k);
+  const int64_t ey = A.nXe + i + int64_t(nx + 1) * (j + int64_t(ny) * k);
+  const int64_t ez = A.nXe + A.nYe + i + int64_t(nx + 1) * (j + int64_t(ny + 1) * k);
+  if (ri.c == 0) { ri.e = ex; ri.b1 = ey; ri.b2 = ez; }
+  else if (ri.c == 1) { ri.e = ey; ri.b1 = ex; ri.b2 = ez; }
+  else { ri.e = ez; ri.b1 = ex; ri.b2 = ey; }
+  return ri;
+}
+
+// y_row = sum_s coef[s][e] x[nb_s]  (real stencil part only)
+template <class V, class T>
+__device__ __forceinline__ C2<T> stencil_row(const StencilView<V>& A, const RowInfo& ri,
+                                             const C2<T>* __restrict__ x) {
+  const int* o = A.off[ri.c];
+  const V* cf = A.coef + ri.e;
+  const int64_t nE = A.nE;
+  C2<T> acc = cscale(T(cf[0]), x[ri.e]);
+#pragma unroll
+  for (int s = 1; s < 7; ++s) {
+    C2<T> xv = x[ri.e + o[s]];
+    T c = T(cf[s * nE]);
+    acc.x += c * xv.x; acc.y += c * xv.y;
+  }
+#pragma unroll
+  for (int s = 7; s < 11; ++s) {
+    C2<T> xv = x[ri.b1 + o[s]];
+    T c = T(cf[s * nE]);
+    acc.x += c * xv.x; acc.y += c * xv.y;
+  }
+#pragma unroll
+  for (int s = 11; s < 15; ++s) {
+    C2<T> xv = x[ri.b2 + o[s]];
+    T c = T(cf[s * nE]);
+    acc.x += c * xv.x; acc.y += c * xv.y;
+  }
+  return acc;
+}
+
+// ---------------------------------------------------------- reductions
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Sums NV complex values over the block; result valid in thread 0.
+template <int NV, int BS>
+__device__ __forceinline__ void block_sum(Z (&v)[NV], Z* lds /* >= NV*BS/64 */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    v[q].x = wave_sum(v[q].x);
+    v[q].y = wave_sum(v[q].y);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) lds[w * NV + q] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      Z s = lds[q];
+      for (int ww = 1; ww < BS / 64; ++ww) s = cadd(s, lds[ww * NV + q]);
+      v[q] = s;
+    }
+  }
+}
+
+// Per-system solver scalars, FP64 whatever the vector precision.
+enum : int { ST_RUN = 0, ST_RECHECK = 1, ST_CONV = 2, ST_BREAK = 3, ST_MAXIT = 4 };
+
+struct SysScal {
+  Z rho, rho1, alpha, omg, beta;  // omg: BiCGStab omega
+  double bnorm, btol, rnorm, rnorm_best, rnorm_start;
+  double freq_omega;              // angular frequency of the system
+  double scale;                   // mixed: residual scaling into FP32
+  int iter, maxit, status, restart_due, best_due, restart, inner, fidx;
+  int tot_inner;                  // mixed (outer record): inner iterations so far
+  int pad[3];
+};
+
+// Block -> (system, tile group) mapping, XCD-aware.  nblk = nact * G with G a
+// multiple of 8.  Block b runs on XCD b % 8; on each XCD consecutive blocks
+// cover all systems of one tile group.
+__device__ __forceinline__ void map_sys_block(int nact, int G, int& si, int& g) {
+  const int b = blockIdx.x;
+  const int xcd = b & 7, w = b >> 3;
+  si = w % nact;
+  g = (w / nact) * 8 + xcd;
+  (void)G;
+}
+
+// Last-block-done: every block stores its partials then bumps the counter;
+// the block that sees count == nblk-1 reduces all partials in block order.
+// The counter is reset by that block, so no memset is ever needed (graph safe).
+__device__ __forceinline__ bool last_block(unsigned* counter, unsigned nblk) {
+  __shared__ bool amLast;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    amLast = (prev == nblk - 1);
+    if (amLast) {
+      __threadfence();
+      atomicExch(counter, 0u);
+    }
+  }
+  __syncthreads();
+  if (amLast) __threadfence();
+  return amLast;
+}
+
+// Reduce partials[(si * G + g) * NV + q] over g for every system, in the last
+// block, in fixed g order (deterministic); result to red[si * NV + q] (LDS).
+template <int NV>
+__device__ void reduce_partials(const Z* partials, int nact, int G, Z* red) {
+  for (int cq = threadIdx.x; cq < nact * NV; cq += blockDim.x) {
+    const int si = cq / NV, q = cq % NV;
+    Z s = {0.0, 0.0};
+    const Z* p = partials + (int64_t)si * G * NV + q;
+    for (int g = 0; g < G; ++g) s = cadd(s, p[(int64_t)g * NV]);
+    red[cq] = s;
+  }
+  __syncthreads();
+}
+
+}  // namespace mt

@@ -1,0 +1,525 @@
+// BiCGStab, operator and preconditioner kernels of libmtsp2.
+//
+// One BiCGStab iteration (right-preconditioned, conjugated inner products:
+// the algorithm of cuBiCG, solver.f90:1548-1902) is 5 fused kernels plus the
+// preconditioner:
+//   k_pupdate     restart ? (rt = r, p = r) : p = r + beta (p - omega v)   [+ Jacobi]
+//   (precond)     ph = M^-1 p
+//   k_spmv_dot<1> v = A ph, <rt,v>      -> last block: alpha = rho / <rt,v>
+//   k_supdate     x += alpha ph, s = r - alpha v  [+ best-iterate copy, Jacobi]
+//   (precond)     sh = M^-1 s
+//   k_spmv_dot<2> t = A sh, <t,t>, <t,s> -> last block: omega = <t,s>/<t,t>
+//   k_rupdate     x += omega sh, r = s - omega t, <rt,r>, <r,r>
+//                                        -> last block: norm, convergence,
+//                                           best iterate, next rho/beta
+//   k_recheck     (only for systems whose recursive residual met tol)
+//                 r = b - A x, true residual check (solver.f90:1855-1895)
+// All kernels take the active-system list and skip finished systems, so a
+// fixed chunk of iterations can be captured once in a hipGraph and replayed.
+#pragma once
+
+#include "mt_device.hpp"
+
+namespace mt {
+
+constexpr int BS = 256;
+enum : int { PCK_DILU = 0, PCK_JACOBI = 1, PCK_NONE = 2 };
+
+template <class T, class V>
+struct IterArgs {
+  StencilView<V> A;
+  SysScal* sc;
+  const int* act;
+  int nact, G;
+  int pc;
+  int64_t nE;
+  C2<T>* x; C2<T>* r; C2<T>* rt; C2<T>* p; C2<T>* ph; C2<T>* v;
+  C2<T>* s; C2<T>* sh; C2<T>* t; C2<T>* xbest;
+  const C2<T>* b;
+  const C2<T>* dinv;     // Jacobi inverse diagonal per frequency [nfreq][nE]
+  Z* partials;
+  unsigned* counter;
+  double* rerr;          // [nsys][rerr_len]
+  int rerr_len;
+};
+
+// i*ISIGN*omega*vms1*x  ==  (ISIGN*omega*vms1) * (i x)
+template <class T>
+__device__ __forceinline__ C2<T> diag_term(T d, C2<T> x) { return {-d * x.y, d * x.x}; }
+
+template <class T, class V>
+__device__ __forceinline__ C2<T> apply_row(const StencilView<V>& A, const RowInfo& ri,
+                                           const C2<T>* __restrict__ x, T dscale) {
+  C2<T> y = stencil_row<V, T>(A, ri, x);
+  const T d = dscale * T(A.vms1[ri.e]);
+  const C2<T> xe = x[ri.e];
+  y.x += -d * xe.y;
+  y.y += d * xe.x;
+  return y;
+}
+
+// ------------------------------------------------------------ scalar logic
+__device__ inline void prep_next(SysScal& sc, Z rt_r, Z r_r) {
+  sc.restart_due = (sc.iter % sc.restart) == 0;
+  if (sc.restart_due) {
+    sc.rho1 = {1.0, 0.0};
+    sc.rho = r_r;
+    sc.omg = {1.0, 0.0};
+  } else {
+    sc.rho1 = sc.rho;
+    sc.rho = rt_r;
+    sc.beta = zmul(zdiv(sc.rho, sc.rho1), zdiv(sc.alpha, sc.omg));
+    if (sc.beta.x == 0.0 && sc.beta.y == 0.0) sc.status = ST_BREAK;
+  }
+}
+
+__device__ inline void finalize_rupdate(SysScal& sc, Z rt_r, Z r_r, double* rerr) {
+  const double rn = sqrt(r_r.x);
+  const int it = ++sc.iter;
+  if (rerr) rerr[it - 1] = rn / sc.bnorm;
+  sc.rnorm = rn;
+  if (rn < sc.btol) {
+    sc.status = sc.inner ? ST_CONV : ST_RECHECK;
+    return;
+  }
+  if (rn < sc.rnorm_best) { sc.rnorm_best = rn; sc.best_due = 1; }
+  if (it >= sc.maxit) { sc.status = ST_MAXIT; return; }
+  prep_next(sc, rt_r, r_r);
+}
+
+__device__ inline void finalize_recheck(SysScal& sc, Z rt_r, Z r_r, double* rerr) {
+  const double rn = sqrt(r_r.x);
+  sc.rnorm = rn;
+  if (rn < sc.btol) {
+    if (rerr) rerr[sc.iter - 1] = rn / sc.bnorm;
+    sc.status = ST_CONV;
+    return;
+  }
+  sc.status = ST_RUN;
+  if (rn < sc.rnorm_best) { sc.rnorm_best = rn; sc.best_due = 1; }
+  if (sc.iter >= sc.maxit) { sc.status = ST_MAXIT; return; }
+  prep_next(sc, rt_r, r_r);
+}
+
+// ------------------------------------------------------------------ kernels
+// r = b - A x, partials <r,r>, <b,b>; the last block sets up each system.
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_init(IterArgs<T, V> a, double tol) {
+  __shared__ Z lds[2 * BS / 64];
+  __shared__ Z red[64 * 2];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const int64_t o = (int64_t)sys * a.nE;
+  const SysScal& sc0 = a.sc[sys];
+  const T dsc = T(kIsignD * sc0.freq_omega);
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  const C2<T>* x = a.x + o;
+  for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+    const RowInfo ri = decode_row(a.A, rr);
+    const C2<T> bv = a.b[o + ri.e];
+    const C2<T> y = csub(bv, apply_row<T, V>(a.A, ri, x, dsc));
+    a.r[o + ri.e] = y;
+    if (a.xbest) a.xbest[o + ri.e] = x[ri.e];
+    acc[0].x += double(y.x) * y.x + double(y.y) * y.y;
+    acc[1].x += double(bv.x) * bv.x + double(bv.y) * bv.y;
+  }
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
+    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
+  }
+  if (!last_block(a.counter, gridDim.x)) return;
+  reduce_partials<2>(a.partials, a.nact, a.G, red);
+  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+    SysScal& sc = a.sc[a.act[q]];
+    const double rn = sqrt(red[q * 2].x), bn = sqrt(red[q * 2 + 1].x);
+    double* rerr = a.rerr ? a.rerr + (int64_t)a.act[q] * a.rerr_len : nullptr;
+    sc.bnorm = bn;
+    sc.btol = tol * bn;
+    sc.rnorm = rn;
+    sc.rnorm_start = rn;
+    sc.rnorm_best = rn;
+    sc.iter = 0;
+    sc.best_due = 0;
+    sc.alpha = {1.0, 0.0};
+    sc.omg = {1.0, 0.0};
+    sc.rho1 = {1.0, 0.0};
+    sc.rho = red[q * 2];
+    sc.restart_due = 1;
+    if (rerr) rerr[0] = (bn > 0.0) ? rn / bn : 0.0;
+    if (isnan(bn)) { sc.status = ST_BREAK; sc.iter = -1; continue; }
+    if (bn == 0.0) { sc.status = ST_CONV; sc.iter = 1; continue; }
+    if (rn <= sc.btol) { sc.status = ST_CONV; sc.iter = 1; continue; }
+    sc.status = ST_RUN;
+  }
+}
+
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_pupdate(IterArgs<T, V> a) {
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc = a.sc[sys];
+  if (sc.status != ST_RUN) return;
+  const int64_t o = (int64_t)sys * a.nE;
+  const bool rs = sc.restart_due;
+  const C2<T> beta = ccast<T>(sc.beta), om = ccast<T>(sc.omg);
+  const C2<T>* dv = (a.pc == PCK_JACOBI) ? a.dinv + (int64_t)sc.fidx * a.nE : nullptr;
+  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+    const C2<T> r = a.r[o + e];
+    C2<T> p;
+    if (rs) {
+      a.rt[o + e] = r;
+      p = r;
+    } else {
+      p = cadd(r, cmul(beta, csub(a.p[o + e], cmul(om, a.v[o + e]))));
+    }
+    a.p[o + e] = p;
+    if (dv) a.ph[o + e] = cmul(dv[e], p);
+    else if (a.pc == PCK_NONE) a.ph[o + e] = p;
+  }
+}
+
+// WHICH=1: v = A ph, <rt,v> -> alpha.   WHICH=2: t = A sh, <t,t>, <t,s> -> omega.
+template <class T, class V, int WHICH>
+__global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
+  __shared__ Z lds[2 * BS / 64];
+  __shared__ Z red[64 * 2];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc0 = a.sc[sys];
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (sc0.status == ST_RUN) {
+    const int64_t o = (int64_t)sys * a.nE;
+    const T dsc = T(kIsignD * sc0.freq_omega);
+    const C2<T>* in = (WHICH == 1 ? a.ph : a.sh) + o;
+    C2<T>* out = (WHICH == 1 ? a.v : a.t) + o;
+    for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+      const RowInfo ri = decode_row(a.A, rr);
+      const C2<T> y = apply_row<T, V>(a.A, ri, in, dsc);
+      out[ri.e] = y;
+      if (WHICH == 1) {
+        const C2<T> w = a.rt[o + ri.e];
+        acc[0].x += double(w.x) * y.x + double(w.y) * y.y;
+        acc[0].y += double(w.x) * y.y - double(w.y) * y.x;
+      } else {
+        const C2<T> w = a.s[o + ri.e];
+        acc[0].x += double(y.x) * y.x + double(y.y) * y.y;
+        acc[1].x += double(y.x) * w.x + double(y.y) * w.y;
+        acc[1].y += double(y.x) * w.y - double(y.y) * w.x;
+      }
+    }
+  }
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
+    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
+  }
+  if (!last_block(a.counter, gridDim.x)) return;
+  reduce_partials<2>(a.partials, a.nact, a.G, red);
+  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+    SysScal& sc = a.sc[a.act[q]];
+    if (sc.status != ST_RUN) continue;
+    if (WHICH == 1) {
+      sc.alpha = zdiv(sc.rho, red[q * 2]);
+      if (sc.alpha.x == 0.0 && sc.alpha.y == 0.0) sc.status = ST_BREAK;
+      if (isnan(sc.alpha.x) || isnan(sc.alpha.y)) sc.status = ST_BREAK;
+    } else {
+      sc.omg = zdiv(red[q * 2 + 1], red[q * 2]);
+      sc.best_due = 0;  // consumed by k_supdate of this iteration
+      if (sc.omg.x == 0.0 && sc.omg.y == 0.0) sc.status = ST_BREAK;
+      if (isnan(sc.omg.x) || isnan(sc.omg.y)) sc.status = ST_BREAK;
+    }
+  }
+}
+
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_supdate(IterArgs<T, V> a) {
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc = a.sc[sys];
+  if (sc.status != ST_RUN) return;
+  const int64_t o = (int64_t)sys * a.nE;
+  const C2<T> al = ccast<T>(sc.alpha);
+  const bool best = sc.best_due && a.xbest;
+  const C2<T>* dv = (a.pc == PCK_JACOBI) ? a.dinv + (int64_t)sc.fidx * a.nE : nullptr;
+  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+    const C2<T> xo = a.x[o + e];
+    if (best) a.xbest[o + e] = xo;
+    a.x[o + e] = cadd(xo, cmul(al, a.ph[o + e]));
+    const C2<T> sv = csub(a.r[o + e], cmul(al, a.v[o + e]));
+    a.s[o + e] = sv;
+    if (dv) a.sh[o + e] = cmul(dv[e], sv);
+    else if (a.pc == PCK_NONE) a.sh[o + e] = sv;
+  }
+}
+
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_rupdate(IterArgs<T, V> a) {
+  __shared__ Z lds[2 * BS / 64];
+  __shared__ Z red[64 * 2];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc0 = a.sc[sys];
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (sc0.status == ST_RUN) {
+    const int64_t o = (int64_t)sys * a.nE;
+    const C2<T> om = ccast<T>(sc0.omg);
+    for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+      a.x[o + e] = cadd(a.x[o + e], cmul(om, a.sh[o + e]));
+      const C2<T> r = csub(a.s[o + e], cmul(om, a.t[o + e]));
+      a.r[o + e] = r;
+      const C2<T> w = a.rt[o + e];
+      acc[0].x += double(w.x) * r.x + double(w.y) * r.y;
+      acc[0].y += double(w.x) * r.y - double(w.y) * r.x;
+      acc[1].x += double(r.x) * r.x + double(r.y) * r.y;
+    }
+  }
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
+    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
+  }
+  if (!last_block(a.counter, gridDim.x)) return;
+  reduce_partials<2>(a.partials, a.nact, a.G, red);
+  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+    SysScal& sc = a.sc[a.act[q]];
+    if (sc.status != ST_RUN) continue;
+    double* rerr = a.rerr ? a.rerr + (int64_t)a.act[q] * a.rerr_len : nullptr;
+    finalize_rupdate(sc, red[q * 2], red[q * 2 + 1], rerr);
+  }
+}
+
+// True-residual recheck for systems in ST_RECHECK (no-op for the others).
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_recheck(IterArgs<T, V> a) {
+  __shared__ Z lds[2 * BS / 64];
+  __shared__ Z red[64 * 2];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc0 = a.sc[sys];
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (sc0.status == ST_RECHECK) {
+    const int64_t o = (int64_t)sys * a.nE;
+    const T dsc = T(kIsignD * sc0.freq_omega);
+    const C2<T>* x = a.x + o;
+    for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+      const RowInfo ri = decode_row(a.A, rr);
+      const C2<T> y = csub(a.b[o + ri.e], apply_row<T, V>(a.A, ri, x, dsc));
+      a.r[o + ri.e] = y;
+      const C2<T> w = a.rt[o + ri.e];
+      acc[0].x += double(w.x) * y.x + double(w.y) * y.y;
+      acc[0].y += double(w.x) * y.y - double(w.y) * y.x;
+      acc[1].x += double(y.x) * y.x + double(y.y) * y.y;
+    }
+  }
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
+    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
+  }
+  if (!last_block(a.counter, gridDim.x)) return;
+  reduce_partials<2>(a.partials, a.nact, a.G, red);
+  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+    SysScal& sc = a.sc[a.act[q]];
+    if (sc.status != ST_RECHECK) continue;
+    double* rerr = a.rerr ? a.rerr + (int64_t)a.act[q] * a.rerr_len : nullptr;
+    finalize_recheck(sc, red[q * 2], red[q * 2 + 1], rerr);
+  }
+}
+
+// ---------------------------------------------------------- plain operator
+// y = A(omega) x for one system (mt_apply, tests, the mixed outer residual).
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_apply(StencilView<V> A, const C2<T>* x, C2<T>* y,
+                                              double omega) {
+  const T dsc = T(kIsignD * omega);
+  for (int64_t rr = (int64_t)blockIdx.x * BS + threadIdx.x; rr < A.nInt;
+       rr += (int64_t)gridDim.x * BS) {
+    const RowInfo ri = decode_row(A, rr);
+    y[ri.e] = apply_row<T, V>(A, ri, x, dsc);
+  }
+}
+
+// --------------------------------------------------- interior <-> padded
+template <class T, class U>
+__global__ void k_scatter(const int* __restrict__ map, int64_t nInt, int64_t nE, int nsys,
+                          const C2<U>* __restrict__ in, C2<T>* __restrict__ out) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nInt * nsys;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = q / nInt, r = q - s * nInt;
+    out[s * nE + map[r]] = ccast<T>(in[q]);
+  }
+}
+template <class T, class U>
+__global__ void k_gather(const int* __restrict__ map, int64_t nInt, int64_t nE, int nsys,
+                         const C2<U>* __restrict__ in, C2<T>* __restrict__ out) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nInt * nsys;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = q / nInt, r = q - s * nInt;
+    out[q] = ccast<T>(in[s * nE + map[r]]);
+  }
+}
+
+// Jacobi: dinv_f[e] = 1 / (a_ee + i*ISIGN*omega_f*vms1_e) on interior rows.
+template <class T, class V>
+__global__ void k_jacobi_dinv(StencilView<V> A, const double* omegas, int nfreq, C2<T>* dinv) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < A.nInt * nfreq;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = q / A.nInt, rr = q - f * A.nInt;
+    const RowInfo ri = decode_row(A, rr);
+    const Z d = {double(A.coef[ri.e]), kIsignD * omegas[f] * double(A.vms1[ri.e])};
+    const Z iv = zdiv({1.0, 0.0}, d);
+    dinv[f * A.nE + ri.e] = {T(iv.x), T(iv.y)};
+  }
+}
+
+}  // namespace mt
+
+namespace mt {
+
+// ------------------------------------------------ mixed precision (outer)
+// FP64 defect correction around the FP32 BiCGStab: the outer record `so`
+// holds the FP64 state of each system, the inner record `si` the FP32 solve
+// of A d = r_scaled.
+struct OuterArgs {
+  StencilView<double> A;
+  SysScal* so;
+  SysScal* si;
+  const int* act;
+  int nact, G;
+  int64_t nE;
+  const Z* b;
+  Z* x;
+  C2<float>* r32;
+  C2<float>* d32;
+  Z* partials;
+  unsigned* counter;
+  double tol, inner_tol;
+  int max_outer, maxit_total;
+  double* rerr;           // outer relative residual history [nsys][rerr_len]
+  int rerr_len;
+};
+
+// <b,b> -> bnorm, btol, FP32 scaling (a power of two near 1/||b||)
+__global__ __launch_bounds__(BS) void k_outer_bnorm(OuterArgs a) {
+  __shared__ Z lds[2 * BS / 64];
+  __shared__ Z red[64 * 2];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const int64_t o = (int64_t)sys * a.nE;
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+    const Z bv = a.b[o + e];
+    acc[0].x += bv.x * bv.x + bv.y * bv.y;
+  }
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
+    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
+  }
+  if (!last_block(a.counter, gridDim.x)) return;
+  reduce_partials<2>(a.partials, a.nact, a.G, red);
+  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+    SysScal& so = a.so[a.act[q]];
+    const double bn = sqrt(red[q * 2].x);
+    so.bnorm = bn;
+    so.btol = a.tol * bn;
+    so.scale = (bn > 0.0 && isfinite(bn)) ? ldexp(1.0, -ilogb(bn)) : 1.0;
+    so.iter = 0;
+    so.tot_inner = 0;
+    so.status = isnan(bn) ? ST_BREAK : (bn == 0.0 ? ST_CONV : ST_RUN);
+    if (isnan(bn)) so.iter = -1;
+  }
+}
+
+// r = b - A x (FP64) -> r32 = scale*r, d32 = 0; <r,r> -> convergence, inner set-up
+__global__ __launch_bounds__(BS) void k_outer_resid(OuterArgs a) {
+  __shared__ Z lds[2 * BS / 64];
+  __shared__ Z red[64 * 2];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& so0 = a.so[sys];
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (so0.status == ST_RUN) {
+    const int64_t o = (int64_t)sys * a.nE;
+    const double dsc = kIsignD * so0.freq_omega, sc = so0.scale;
+    for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+      const RowInfo ri = decode_row(a.A, rr);
+      const Z y = csub(a.b[o + ri.e], apply_row<double, double>(a.A, ri, a.x + o, dsc));
+      a.r32[o + ri.e] = {float(sc * y.x), float(sc * y.y)};
+      a.d32[o + ri.e] = {0.0f, 0.0f};
+      acc[0].x += y.x * y.x + y.y * y.y;
+    }
+  }
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
+    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
+  }
+  if (!last_block(a.counter, gridDim.x)) return;
+  reduce_partials<2>(a.partials, a.nact, a.G, red);
+  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+    const int s = a.act[q];
+    SysScal& so = a.so[s];
+    SysScal& sn = a.si[s];
+    if (so.status != ST_RUN) { sn.status = ST_CONV; continue; }
+    const double rn = sqrt(red[q * 2].x);
+    so.rnorm = rn;
+    if (a.rerr && so.iter < a.rerr_len) a.rerr[(int64_t)s * a.rerr_len + so.iter] = rn / so.bnorm;
+    so.iter += 1;
+    if (rn < so.btol) { so.status = ST_CONV; sn.status = ST_CONV; continue; }
+    if (so.iter > a.max_outer || so.tot_inner >= a.maxit_total) {
+      so.status = ST_MAXIT; sn.status = ST_CONV; continue;
+    }
+    const double bi = rn * so.scale;
+    sn.bnorm = bi;
+    // ask the inner solve for the reduction still needed, at least inner_tol
+    const double need = 0.5 * so.btol / rn;
+    sn.btol = bi * fmax(a.inner_tol, need);
+    sn.rnorm = bi;
+    sn.rnorm_start = bi;
+    sn.rnorm_best = bi;
+    sn.iter = 0;
+    sn.maxit = a.maxit_total - so.tot_inner;
+    sn.best_due = 0;
+    sn.restart_due = 1;
+    sn.restart = so.restart;
+    sn.rho = {bi * bi, 0.0};
+    sn.rho1 = {1.0, 0.0};
+    sn.alpha = {1.0, 0.0};
+    sn.omg = {1.0, 0.0};
+    sn.inner = 1;
+    sn.fidx = so.fidx;
+    sn.freq_omega = so.freq_omega;
+    sn.status = ST_RUN;
+  }
+}
+
+// x += d / scale for systems still refining; count inner iterations.
+__global__ __launch_bounds__(BS) void k_outer_update(OuterArgs a) {
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& so = a.so[sys];
+  if (so.status != ST_RUN) return;
+  const int64_t o = (int64_t)sys * a.nE;
+  const double inv = 1.0 / so.scale;
+  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+    const C2<float> d = a.d32[o + e];
+    Z x = a.x[o + e];
+    x.x += inv * double(d.x);
+    x.y += inv * double(d.y);
+    a.x[o + e] = x;
+  }
+  if (g == 0 && threadIdx.x == 0) a.so[sys].tot_inner += a.si[sys].iter;
+}
+
+}  // namespace mt

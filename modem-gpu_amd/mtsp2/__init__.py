@@ -1,0 +1,306 @@
+"""Python host binding of libmtsp2 (the MI355X SP2 3D-MT solver library).
+
+The product boundary is the C ABI in include/mtsp2.h; the reference's own host
+is Fortran and binds it through modem-gpu_amd/fortran/mtsp2_fortran.f90.  This
+module is the same binding for Python callers (tests, benchmark): ctypes over
+plain pointers, mirroring the reference's solver interface
+(cuBiCG(b, x, KSPiter, device_idx), f90/3D_MT/FWD_SP2/solver.f90:931-965).
+
+The library must be built (``make -C modem-gpu_amd``); loading fails loudly
+otherwise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmtsp2.so")
+
+MT_OK, MT_E_ARG, MT_E_DEVICE, MT_E_OOM, MT_E_NAN_RHS = 0, 1, 2, 3, 4
+MT_E_BREAKDOWN, MT_E_MAXIT, MT_E_STATE, MT_E_STENCIL = 5, 6, 7, 8
+PC_DILU, PC_JACOBI, PC_NONE = 0, 1, 2
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class MTError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"libmtsp2 error {code}: {msg}")
+        self.code = code
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("maxit", C.c_int), ("tol", C.c_double), ("precond", C.c_int),
+                ("precision", C.c_int), ("restart", C.c_int), ("best_iterate", C.c_int),
+                ("inner_tol", C.c_double), ("max_outer", C.c_int), ("op_transpose", C.c_int),
+                ("chunk", C.c_int), ("use_graph", C.c_int)]
+
+
+class SysStats(C.Structure):
+    _fields_ = [("niter", C.c_int), ("nouter", C.c_int), ("status", C.c_int),
+                ("relres", C.c_double)]
+
+
+_lib = None
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_PI = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_PD = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_PZ = np.ctypeslib.ndpointer(np.complex128, flags="C_CONTIGUOUS")
+
+_SIGS = {
+    "mt_default_opts": (None, [C.POINTER(SolveOpts)]),
+    "mt_ctx_create": (_I, [_I, C.POINTER(_P)]),
+    "mt_ctx_destroy": (_I, [_P]),
+    "mt_last_error": (C.c_char_p, [_P]),
+    "mt_device_count": (_I, [C.POINTER(_I)]),
+    "mt_set_grid": (_I, [_P, _I, _I, _I]),
+    "mt_operator_set_csr": (_I, [_P, _I, _I, _PI, _PI, _I, _PD, _PD]),
+    "mt_assemble": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _PD, _D]),
+    "mt_host_assemble": (_I, [_I, _I, _I, _I, _PD, _PD, _PD, _PD, _D, C.POINTER(_I),
+                              C.POINTER(_I), _P, _P, _P, _P]),
+    "mt_host_dilu_pivots": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _I, _PD, _PD, _D, _PZ]),
+    "mt_operator_size": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
+    "mt_operator_get_csr": (_I, [_P, _PI, _PI, _PD, _PD]),
+    "mt_bicg": (_I, [_P, _D, _PZ, _PZ, _I, _D, _I, C.POINTER(_I), _PD, C.POINTER(_I)]),
+    "mt_solve_batch": (_I, [_P, _I, _PD, _PZ, _PZ, C.POINTER(SolveOpts), C.POINTER(SysStats)]),
+    "mt_solve_batch_dev": (_I, [_P, _I, _PD, _P, _P, C.POINTER(SolveOpts), C.POINTER(SysStats)]),
+    "mt_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
+    "mt_precond_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
+    "mt_rhs_fwd": (_I, [_P, _PZ, _PZ]),
+    "mt_rhs_trn": (_I, [_P, _D, _PZ, _PZ]),
+    "mt_post_trn": (_I, [_P, _PZ, _PZ]),
+    "mt_divcor": (_I, [_P, _D, _PZ, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
+    "mt_pmultT": (_I, [_P, _D, _I, _PZ, _PZ, _PD, _I]),
+    "mt_autotune": (_I, [_P, _I, C.POINTER(_I)]),
+    "mt_set_tuning": (_I, [_P, _I]),
+    "mt_bench_spmv": (_I, [_P, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
+    "mt_last_spmv_timing": (_I, [_P, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)]),
+}
+
+
+def exported_symbols():
+    """Names the C ABI exports (include/mtsp2.h)."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """Loads libmtsp2.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise RuntimeError(f"libmtsp2 not built: {LIB_PATH} missing "
+                               "(run `make -C modem-gpu_amd` or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count():
+    n = _I(0)
+    lib().mt_device_count(C.byref(n))
+    return n.value
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().mt_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def host_assemble(dx, dy, dz, nzAir, logcond, air_log=np.log(1e-10)):
+    """Host-only assembly (no GPU): AAii as 0-based interior CSR plus vms1."""
+    dx = np.ascontiguousarray(dx, np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    dz = np.ascontiguousarray(dz, np.float64)
+    nze = len(dz) - int(nzAir)
+    lc = np.ascontiguousarray(np.asarray(logcond, np.float64).ravel(order="F"))
+    ni, nz = _I(), _I()
+    args = (len(dx), len(dy), int(nzAir), nze, dx, dy, dz, lc, float(air_log))
+    rc = lib().mt_host_assemble(*args, C.byref(ni), C.byref(nz), None, None, None, None)
+    if rc:
+        raise MTError(rc, "mt_host_assemble failed")
+    row = np.zeros(ni.value + 1, np.int32)
+    col = np.zeros(nz.value, np.int32)
+    val = np.zeros(nz.value)
+    vms = np.zeros(ni.value)
+    rc = lib().mt_host_assemble(*args, C.byref(ni), C.byref(nz), row.ctypes.data, col.ctypes.data,
+                                val.ctypes.data, vms.ctypes.data)
+    if rc:
+        raise MTError(rc, "mt_host_assemble failed")
+    return {"row": row, "col": col, "val": val, "vms1": vms, "nrow": ni.value}
+
+
+def host_dilu_pivots(grid_cells, row, col, val, vms1, omega, base=0):
+    """Host-only inverse block-DILU pivots (interior order)."""
+    nx, ny, nz = grid_cells
+    n = len(row) - 1
+    out = np.zeros(n, np.complex128)
+    rc = lib().mt_host_dilu_pivots(nx, ny, nz, n, int(len(col)), np.ascontiguousarray(row, np.int32),
+                                   np.ascontiguousarray(col, np.int32), int(base),
+                                   np.ascontiguousarray(val, np.float64),
+                                   np.ascontiguousarray(vms1, np.float64), float(omega), out)
+    if rc:
+        raise MTError(rc, "mt_host_dilu_pivots failed")
+    return out
+
+
+class Context:
+    """One device, one stream, one device-resident operator."""
+
+    def __init__(self, device=0):
+        h = _P()
+        rc = lib().mt_ctx_create(int(device), C.byref(h))
+        if rc != MT_OK:
+            raise MTError(rc, f"mt_ctx_create(device={device}) failed")
+        self._h = h
+        self.n_int = None
+        self.n_edge = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mt_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != MT_OK:
+            raise MTError(rc, lib().mt_last_error(self._h).decode())
+
+    def _sizes(self):
+        ni, ne, nz = _I(), _I(), _I()
+        self._chk(lib().mt_operator_size(self._h, C.byref(ni), C.byref(ne), C.byref(nz)))
+        self.n_int, self.n_edge, self.nnz = ni.value, ne.value, nz.value
+
+    # -- operator ---------------------------------------------------------
+    def set_operator_csr(self, grid_cells, n, rowptr, col, val, vms1, base=1):
+        nx, ny, nz = grid_cells
+        self._chk(lib().mt_set_grid(self._h, nx, ny, nz))
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        col = np.ascontiguousarray(col, np.int32)
+        self._chk(lib().mt_operator_set_csr(self._h, int(n), int(len(col)), rowptr, col, int(base),
+                                            np.ascontiguousarray(val, np.float64),
+                                            np.ascontiguousarray(vms1, np.float64)))
+        self._sizes()
+
+    def assemble(self, dx, dy, dz, nzAir, logcond, air_log=np.log(1e-10)):
+        dx = np.ascontiguousarray(dx, np.float64)
+        dy = np.ascontiguousarray(dy, np.float64)
+        dz = np.ascontiguousarray(dz, np.float64)
+        nze = len(dz) - int(nzAir)
+        lc = np.asarray(logcond, np.float64)
+        assert lc.shape == (len(dx), len(dy), nze)
+        self._chk(lib().mt_assemble(self._h, len(dx), len(dy), int(nzAir), nze, dx, dy, dz,
+                                    np.ascontiguousarray(lc.ravel(order="F")), float(air_log)))
+        self._sizes()
+
+    def operator_csr(self):
+        row = np.zeros(self.n_int + 1, np.int32)
+        col = np.zeros(self.nnz, np.int32)
+        val = np.zeros(self.nnz)
+        vms = np.zeros(self.n_int)
+        self._chk(lib().mt_operator_get_csr(self._h, row, col, val, vms))
+        return {"row": row, "col": col, "val": val, "vms1": vms}
+
+    # -- solves -------------------------------------------------------------
+    def bicg(self, omega, b, x0=None, maxit=1200, tol=1e-7, precond=PC_DILU):
+        """cuBiCG semantics: returns (x, niter, rerr, failed)."""
+        x = np.zeros(self.n_int, np.complex128) if x0 is None else np.array(x0, np.complex128)
+        rerr = np.zeros(maxit)
+        nit, fl = _I(), _I()
+        self._chk(lib().mt_bicg(self._h, float(omega), np.ascontiguousarray(b, np.complex128), x,
+                                int(maxit), float(tol), int(precond), C.byref(nit), rerr,
+                                C.byref(fl)))
+        return x, nit.value, rerr, bool(fl.value)
+
+    def solve_batch(self, omegas, b, x0=None, **opts):
+        omegas = np.ascontiguousarray(omegas, np.float64)
+        ns = len(omegas)
+        b = np.ascontiguousarray(np.reshape(b, (ns, self.n_int)), np.complex128)
+        x = np.zeros_like(b) if x0 is None else np.array(np.reshape(x0, b.shape), np.complex128)
+        o = default_opts(**opts)
+        st = (SysStats * ns)()
+        self._chk(lib().mt_solve_batch(self._h, ns, omegas, b, x, C.byref(o), st))
+        stats = [{"niter": s.niter, "nouter": s.nouter, "status": s.status, "relres": s.relres}
+                 for s in st]
+        return x, stats
+
+    def solve_batch_dev(self, omegas, b_ptr, x_ptr, **opts):
+        omegas = np.ascontiguousarray(omegas, np.float64)
+        ns = len(omegas)
+        o = default_opts(**opts)
+        st = (SysStats * ns)()
+        self._chk(lib().mt_solve_batch_dev(self._h, ns, omegas, C.c_void_p(b_ptr),
+                                           C.c_void_p(x_ptr), C.byref(o), st))
+        return [{"niter": s.niter, "nouter": s.nouter, "status": s.status, "relres": s.relres}
+                for s in st]
+
+    # -- operator applications -----------------------------------------------
+    def apply(self, omega, x, transpose=0):
+        y = np.zeros(self.n_int, np.complex128)
+        self._chk(lib().mt_apply(self._h, float(omega), int(transpose),
+                                 np.ascontiguousarray(x, np.complex128), y))
+        return y
+
+    def precond_apply(self, omega, x, precond=PC_DILU):
+        y = np.zeros(self.n_int, np.complex128)
+        self._chk(lib().mt_precond_apply(self._h, float(omega), int(precond),
+                                         np.ascontiguousarray(x, np.complex128), y))
+        return y
+
+    def rhs_fwd(self, e_all):
+        b = np.zeros(self.n_int, np.complex128)
+        self._chk(lib().mt_rhs_fwd(self._h, np.ascontiguousarray(e_all, np.complex128), b))
+        return b
+
+    def rhs_trn(self, omega, s_all):
+        b = np.zeros(self.n_int, np.complex128)
+        self._chk(lib().mt_rhs_trn(self._h, float(omega), np.ascontiguousarray(s_all, np.complex128), b))
+        return b
+
+    def post_trn(self, ei):
+        e = np.zeros(self.n_edge, np.complex128)
+        self._chk(lib().mt_post_trn(self._h, np.ascontiguousarray(ei, np.complex128), e))
+        return e
+
+    def pmultT(self, omega, e0_all, lam_all, shape, dm=None):
+        e0 = np.ascontiguousarray(np.atleast_2d(e0_all), np.complex128)
+        lam = np.ascontiguousarray(np.atleast_2d(lam_all), np.complex128)
+        acc = dm is not None
+        out = np.zeros(int(np.prod(shape))) if dm is None else np.ascontiguousarray(
+            np.asarray(dm, np.float64).ravel(order="F"))
+        self._chk(lib().mt_pmultT(self._h, float(omega), e0.shape[0], e0, lam, out, int(acc)))
+        return out.reshape(shape, order="F")
+
+    def divcor(self, omega, ei, maxit=100, tol=1e-7):
+        e = np.array(ei, np.complex128)
+        nit, rr = _I(), _D()
+        self._chk(lib().mt_divcor(self._h, float(omega), e, None, int(maxit), float(tol),
+                                  C.byref(nit), C.byref(rr)))
+        return e, nit.value, rr.value
+
+    # -- tuning / measurement ---------------------------------------------------
+    def autotune(self, nsys):
+        v = _I()
+        self._chk(lib().mt_autotune(self._h, int(nsys), C.byref(v)))
+        return v.value
+
+    def set_tuning(self, groups):
+        self._chk(lib().mt_set_tuning(self._h, int(groups)))
+
+    def bench_spmv(self, nsys, precision=PREC_FP64, reps=20):
+        ms, by = _D(), _D()
+        self._chk(lib().mt_bench_spmv(self._h, int(nsys), int(precision), int(reps),
+                                      C.byref(ms), C.byref(by)))
+        return ms.value, by.value

@@ -1,0 +1,122 @@
+"""GPU parity of libmtsp2 against the oracle (oracle/sp2_oracle.c, itself
+bit-exact against the flang build of the reference, tests/test_oracle_vs_ref.py).
+
+Tolerances (FP64 unless stated):
+  operator apply A x, A^T x, A^H x       rel L2 <= 1e-13 (summation order only)
+  block-DILU apply U^-1 L^-1 x           rel L2 <= 1e-12
+  BiCGStab (cuBiCG semantics, tol 1e-10) rel L2 of x <= 1e-8, true relres <= 1e-10
+  batched FP64 / mixed (tol 1e-10)       rel L2 of x <= 1e-7, true FP64 relres <= tol
+"""
+import numpy as np
+import pytest
+
+import mtsp2
+import sp2oracle
+from conftest import rel, small_problem
+from mtsp2 import problem as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ctx, seed=0, **kw):
+    dx, dy, dz, nza, lc = small_problem(seed, **kw)
+    o = sp2oracle.Problem(dx, dy, dz, nza, lc)
+    ctx.assemble(dx, dy, dz, nza, lc)
+    assert ctx.n_int == o.nInt
+    return o, (dx, dy, dz, nza, lc)
+
+
+def _cvec(rng, n):
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_apply_matches_oracle(gpu_ctx, seed):
+    print(f"case apply seed={seed}", flush=True)
+    o, _ = _setup(gpu_ctx, seed)
+    rng = np.random.default_rng(10 + seed)
+    x = _cvec(rng, o.nInt)
+    for w in (2 * np.pi * 1e-3, 2 * np.pi * 10.0):
+        o.update_freq(w)
+        assert rel(gpu_ctx.apply(w, x), o.mult_aii(x)) <= 1e-13
+        # A^H = the reference's Mult_Aii(adjt=.true.)
+        assert rel(gpu_ctx.apply(w, x, transpose=2), o.mult_aii(x, adjt=True)) <= 1e-13
+        # A^T x = conj(A^H conj(x)) because AAii is real
+        assert rel(gpu_ctx.apply(w, x, transpose=1), np.conj(o.mult_aii(np.conj(x), adjt=True))) <= 1e-13
+
+
+def test_apply_from_reference_csr(gpu_ctx):
+    """Drop-in path: the reference's 1-based AAii CSR uploaded as-is."""
+    print("case apply-from-csr", flush=True)
+    dx, dy, dz, nza, lc = small_problem(2)
+    o = sp2oracle.Problem(dx, dy, dz, nza, lc)
+    A = o.csr("AAii")
+    gpu_ctx.set_operator_csr((len(dx), len(dy), len(dz)), o.nInt, A["row"] + 1, A["col"] + 1,
+                             A["val"], o.vms(), base=1)
+    rng = np.random.default_rng(7)
+    x = _cvec(rng, o.nInt)
+    w = 2 * np.pi * 0.5
+    o.update_freq(w)
+    assert rel(gpu_ctx.apply(w, x), o.mult_aii(x)) <= 1e-14
+
+
+@pytest.mark.parametrize("f_hz", [1e-3, 1.0, 100.0])
+def test_dilu_apply_matches_oracle(gpu_ctx, f_hz):
+    print(f"case dilu f={f_hz}", flush=True)
+    o, _ = _setup(gpu_ctx, 3, nx=19, ny=17, nzAir=4, nzEarth=13)
+    w = 2 * np.pi * f_hz
+    o.update_freq(w)
+    rng = np.random.default_rng(5)
+    x = _cvec(rng, o.nInt)
+    ref = o.pc_usolve(o.pc_lsolve(x))
+    assert rel(gpu_ctx.precond_apply(w, x, mtsp2.PC_DILU), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("pc", [mtsp2.PC_DILU, mtsp2.PC_JACOBI])
+def test_bicg_matches_oracle(gpu_ctx, pc):
+    print(f"case bicg pc={pc}", flush=True)
+    o, g = _setup(gpu_ctx, 4, nx=14, ny=12, nzAir=4, nzEarth=10)
+    w = 2 * np.pi * 0.1
+    o.update_freq(w)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    e = P.plane_wave(grid, w, 0)
+    b = o.rhs_fwd(e)
+    assert rel(gpu_ctx.rhs_fwd(e), b) <= 1e-13
+    x0 = e[o.index()[0]]
+    xg, nit, rerr, failed = gpu_ctx.bicg(w, b, x0, maxit=600, tol=1e-10, precond=pc)
+    xo, info = o.bicg(b, x0, maxit=600, tol=1e-10, gpu_semantics=True, precond=pc)
+    assert info["status"] == 0 and not failed
+    assert abs(nit - info["niter"]) <= max(3, info["niter"] // 10)
+    assert rel(xg, xo) <= 1e-8
+    r = b - o.mult_aii(xg)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+    assert rerr[nit - 1] <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [mtsp2.PREC_FP64, mtsp2.PREC_MIXED])
+def test_batch_matches_oracle(gpu_ctx, precision):
+    print(f"case batch precision={precision}", flush=True)
+    o, g = _setup(gpu_ctx, 5, nx=14, ny=12, nzAir=4, nzEarth=10)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [1e-3, 1.0, 100.0])
+    X, stats = gpu_ctx.solve_batch(om, B, X0, tol=1e-10, maxit=2000, precision=precision,
+                                   precond=mtsp2.PC_DILU)
+    for s in range(len(om)):
+        o.update_freq(om[s])
+        assert stats[s]["status"] == 0, stats[s]
+        r = B[s] - o.mult_aii(X[s])
+        assert np.linalg.norm(r) / np.linalg.norm(B[s]) <= 1e-10 * 1.0001
+        xo, info = o.bicg(B[s], X0[s], maxit=2000, tol=1e-10, gpu_semantics=True)
+        assert rel(X[s], xo) <= 1e-7
+
+
+def test_graph_and_eager_agree(gpu_ctx):
+    """The hipGraph-replayed chunks give the same iterates as eager launches."""
+    print("case graph-vs-eager", flush=True)
+    o, g = _setup(gpu_ctx, 6, nx=12, ny=12, nzAir=4, nzEarth=8)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [0.01, 10.0])
+    Xg, sg = gpu_ctx.solve_batch(om, B, X0, tol=1e-9, use_graph=1, chunk=5)
+    Xe, se = gpu_ctx.solve_batch(om, B, X0, tol=1e-9, use_graph=0, chunk=5)
+    assert [s["niter"] for s in sg] == [s["niter"] for s in se]
+    assert np.array_equal(Xg, Xe)
