@@ -1,0 +1,184 @@
+"""Benchmark of the SP2 3D-MT solve on one MI355X.
+
+Workload (the config the headline metric is quoted on): 100x100x60 cells + 10
+air layers (2,052,270 interior edges), 200 seeded blocky boxes over 100 ohm-m,
+8 frequencies 1e-3..1e2 Hz x 2 polarizations = 16 forward systems, FP64 true
+relative residual <= 1e-7 for every system.
+
+A step is one pass of the hot path over the batch: the 16 forward systems
+solved together in the mixed-precision batched solver (FP32 BiCGStab inner
+iterations, FP64 defect correction, block-DILU, hipGraph-replayed chunks), with
+right-hand sides and initial guesses already resident in HBM.  value =
+solves/s over the timed steps.  Beside it: the plain FP64 GPU solver (one
+system at a time, the drop-in cuBiCG semantics), the fused SpMV kernel's HBM
+roofline from HIP events, and the reference CPU path timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "modem-gpu_amd"))
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(grid, lc, om, B, X0, iters_per_solve, sample_iters=12):
+    """Reference CPU SP2 BiCG (flang build in oracle/_ref) timed for a bounded
+    number of iterations on one system of the same workload; falls back to the
+    oracle's C restatement when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    try:
+        import refdriver
+        if refdriver.available():
+            t0 = time.time()
+            out = refdriver.run(grid, lc, np.log(1e-10), [om[0]],
+                                solves=[(0, sample_iters, 1e-30, B[0], X0[0])], dump=False,
+                                timeout=900)
+            r = out["bicg_0"]
+            per_it = r["seconds"] / max(1, sample_iters)
+            kind, total = "reference", time.time() - t0
+        else:
+            raise FileNotFoundError
+    except (ImportError, FileNotFoundError, OSError):
+        import sp2oracle
+        t0 = time.time()
+        p = sp2oracle.Problem(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
+        p.update_freq(om[0])
+        t1 = time.time()
+        p.bicg(B[0], X0[0], maxit=sample_iters, tol=1e-30, gpu_semantics=False)
+        per_it = (time.time() - t1) / sample_iters
+        kind, total = "port", time.time() - t0
+    solves_per_s = 1.0 / (per_it * iters_per_solve)
+    return {"value": solves_per_s, "unit": "solves/s", "cores": 1, "kind": kind,
+            "sample": (f"{sample_iters} BiCGStab iterations of one system (f={om[0]/(2*np.pi):.3g} Hz) "
+                       f"of the same 100x100x60+10 workload on 1 host core: {per_it:.3f} s/iteration, "
+                       f"scaled by the mean FP64 iterations per solve ({iters_per_solve:.1f}); "
+                       f"{total:.1f} s wall incl. operator set-up")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="cfg1")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=1e-7)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--precond", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(systems shard across GPUs by frequency with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mtsp2
+    from mtsp2 import problem as P
+
+    if mtsp2.device_count() < 1:
+        raise RuntimeError("no HIP device visible")
+    grid = P.config_grid(args.config)
+    lc = P.config_model(args.config, grid, args.seed)
+    fhz = P.config_freqs(args.config)
+    ctx = mtsp2.Context(0)
+    t0 = time.time()
+    ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
+    log(f"assembled n_int={ctx.n_int} nnz={ctx.nnz} in {time.time()-t0:.1f}s")
+    om, B, X0 = P.forward_systems(ctx, grid, fhz)
+    nsys, n = B.shape
+    zb = nsys * n * 16
+    d_b, d_x0, d_x = ctx.dev_alloc(zb), ctx.dev_alloc(zb), ctx.dev_alloc(zb)
+    ctx.upload(d_b, B)
+    ctx.upload(d_x0, X0)
+    tuned = ctx.autotune(nsys)
+    log(f"autotune: {tuned} row-tile groups per system")
+    opts = dict(tol=args.tol, maxit=4000, precision=mtsp2.PREC_MIXED, precond=args.precond,
+                inner_tol=1e-4, chunk=10, use_graph=1)
+
+    def step():
+        ctx.dev_copy(d_x, d_x0, zb)
+        return ctx.solve_batch_dev(om, d_b, d_x, **opts)
+
+    for w in range(args.warmup):
+        st = step()
+        log(f"warmup {w}: iters {[s['niter'] for s in st]}")
+    t0 = time.time()
+    for k in range(args.steps):
+        st = step()
+    el = time.time() - t0
+    ms = 1e3 * el / args.steps
+    value = nsys * args.steps / el
+    ok = all(s["status"] == 0 and s["relres"] <= args.tol for s in st)
+    relres = max(s["relres"] for s in st)
+    inner = [s["niter"] for s in st]
+    log(f"mixed+batched: {ms:.1f} ms/step, {value:.2f} solves/s, max relres {relres:.2e}, "
+        f"inner iters {inner}, outer {[s['nouter'] for s in st]}")
+
+    fp64 = None
+    fp64_iters = None
+    if not args.no_fp64:
+        # plain FP64 GPU solver: one system at a time (drop-in semantics)
+        t0 = time.time()
+        its = []
+        for s in range(nsys):
+            bs = d_b + s * n * 16
+            xs = d_x + s * n * 16
+            ctx.dev_copy(xs, d_x0 + s * n * 16, n * 16)
+            r = ctx.solve_batch_dev(om[s:s + 1], bs, xs, tol=args.tol, maxit=4000,
+                                    precision=mtsp2.PREC_FP64, precond=args.precond,
+                                    chunk=10, use_graph=1)
+            its.append(r[0]["niter"])
+        fp64 = nsys / (time.time() - t0)
+        fp64_iters = float(np.mean(its))
+        log(f"plain FP64 GPU: {fp64:.2f} solves/s, iters {its}")
+
+    spmv_ms, spmv_bytes = ctx.bench_spmv(nsys, mtsp2.PREC_MIXED, reps=20)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    log(f"SpMV (FP32, {nsys} systems): {spmv_ms*1e3:.1f} us/launch, {spmv_bytes/1e6:.1f} MB, "
+        f"{achieved:.0f} GB/s")
+    spmv64_ms, spmv64_bytes = ctx.bench_spmv(1, mtsp2.PREC_FP64, reps=20)
+    log(f"SpMV (FP64, 1 system): {spmv64_ms*1e3:.1f} us/launch, "
+        f"{spmv64_bytes/spmv64_ms/1e6:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(grid, lc, om, B, X0, fp64_iters or float(np.mean(inner)))
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "solves/s at 100x100x60 grid, tol 1e-7: mixed+batched vs FP64 GPU; SpMV HBM GB/s",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c32 inner / c64 refinement",
+        "data": "synthetic (seeded blocky model, plane-wave boundary data)",
+        "config": {"workload": f"{args.config}: 100x100x60+10 air, 2052270 interior edges, "
+                               "8 f x 2 pol forward systems, tol 1e-7 FP64 true residual",
+                   "nsys": nsys, "seed": args.seed, "precond": "block-DILU"},
+        "converged": ok, "max_relres": relres,
+        "fp64_gpu_solves_per_s": fp64,
+        "speedup_vs_fp64_gpu": (value / fp64) if fp64 else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "k_spmv_dot<float,float,1> (v = A x, <rt,v>), 16 systems",
+                     "bytes_per_launch": spmv_bytes, "us_per_launch": spmv_ms * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for p in (d_b, d_x0, d_x):
+        ctx.dev_free(p)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

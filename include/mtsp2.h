@@ -22,6 +22,7 @@
 #ifndef MTSP2_H
 #define MTSP2_H
 
+#include <stddef.h>
 #ifdef __cplusplus
 #include <complex>
 typedef std::complex<double> mt_zdouble;
@@ -143,6 +144,13 @@ int mt_solve_batch(mt_ctx *ctx, int nsys, const double *omegas, const mt_zdouble
 /* Same with device-resident b/x (interior-edge order, system-major). */
 int mt_solve_batch_dev(mt_ctx *ctx, int nsys, const double *omegas, const void *b_dev,
                        void *x_dev, const mt_solve_opts *opts, mt_sys_stats *stats);
+
+/* Device buffers for callers without their own allocator (e.g. the
+ * benchmark): allocate, free, and copy on the context's stream (kind: 0 H2D,
+ * 1 D2H, 2 D2D; synchronous). */
+int mt_dev_alloc(mt_ctx *ctx, size_t bytes, void **ptr);
+int mt_dev_free(mt_ctx *ctx, void *ptr);
+int mt_dev_copy(mt_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 
 /* ---- operator applications (Mult_Aii, PC_Lsolve/PC_Usolve) -------------- */
 /* y = A(omega) x (Mult_Aii, modelOperator3D.f90:415-436).  op=0: AAii x +

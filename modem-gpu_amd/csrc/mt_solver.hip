@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <string>
 #include <vector>
@@ -48,6 +49,10 @@ struct mt_ctx {
   TilePlan tiles;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  // block-DILU inverse pivots per (frequency, precision, transpose), on the
+  // device: PC_setup runs once per frequency, as UpdateFreq does
+  // (modelOperator3D.f90:155-164), not once per solve
+  std::map<std::tuple<uint64_t, int, int>, void*> pivots;
   // last solve timing of the SpMV kernels
   int spmv_launches = 0;
   double spmv_ms = 0.0;
@@ -182,9 +187,20 @@ static int build_tile_plan(mt_ctx* ctx) {
   return MT_OK;
 }
 
+static void free_workspaces(mt_ctx* ctx) {
+  hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->ws) hipFree(kv.second.first);
+  ctx->ws.clear();
+  for (auto& kv : ctx->pivots) hipFree(kv.second);
+  ctx->pivots.clear();
+}
+
 static int upload_operator(mt_ctx* ctx) {
   const Grid& g = ctx->hop.g;
   free_dop(ctx->dop);
+  // Padded vectors rely on zero boundary entries and a zero tail; buffers laid
+  // out for another grid would carry stale values there, so drop them all.
+  free_workspaces(ctx);
   DeviceOperator& d = ctx->dop;
   const size_t nc = size_t(kSlots) * g.nE;
   std::vector<float> tmp(nc);
@@ -251,11 +267,23 @@ static int build_dinv(mt_ctx* ctx, int pc, bool transpose, const std::vector<dou
     src = &tmpT;
   }
   std::vector<zc> dv;
-  std::vector<C2<T>> hv(g.nE);
+  std::vector<C2<T>> hv;
   for (int f = 0; f < nf; ++f) {
-    dilu_inv_pivots(*src, freqs[f], dv);
-    for (int64_t e = 0; e < g.nE; ++e) hv[e] = {T(dv[e].real()), T(dv[e].imag())};
-    MT_HIP_CHECK(ctx, hipMemcpy(*out + size_t(f) * g.nE, hv.data(), g.nE * sizeof(C2<T>), hipMemcpyHostToDevice));
+    uint64_t bits;
+    std::memcpy(&bits, &freqs[f], sizeof(bits));
+    const auto key = std::make_tuple(bits, int(sizeof(T)), int(transpose));
+    auto it = ctx->pivots.find(key);
+    if (it == ctx->pivots.end()) {
+      dilu_inv_pivots(*src, freqs[f], dv);
+      hv.resize(g.nE);
+      for (int64_t e = 0; e < g.nE; ++e) hv[e] = {T(dv[e].real()), T(dv[e].imag())};
+      void* d = nullptr;
+      MT_HIP_CHECK(ctx, hipMalloc(&d, g.nE * sizeof(C2<T>)));
+      MT_HIP_CHECK(ctx, hipMemcpy(d, hv.data(), g.nE * sizeof(C2<T>), hipMemcpyHostToDevice));
+      it = ctx->pivots.emplace(key, d).first;
+    }
+    MT_HIP_CHECK(ctx, hipMemcpyAsync(*out + size_t(f) * g.nE, it->second, g.nE * sizeof(C2<T>),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
   }
   return MT_OK;
 }
@@ -710,7 +738,7 @@ int mt_ctx_destroy(mt_ctx* ctx) {
   if (!ctx) return MT_OK;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
-  for (auto& kv : ctx->ws) hipFree(kv.second.first);
+  free_workspaces(ctx);
   free_dop(ctx->dop);
   hipFree(ctx->tiles.tiles);
   hipFree(ctx->tiles.shape);
@@ -904,6 +932,31 @@ int mt_solve_batch_dev(mt_ctx* ctx, int nsys, const double* omegas, const void* 
   if ((rc = solve_batch_common(ctx, nsys, omegas, db, dx, opts, stats))) return rc;
   k_gather<double, double><<<1024, 256, 0, ctx->stream>>>(ctx->dop.imap, g.nInt, g.nE, nsys, dx, (Z*)x_dev);
   MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_dev_alloc(mt_ctx* ctx, size_t bytes, void** ptr) {
+  if (!ctx || !ptr) return MT_E_ARG;
+  hipSetDevice(ctx->device);
+  if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) {
+    (void)hipGetLastError();
+    return mt_fail(ctx, MT_E_OOM, "mt_dev_alloc: hipMalloc failed");
+  }
+  return MT_OK;
+}
+int mt_dev_free(mt_ctx* ctx, void* ptr) {
+  if (!ctx) return MT_E_ARG;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  MT_HIP_CHECK(ctx, hipFree(ptr));
+  return MT_OK;
+}
+int mt_dev_copy(mt_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+  if (!ctx || kind < 0 || kind > 2) return MT_E_ARG;
+  hipSetDevice(ctx->device);
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
   MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MT_OK;
 }

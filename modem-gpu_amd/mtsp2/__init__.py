@@ -68,6 +68,9 @@ _SIGS = {
     "mt_bicg": (_I, [_P, _D, _PZ, _PZ, _I, _D, _I, C.POINTER(_I), _PD, C.POINTER(_I)]),
     "mt_solve_batch": (_I, [_P, _I, _PD, _PZ, _PZ, C.POINTER(SolveOpts), C.POINTER(SysStats)]),
     "mt_solve_batch_dev": (_I, [_P, _I, _PD, _P, _P, C.POINTER(SolveOpts), C.POINTER(SysStats)]),
+    "mt_dev_alloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
+    "mt_dev_free": (_I, [_P, _P]),
+    "mt_dev_copy": (_I, [_P, _P, _P, C.c_size_t, _I]),
     "mt_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
     "mt_precond_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
     "mt_rhs_fwd": (_I, [_P, _PZ, _PZ]),
@@ -245,6 +248,26 @@ class Context:
                                            C.c_void_p(x_ptr), C.byref(o), st))
         return [{"niter": s.niter, "nouter": s.nouter, "status": s.status, "relres": s.relres}
                 for s in st]
+
+    # -- device buffers (benchmark plumbing) ----------------------------------
+    def dev_alloc(self, nbytes):
+        p = _P()
+        self._chk(lib().mt_dev_alloc(self._h, int(nbytes), C.byref(p)))
+        return p.value
+
+    def dev_free(self, ptr):
+        self._chk(lib().mt_dev_free(self._h, C.c_void_p(ptr)))
+
+    def upload(self, ptr, arr):
+        arr = np.ascontiguousarray(arr)
+        self._chk(lib().mt_dev_copy(self._h, C.c_void_p(ptr), arr.ctypes.data, arr.nbytes, 0))
+
+    def download(self, ptr, out):
+        self._chk(lib().mt_dev_copy(self._h, out.ctypes.data, C.c_void_p(ptr), out.nbytes, 1))
+        return out
+
+    def dev_copy(self, dst, src, nbytes):
+        self._chk(lib().mt_dev_copy(self._h, C.c_void_p(dst), C.c_void_p(src), int(nbytes), 2))
 
     # -- operator applications -----------------------------------------------
     def apply(self, omega, x, transpose=0):

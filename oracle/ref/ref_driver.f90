@@ -19,6 +19,7 @@
 !   edge -> model adjoint  ModelMap.inc:281-372 dEdgeToModelParam
 !
 ! Input file layout (stream, native endianness), written by oracle/refdriver.py:
+!   int32 flags (bit 0: dump index sets, metrics and operators)
 !   int32 nx, ny, nzAir, nzEarth
 !   real64 dx(nx), dy(ny), dz(nzAir+nzEarth)
 !   real64 logcond(nx*ny*nzEarth) (Fortran order), airLogCond
@@ -57,13 +58,16 @@ program ref_driver
    complex(kind=prec), allocatable :: b(:), x(:), y(:), t1(:), t2(:), rhs(:), phi(:)
    real(kind=prec), allocatable  :: e(:)
    type(solverControl_t)         :: ctl
-   integer                       :: nInt, nAll, nNi
+   integer                       :: nInt, nAll, nNi, flags
+   integer(kind=8)               :: c0, c1, crate
+   real(kind=8)                  :: secs
    character(len=16)             :: tag
 
    call get_command_argument(1, infile)
    call get_command_argument(2, outdir)
 
    open(newunit=fin, file=trim(infile), access='stream', form='unformatted', status='old')
+   read(fin) flags
    read(fin) nx, ny, nzAir, nzEarth
    nz = nzAir + nzEarth
    allocate(dx(nx), dy(ny), dz(nz), logc(nx*ny*nzEarth))
@@ -87,6 +91,10 @@ program ref_driver
    nAll = nInt + size(EDGEb)
    nNi = size(NODEi)
 
+   call wr_open(trim(outdir)//'/sizes.bin', s)
+   write(s) nAll, nInt, size(EDGEb), size(NODEi)+size(NODEb), nNi, size(NODEb)
+   close(s)
+   if (iand(flags, 1) .ne. 0) then
    ! index sets and metric vectors
    call wr_open(trim(outdir)//'/index.bin', s)
    write(s) nAll, nInt, size(EDGEb), size(NODEi)+size(NODEb), nNi, size(NODEb)
@@ -115,6 +123,7 @@ program ref_driver
    call wr_csr_r(trim(outdir)//'/GDii.csr', GDii)
    call wr_csr_r(trim(outdir)//'/T.csr', T)
    call wr_csr_r(trim(outdir)//'/G.csr', G)
+   endif
 
    read(fin) nomega
    allocate(omegas(nomega))
@@ -125,8 +134,10 @@ program ref_driver
       call wr_open(trim(outdir)//'/vms_'//trim(tag)//'.bin', s)
       write(s) VomegaMuSig
       close(s)
-      call wr_csr_c(trim(outdir)//'/L_'//trim(tag)//'.csr', L)
-      call wr_csr_c(trim(outdir)//'/U_'//trim(tag)//'.csr', U)
+      if (iand(flags, 1) .ne. 0) then
+         call wr_csr_c(trim(outdir)//'/L_'//trim(tag)//'.csr', L)
+         call wr_csr_c(trim(outdir)//'/U_'//trim(tag)//'.csr', U)
+      endif
    enddo
 
    allocate(b(nInt), x(nInt), y(nInt), t1(nInt), t2(nInt))
@@ -144,12 +155,16 @@ program ref_driver
       ctl%failed = .false.
       allocate(ctl%rerr(maxit))
       ctl%rerr = 0.0
+      call system_clock(c0, crate)
       call BICG(b, x, ctl)
+      call system_clock(c1)
+      secs = dble(c1 - c0) / dble(crate)
       write(tag, '(I0)') k
       call wr_open(trim(outdir)//'/bicg_'//trim(tag)//'.bin', s)
       i = 0
       if (ctl%failed) i = 1
       write(s) ctl%niter, i, maxit
+      write(s) secs
       write(s) ctl%rerr
       write(s) x
       close(s)

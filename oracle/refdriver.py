@@ -32,7 +32,7 @@ def _rd_csr(path, cplx=False):
 
 
 def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
-        e2m=None, workdir=None, timeout=600):
+        e2m=None, workdir=None, timeout=600, dump=True):
     """grid: dict(dx, dy, dz, nzAir); logcond: (nx, ny, nzEarth) natural-log
     conductivities.  solves: list of (iomega0, maxit, tol, b, x0).  applies:
     list of (iomega0, x).  divcor: (maxit, tol, rhs) or None.  e2m: real
@@ -49,6 +49,7 @@ def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
     wd = tempfile.mkdtemp(prefix="mtref_") if own else workdir
     inp = os.path.join(wd, "input.bin")
     with open(inp, "wb") as f:
+        np.array([1 if dump else 0], np.int32).tofile(f)
         np.array([nx, ny, nza, nze], np.int32).tofile(f)
         dx.tofile(f); dy.tofile(f); dz.tofile(f)
         lc.ravel(order="F").tofile(f)
@@ -80,14 +81,27 @@ def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
     subprocess.run([REF_BIN, inp, wd], check=True, timeout=timeout,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
     out = {}
+    nall, nint, nb, nnall, nni, nnb = np.fromfile(os.path.join(wd, "sizes.bin"), np.int32, 6)
+    out["sizes"] = (int(nall), int(nint), int(nb), int(nnall), int(nni), int(nnb))
+    nfx, nfy, nfz = (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
+    nface = nfx + nfy + nfz
+    for k in range(len(solves)):
+        with open(os.path.join(wd, f"bicg_{k+1}.bin"), "rb") as f:
+            niter, failed, maxit = np.fromfile(f, np.int32, 3)
+            secs = float(np.fromfile(f, np.float64, 1)[0])
+            rerr = np.fromfile(f, np.float64, maxit)
+            x = np.fromfile(f, np.complex128, nint)
+        out[f"bicg_{k}"] = {"niter": int(niter), "failed": bool(failed), "rerr": rerr, "x": x,
+                            "seconds": secs}
+    if not dump:
+        _cleanup(wd, own)
+        return out
     with open(os.path.join(wd, "index.bin"), "rb") as f:
-        nall, nint, nb, nnall, nni, nnb = np.fromfile(f, np.int32, 6)
+        np.fromfile(f, np.int32, 6)
         out["EDGEi"] = np.fromfile(f, np.int32, nint)
         out["EDGEb"] = np.fromfile(f, np.int32, nb)
         out["NODEi"] = np.fromfile(f, np.int32, nni)
         out["NODEb"] = np.fromfile(f, np.int32, nnb)
-    nfx, nfy, nfz = (nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)
-    nface = nfx + nfy + nfz
     with open(os.path.join(wd, "metric.bin"), "rb") as f:
         out["FaceA"] = np.fromfile(f, np.float64, nface)
         out["EdgeL"] = np.fromfile(f, np.float64, nall)
@@ -104,12 +118,6 @@ def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
         out[f"vms_{k}"] = np.fromfile(os.path.join(wd, f"vms_{k+1}.bin"), np.float64)
         out[f"L_{k}"] = _rd_csr(os.path.join(wd, f"L_{k+1}.csr"), cplx=True)
         out[f"U_{k}"] = _rd_csr(os.path.join(wd, f"U_{k+1}.csr"), cplx=True)
-    for k in range(len(solves)):
-        with open(os.path.join(wd, f"bicg_{k+1}.bin"), "rb") as f:
-            niter, failed, maxit = np.fromfile(f, np.int32, 3)
-            rerr = np.fromfile(f, np.float64, maxit)
-            x = np.fromfile(f, np.complex128, nint)
-        out[f"bicg_{k}"] = {"niter": int(niter), "failed": bool(failed), "rerr": rerr, "x": x}
     for k in range(len(applies)):
         a = np.fromfile(os.path.join(wd, f"apply_{k+1}.bin"), np.complex128)
         out[f"apply_{k}"] = {"Ax": a[:nint], "Lsolve": a[nint:2 * nint], "LUsolve": a[2 * nint:]}
@@ -124,8 +132,12 @@ def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
     if e2m is not None:
         out["e2m"] = np.fromfile(os.path.join(wd, "e2m.bin"), np.float64).reshape(
             (nx, ny, nze), order="F")
+    _cleanup(wd, own)
+    return out
+
+
+def _cleanup(wd, own):
     if own:
         for fn in os.listdir(wd):
             os.remove(os.path.join(wd, fn))
         os.rmdir(wd)
-    return out
