@@ -176,36 +176,31 @@ struct SysScal {
   int pad[3];
 };
 
-// Block -> (system, tile group) mapping, XCD-aware.  nblk = nact * G with G a
-// multiple of 8.  Block b runs on XCD b % 8; on each XCD consecutive blocks
-// cover all systems of one tile group.
+// Block -> (system, row chunk) mapping, XCD-aware.  nblk = nact * G with G a
+// multiple of 8.  Block b is dispatched to XCD b % 8.  Each XCD owns a
+// contiguous eighth of the rows (chunks xcd*G/8 .. (xcd+1)*G/8-1), so the
+// neighbouring lattice planes a stencil row reads sit in the same XCD's L2,
+// and consecutive blocks on one XCD cover all systems of one chunk, so the
+// shared stencil coefficients are fetched once per XCD.
 __device__ __forceinline__ void map_sys_block(int nact, int G, int& si, int& g) {
   const int b = blockIdx.x;
   const int xcd = b & 7, w = b >> 3;
   si = w % nact;
-  g = (w / nact) * 8 + xcd;
-  (void)G;
+  g = xcd * (G >> 3) + w / nact;
 }
 
-// Last-block-done: every block stores its partials then bumps the counter;
-// the block that sees count == nblk-1 reduces all partials in block order.
-// The counter is reset by that block, so no memset is ever needed (graph safe).
-__device__ __forceinline__ bool last_block(unsigned* counter, unsigned nblk) {
-  __shared__ bool amLast;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    amLast = (prev == nblk - 1);
-    if (amLast) {
-      __threadfence();
-      atomicExch(counter, 0u);
-    }
-  }
-  __syncthreads();
-  if (amLast) __threadfence();
-  return amLast;
+// Contiguous row range [lo, hi) of chunk g out of G over n rows (256-aligned).
+__device__ __forceinline__ void chunk_range(int64_t n, int G, int g, int64_t& lo, int64_t& hi) {
+  const int64_t per = ((n + G - 1) / G + 255) & ~int64_t(255);
+  lo = per * g;
+  hi = lo + per < n ? lo + per : n;
 }
+
+// Cross-block reductions never communicate inside a kernel: each block
+// stores its partial sums, and a one-block finalize kernel launched next on
+// the stream reduces them in fixed order.  (An in-kernel last-block scheme
+// needs a device-scope fence per block, which on gfx950 writes back the
+// whole L2 of the XCD each time.)
 
 // Reduce partials[(si * G + g) * NV + q] over g for every system, in the last
 // block, in fixed g order (deterministic); result to red[si * NV + q] (LDS).

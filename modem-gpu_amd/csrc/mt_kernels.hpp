@@ -1,21 +1,22 @@
 // BiCGStab, operator and preconditioner kernels of libmtsp2.
 //
 // One BiCGStab iteration (right-preconditioned, conjugated inner products:
-// the algorithm of cuBiCG, solver.f90:1548-1902) is 5 fused kernels plus the
-// preconditioner:
+// the algorithm of cuBiCG, solver.f90:1548-1902) is 5 fused streaming kernels,
+// 3 one-block finalize kernels and the preconditioner:
 //   k_pupdate     restart ? (rt = r, p = r) : p = r + beta (p - omega v)   [+ Jacobi]
 //   (precond)     ph = M^-1 p
-//   k_spmv_dot<1> v = A ph, <rt,v>      -> last block: alpha = rho / <rt,v>
+//   k_spmv_dot<1> v = A ph, partials of <rt,v>      -> k_finalize<FIN_ALPHA>
 //   k_supdate     x += alpha ph, s = r - alpha v  [+ best-iterate copy, Jacobi]
 //   (precond)     sh = M^-1 s
-//   k_spmv_dot<2> t = A sh, <t,t>, <t,s> -> last block: omega = <t,s>/<t,t>
-//   k_rupdate     x += omega sh, r = s - omega t, <rt,r>, <r,r>
-//                                        -> last block: norm, convergence,
-//                                           best iterate, next rho/beta
-//   k_recheck     (only for systems whose recursive residual met tol)
-//                 r = b - A x, true residual check (solver.f90:1855-1895)
-// All kernels take the active-system list and skip finished systems, so a
-// fixed chunk of iterations can be captured once in a hipGraph and replayed.
+//   k_spmv_dot<2> t = A sh, partials of <t,t>, <t,s> -> k_finalize<FIN_OMEGA>
+//   k_rupdate     x += omega sh, r = s - omega t, partials of <rt,r>, <r,r>
+//                                                    -> k_finalize<FIN_RUPDATE>
+//   k_recheck     only systems whose recursive residual met tol:
+//                 r = b - A x (solver.f90:1855-1895)  -> k_finalize<FIN_RECHECK>
+// The finalize kernels hold all solver scalar logic (convergence, breakdown,
+// best iterate, restart, next rho/beta) on the device, so nothing is read back
+// per dot.  Every kernel takes the active-system list and skips finished
+// systems, so a fixed chunk of iterations is captured once in a hipGraph.
 #pragma once
 
 #include "mt_device.hpp"
@@ -24,6 +25,7 @@ namespace mt {
 
 constexpr int BS = 256;
 enum : int { PCK_DILU = 0, PCK_JACOBI = 1, PCK_NONE = 2 };
+enum : int { FIN_INIT = 0, FIN_ALPHA = 1, FIN_OMEGA = 2, FIN_RUPDATE = 3, FIN_RECHECK = 4 };
 
 template <class T, class V>
 struct IterArgs {
@@ -37,25 +39,31 @@ struct IterArgs {
   C2<T>* s; C2<T>* sh; C2<T>* t; C2<T>* xbest;
   const C2<T>* b;
   const C2<T>* dinv;     // Jacobi inverse diagonal per frequency [nfreq][nE]
-  Z* partials;
-  unsigned* counter;
+  Z* partials;           // [nact][G][2]
   double* rerr;          // [nsys][rerr_len]
   int rerr_len;
+  double tol;
 };
-
-// i*ISIGN*omega*vms1*x  ==  (ISIGN*omega*vms1) * (i x)
-template <class T>
-__device__ __forceinline__ C2<T> diag_term(T d, C2<T> x) { return {-d * x.y, d * x.x}; }
 
 template <class T, class V>
 __device__ __forceinline__ C2<T> apply_row(const StencilView<V>& A, const RowInfo& ri,
                                            const C2<T>* __restrict__ x, T dscale) {
   C2<T> y = stencil_row<V, T>(A, ri, x);
+  // i*ISIGN*omega*vms1*x == (ISIGN*omega*vms1) * (i x)
   const T d = dscale * T(A.vms1[ri.e]);
   const C2<T> xe = x[ri.e];
   y.x += -d * xe.y;
   y.y += d * xe.x;
   return y;
+}
+
+__device__ __forceinline__ void store_partials(Z* partials, int si, int G, int g, Z (&acc)[2],
+                                               Z* lds) {
+  block_sum<2, BS>(acc, lds);
+  if (threadIdx.x == 0) {
+    partials[((int64_t)si * G + g) * 2 + 0] = acc[0];
+    partials[((int64_t)si * G + g) * 2 + 1] = acc[1];
+  }
 }
 
 // ------------------------------------------------------------ scalar logic
@@ -73,7 +81,7 @@ __device__ inline void prep_next(SysScal& sc, Z rt_r, Z r_r) {
   }
 }
 
-__device__ inline void finalize_rupdate(SysScal& sc, Z rt_r, Z r_r, double* rerr) {
+__device__ inline void fin_rupdate(SysScal& sc, Z rt_r, Z r_r, double* rerr) {
   const double rn = sqrt(r_r.x);
   const int it = ++sc.iter;
   if (rerr) rerr[it - 1] = rn / sc.bnorm;
@@ -87,7 +95,7 @@ __device__ inline void finalize_rupdate(SysScal& sc, Z rt_r, Z r_r, double* rerr
   prep_next(sc, rt_r, r_r);
 }
 
-__device__ inline void finalize_recheck(SysScal& sc, Z rt_r, Z r_r, double* rerr) {
+__device__ inline void fin_recheck(SysScal& sc, Z rt_r, Z r_r, double* rerr) {
   const double rn = sqrt(r_r.x);
   sc.rnorm = rn;
   if (rn < sc.btol) {
@@ -101,21 +109,78 @@ __device__ inline void finalize_recheck(SysScal& sc, Z rt_r, Z r_r, double* rerr
   prep_next(sc, rt_r, r_r);
 }
 
-// ------------------------------------------------------------------ kernels
-// r = b - A x, partials <r,r>, <b,b>; the last block sets up each system.
-template <class T, class V>
-__global__ __launch_bounds__(BS) void k_init(IterArgs<T, V> a, double tol) {
-  __shared__ Z lds[2 * BS / 64];
+__device__ inline void fin_init(SysScal& sc, Z rr, Z bb, double tol, double* rerr) {
+  const double rn = sqrt(rr.x), bn = sqrt(bb.x);
+  sc.bnorm = bn;
+  sc.btol = tol * bn;
+  sc.rnorm = rn;
+  sc.rnorm_start = rn;
+  sc.rnorm_best = rn;
+  sc.iter = 0;
+  sc.best_due = 0;
+  sc.alpha = {1.0, 0.0};
+  sc.omg = {1.0, 0.0};
+  sc.rho1 = {1.0, 0.0};
+  sc.rho = rr;
+  sc.restart_due = 1;
+  if (rerr) rerr[0] = (bn > 0.0) ? rn / bn : 0.0;
+  if (isnan(bn)) { sc.status = ST_BREAK; sc.iter = -1; return; }
+  if (bn == 0.0) { sc.status = ST_CONV; sc.iter = 1; return; }
+  if (rn <= sc.btol) { sc.status = ST_CONV; sc.iter = 1; return; }
+  sc.status = ST_RUN;
+}
+
+// One block: reduce the partials of the kernel just run, in fixed order, and
+// update the solver scalars of every active system.
+template <int KIND>
+__global__ __launch_bounds__(BS) void k_finalize(SysScal* scs, const int* act, int nact, int G,
+                                                 const Z* partials, double* rerr_base,
+                                                 int rerr_len, double tol) {
   __shared__ Z red[64 * 2];
+  reduce_partials<2>(partials, nact, G, red);
+  for (int q = threadIdx.x; q < nact; q += blockDim.x) {
+    const int s = act[q];
+    SysScal& sc = scs[s];
+    double* rerr = rerr_base ? rerr_base + (int64_t)s * rerr_len : nullptr;
+    const Z a0 = red[q * 2], a1 = red[q * 2 + 1];
+    if (KIND == FIN_INIT) {
+      fin_init(sc, a0, a1, tol, rerr);
+    } else if (KIND == FIN_ALPHA) {
+      if (sc.status != ST_RUN) continue;
+      sc.alpha = zdiv(sc.rho, a0);
+      if ((sc.alpha.x == 0.0 && sc.alpha.y == 0.0) || isnan(sc.alpha.x) || isnan(sc.alpha.y))
+        sc.status = ST_BREAK;
+    } else if (KIND == FIN_OMEGA) {
+      if (sc.status != ST_RUN) continue;
+      sc.omg = zdiv(a1, a0);
+      sc.best_due = 0;  // consumed by k_supdate of this iteration
+      if ((sc.omg.x == 0.0 && sc.omg.y == 0.0) || isnan(sc.omg.x) || isnan(sc.omg.y))
+        sc.status = ST_BREAK;
+    } else if (KIND == FIN_RUPDATE) {
+      if (sc.status != ST_RUN) continue;
+      fin_rupdate(sc, a0, a1, rerr);
+    } else if (KIND == FIN_RECHECK) {
+      if (sc.status != ST_RECHECK) continue;
+      fin_recheck(sc, a0, a1, rerr);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ kernels
+// r = b - A x, partials <r,r>, <b,b> (then FIN_INIT); xbest = x0.
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_init(IterArgs<T, V> a) {
+  __shared__ Z lds[2 * BS / 64];
   int si, g;
   map_sys_block(a.nact, a.G, si, g);
   const int sys = a.act[si];
   const int64_t o = (int64_t)sys * a.nE;
-  const SysScal& sc0 = a.sc[sys];
-  const T dsc = T(kIsignD * sc0.freq_omega);
+  const T dsc = T(kIsignD * a.sc[sys].freq_omega);
   Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
   const C2<T>* x = a.x + o;
-  for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+  int64_t lo, hi;
+  chunk_range(a.A.nInt, a.G, g, lo, hi);
+  for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
     const RowInfo ri = decode_row(a.A, rr);
     const C2<T> bv = a.b[o + ri.e];
     const C2<T> y = csub(bv, apply_row<T, V>(a.A, ri, x, dsc));
@@ -124,35 +189,7 @@ __global__ __launch_bounds__(BS) void k_init(IterArgs<T, V> a, double tol) {
     acc[0].x += double(y.x) * y.x + double(y.y) * y.y;
     acc[1].x += double(bv.x) * bv.x + double(bv.y) * bv.y;
   }
-  block_sum<2, BS>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
-    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
-  }
-  if (!last_block(a.counter, gridDim.x)) return;
-  reduce_partials<2>(a.partials, a.nact, a.G, red);
-  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
-    SysScal& sc = a.sc[a.act[q]];
-    const double rn = sqrt(red[q * 2].x), bn = sqrt(red[q * 2 + 1].x);
-    double* rerr = a.rerr ? a.rerr + (int64_t)a.act[q] * a.rerr_len : nullptr;
-    sc.bnorm = bn;
-    sc.btol = tol * bn;
-    sc.rnorm = rn;
-    sc.rnorm_start = rn;
-    sc.rnorm_best = rn;
-    sc.iter = 0;
-    sc.best_due = 0;
-    sc.alpha = {1.0, 0.0};
-    sc.omg = {1.0, 0.0};
-    sc.rho1 = {1.0, 0.0};
-    sc.rho = red[q * 2];
-    sc.restart_due = 1;
-    if (rerr) rerr[0] = (bn > 0.0) ? rn / bn : 0.0;
-    if (isnan(bn)) { sc.status = ST_BREAK; sc.iter = -1; continue; }
-    if (bn == 0.0) { sc.status = ST_CONV; sc.iter = 1; continue; }
-    if (rn <= sc.btol) { sc.status = ST_CONV; sc.iter = 1; continue; }
-    sc.status = ST_RUN;
-  }
+  store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
 template <class T, class V>
@@ -166,7 +203,9 @@ __global__ __launch_bounds__(BS) void k_pupdate(IterArgs<T, V> a) {
   const bool rs = sc.restart_due;
   const C2<T> beta = ccast<T>(sc.beta), om = ccast<T>(sc.omg);
   const C2<T>* dv = (a.pc == PCK_JACOBI) ? a.dinv + (int64_t)sc.fidx * a.nE : nullptr;
-  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+  int64_t lo, hi;
+  chunk_range(a.nE, a.G, g, lo, hi);
+  for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
     const C2<T> r = a.r[o + e];
     C2<T> p;
     if (rs) {
@@ -181,11 +220,10 @@ __global__ __launch_bounds__(BS) void k_pupdate(IterArgs<T, V> a) {
   }
 }
 
-// WHICH=1: v = A ph, <rt,v> -> alpha.   WHICH=2: t = A sh, <t,t>, <t,s> -> omega.
+// WHICH=1: v = A ph, <rt,v>.   WHICH=2: t = A sh, <t,t>, <t,s>.
 template <class T, class V, int WHICH>
 __global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
   __shared__ Z lds[2 * BS / 64];
-  __shared__ Z red[64 * 2];
   int si, g;
   map_sys_block(a.nact, a.G, si, g);
   const int sys = a.act[si];
@@ -196,7 +234,9 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
     const T dsc = T(kIsignD * sc0.freq_omega);
     const C2<T>* in = (WHICH == 1 ? a.ph : a.sh) + o;
     C2<T>* out = (WHICH == 1 ? a.v : a.t) + o;
-    for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+    int64_t lo, hi;
+    chunk_range(a.A.nInt, a.G, g, lo, hi);
+    for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
       const RowInfo ri = decode_row(a.A, rr);
       const C2<T> y = apply_row<T, V>(a.A, ri, in, dsc);
       out[ri.e] = y;
@@ -212,27 +252,7 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
       }
     }
   }
-  block_sum<2, BS>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
-    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
-  }
-  if (!last_block(a.counter, gridDim.x)) return;
-  reduce_partials<2>(a.partials, a.nact, a.G, red);
-  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
-    SysScal& sc = a.sc[a.act[q]];
-    if (sc.status != ST_RUN) continue;
-    if (WHICH == 1) {
-      sc.alpha = zdiv(sc.rho, red[q * 2]);
-      if (sc.alpha.x == 0.0 && sc.alpha.y == 0.0) sc.status = ST_BREAK;
-      if (isnan(sc.alpha.x) || isnan(sc.alpha.y)) sc.status = ST_BREAK;
-    } else {
-      sc.omg = zdiv(red[q * 2 + 1], red[q * 2]);
-      sc.best_due = 0;  // consumed by k_supdate of this iteration
-      if (sc.omg.x == 0.0 && sc.omg.y == 0.0) sc.status = ST_BREAK;
-      if (isnan(sc.omg.x) || isnan(sc.omg.y)) sc.status = ST_BREAK;
-    }
-  }
+  store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
 template <class T, class V>
@@ -246,7 +266,9 @@ __global__ __launch_bounds__(BS) void k_supdate(IterArgs<T, V> a) {
   const C2<T> al = ccast<T>(sc.alpha);
   const bool best = sc.best_due && a.xbest;
   const C2<T>* dv = (a.pc == PCK_JACOBI) ? a.dinv + (int64_t)sc.fidx * a.nE : nullptr;
-  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+  int64_t lo, hi;
+  chunk_range(a.nE, a.G, g, lo, hi);
+  for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
     const C2<T> xo = a.x[o + e];
     if (best) a.xbest[o + e] = xo;
     a.x[o + e] = cadd(xo, cmul(al, a.ph[o + e]));
@@ -260,7 +282,6 @@ __global__ __launch_bounds__(BS) void k_supdate(IterArgs<T, V> a) {
 template <class T, class V>
 __global__ __launch_bounds__(BS) void k_rupdate(IterArgs<T, V> a) {
   __shared__ Z lds[2 * BS / 64];
-  __shared__ Z red[64 * 2];
   int si, g;
   map_sys_block(a.nact, a.G, si, g);
   const int sys = a.act[si];
@@ -269,7 +290,9 @@ __global__ __launch_bounds__(BS) void k_rupdate(IterArgs<T, V> a) {
   if (sc0.status == ST_RUN) {
     const int64_t o = (int64_t)sys * a.nE;
     const C2<T> om = ccast<T>(sc0.omg);
-    for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+    int64_t lo, hi;
+    chunk_range(a.nE, a.G, g, lo, hi);
+    for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
       a.x[o + e] = cadd(a.x[o + e], cmul(om, a.sh[o + e]));
       const C2<T> r = csub(a.s[o + e], cmul(om, a.t[o + e]));
       a.r[o + e] = r;
@@ -279,26 +302,13 @@ __global__ __launch_bounds__(BS) void k_rupdate(IterArgs<T, V> a) {
       acc[1].x += double(r.x) * r.x + double(r.y) * r.y;
     }
   }
-  block_sum<2, BS>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
-    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
-  }
-  if (!last_block(a.counter, gridDim.x)) return;
-  reduce_partials<2>(a.partials, a.nact, a.G, red);
-  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
-    SysScal& sc = a.sc[a.act[q]];
-    if (sc.status != ST_RUN) continue;
-    double* rerr = a.rerr ? a.rerr + (int64_t)a.act[q] * a.rerr_len : nullptr;
-    finalize_rupdate(sc, red[q * 2], red[q * 2 + 1], rerr);
-  }
+  store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
 // True-residual recheck for systems in ST_RECHECK (no-op for the others).
 template <class T, class V>
 __global__ __launch_bounds__(BS) void k_recheck(IterArgs<T, V> a) {
   __shared__ Z lds[2 * BS / 64];
-  __shared__ Z red[64 * 2];
   int si, g;
   map_sys_block(a.nact, a.G, si, g);
   const int sys = a.act[si];
@@ -308,7 +318,9 @@ __global__ __launch_bounds__(BS) void k_recheck(IterArgs<T, V> a) {
     const int64_t o = (int64_t)sys * a.nE;
     const T dsc = T(kIsignD * sc0.freq_omega);
     const C2<T>* x = a.x + o;
-    for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+    int64_t lo, hi;
+    chunk_range(a.A.nInt, a.G, g, lo, hi);
+    for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
       const RowInfo ri = decode_row(a.A, rr);
       const C2<T> y = csub(a.b[o + ri.e], apply_row<T, V>(a.A, ri, x, dsc));
       a.r[o + ri.e] = y;
@@ -318,23 +330,11 @@ __global__ __launch_bounds__(BS) void k_recheck(IterArgs<T, V> a) {
       acc[1].x += double(y.x) * y.x + double(y.y) * y.y;
     }
   }
-  block_sum<2, BS>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
-    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
-  }
-  if (!last_block(a.counter, gridDim.x)) return;
-  reduce_partials<2>(a.partials, a.nact, a.G, red);
-  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
-    SysScal& sc = a.sc[a.act[q]];
-    if (sc.status != ST_RECHECK) continue;
-    double* rerr = a.rerr ? a.rerr + (int64_t)a.act[q] * a.rerr_len : nullptr;
-    finalize_recheck(sc, red[q * 2], red[q * 2 + 1], rerr);
-  }
+  store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
 // ---------------------------------------------------------- plain operator
-// y = A(omega) x for one system (mt_apply, tests, the mixed outer residual).
+// y = A(omega) x for one system (mt_apply, tests).
 template <class T, class V>
 __global__ __launch_bounds__(BS) void k_apply(StencilView<V> A, const C2<T>* x, C2<T>* y,
                                               double omega) {
@@ -379,10 +379,6 @@ __global__ void k_jacobi_dinv(StencilView<V> A, const double* omegas, int nfreq,
   }
 }
 
-}  // namespace mt
-
-namespace mt {
-
 // ------------------------------------------------ mixed precision (outer)
 // FP64 defect correction around the FP32 BiCGStab: the outer record `so`
 // holds the FP64 state of each system, the inner record `si` the FP32 solve
@@ -399,50 +395,32 @@ struct OuterArgs {
   C2<float>* r32;
   C2<float>* d32;
   Z* partials;
-  unsigned* counter;
   double tol, inner_tol;
   int max_outer, maxit_total;
   double* rerr;           // outer relative residual history [nsys][rerr_len]
   int rerr_len;
 };
 
-// <b,b> -> bnorm, btol, FP32 scaling (a power of two near 1/||b||)
+// partials of <b,b> (then k_outer_fin<0>)
 __global__ __launch_bounds__(BS) void k_outer_bnorm(OuterArgs a) {
   __shared__ Z lds[2 * BS / 64];
-  __shared__ Z red[64 * 2];
   int si, g;
   map_sys_block(a.nact, a.G, si, g);
   const int sys = a.act[si];
   const int64_t o = (int64_t)sys * a.nE;
   Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+  int64_t lo, hi;
+  chunk_range(a.nE, a.G, g, lo, hi);
+  for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
     const Z bv = a.b[o + e];
     acc[0].x += bv.x * bv.x + bv.y * bv.y;
   }
-  block_sum<2, BS>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
-    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
-  }
-  if (!last_block(a.counter, gridDim.x)) return;
-  reduce_partials<2>(a.partials, a.nact, a.G, red);
-  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
-    SysScal& so = a.so[a.act[q]];
-    const double bn = sqrt(red[q * 2].x);
-    so.bnorm = bn;
-    so.btol = a.tol * bn;
-    so.scale = (bn > 0.0 && isfinite(bn)) ? ldexp(1.0, -ilogb(bn)) : 1.0;
-    so.iter = 0;
-    so.tot_inner = 0;
-    so.status = isnan(bn) ? ST_BREAK : (bn == 0.0 ? ST_CONV : ST_RUN);
-    if (isnan(bn)) so.iter = -1;
-  }
+  store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
-// r = b - A x (FP64) -> r32 = scale*r, d32 = 0; <r,r> -> convergence, inner set-up
+// r = b - A x (FP64) -> r32 = scale*r, d32 = 0; partials of <r,r> (then k_outer_fin<1>)
 __global__ __launch_bounds__(BS) void k_outer_resid(OuterArgs a) {
   __shared__ Z lds[2 * BS / 64];
-  __shared__ Z red[64 * 2];
   int si, g;
   map_sys_block(a.nact, a.G, si, g);
   const int sys = a.act[si];
@@ -451,7 +429,9 @@ __global__ __launch_bounds__(BS) void k_outer_resid(OuterArgs a) {
   if (so0.status == ST_RUN) {
     const int64_t o = (int64_t)sys * a.nE;
     const double dsc = kIsignD * so0.freq_omega, sc = so0.scale;
-    for (int64_t rr = (int64_t)g * BS + threadIdx.x; rr < a.A.nInt; rr += (int64_t)a.G * BS) {
+    int64_t lo, hi;
+    chunk_range(a.A.nInt, a.G, g, lo, hi);
+    for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
       const RowInfo ri = decode_row(a.A, rr);
       const Z y = csub(a.b[o + ri.e], apply_row<double, double>(a.A, ri, a.x + o, dsc));
       a.r32[o + ri.e] = {float(sc * y.x), float(sc * y.y)};
@@ -459,16 +439,29 @@ __global__ __launch_bounds__(BS) void k_outer_resid(OuterArgs a) {
       acc[0].x += y.x * y.x + y.y * y.y;
     }
   }
-  block_sum<2, BS>(acc, lds);
-  if (threadIdx.x == 0) {
-    a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[0];
-    a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[1];
-  }
-  if (!last_block(a.counter, gridDim.x)) return;
+  store_partials(a.partials, si, a.G, g, acc, lds);
+}
+
+// STAGE 0: bnorm, btol, FP32 scaling (a power of two near 1/||b||).
+// STAGE 1: outer convergence and, for systems still refining, the inner set-up.
+template <int STAGE>
+__global__ __launch_bounds__(BS) void k_outer_fin(OuterArgs a) {
+  __shared__ Z red[64 * 2];
   reduce_partials<2>(a.partials, a.nact, a.G, red);
   for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
     const int s = a.act[q];
     SysScal& so = a.so[s];
+    if (STAGE == 0) {
+      const double bn = sqrt(red[q * 2].x);
+      so.bnorm = bn;
+      so.btol = a.tol * bn;
+      so.scale = (bn > 0.0 && isfinite(bn)) ? ldexp(1.0, -ilogb(bn)) : 1.0;
+      so.iter = 0;
+      so.tot_inner = 0;
+      so.status = isnan(bn) ? ST_BREAK : (bn == 0.0 ? ST_CONV : ST_RUN);
+      if (isnan(bn)) so.iter = -1;
+      continue;
+    }
     SysScal& sn = a.si[s];
     if (so.status != ST_RUN) { sn.status = ST_CONV; continue; }
     const double rn = sqrt(red[q * 2].x);
@@ -512,7 +505,9 @@ __global__ __launch_bounds__(BS) void k_outer_update(OuterArgs a) {
   if (so.status != ST_RUN) return;
   const int64_t o = (int64_t)sys * a.nE;
   const double inv = 1.0 / so.scale;
-  for (int64_t e = (int64_t)g * BS + threadIdx.x; e < a.nE; e += (int64_t)a.G * BS) {
+  int64_t lo, hi;
+  chunk_range(a.nE, a.G, g, lo, hi);
+  for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
     const C2<float> d = a.d32[o + e];
     Z x = a.x[o + e];
     x.x += inv * double(d.x);

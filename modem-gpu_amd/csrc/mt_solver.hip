@@ -337,11 +337,17 @@ struct Engine {
     k_pupdate<T, V><<<nblk, BS, 0, s>>>(a);
     if (a.pc == PCK_DILU) dilu_apply<T, V>(ctx, transpose, a.act, a.nact, a.sc, a.p, a.ph, a.dinv, ST_RUN);
     k_spmv_dot<T, V, 1><<<nblk, BS, 0, s>>>(a);
+    k_finalize<FIN_ALPHA><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     k_supdate<T, V><<<nblk, BS, 0, s>>>(a);
     if (a.pc == PCK_DILU) dilu_apply<T, V>(ctx, transpose, a.act, a.nact, a.sc, a.s, a.sh, a.dinv, ST_RUN);
     k_spmv_dot<T, V, 2><<<nblk, BS, 0, s>>>(a);
+    k_finalize<FIN_OMEGA><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     k_rupdate<T, V><<<nblk, BS, 0, s>>>(a);
-    if (recheck) k_recheck<T, V><<<nblk, BS, 0, s>>>(a);
+    k_finalize<FIN_RUPDATE><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    if (recheck) {
+      k_recheck<T, V><<<nblk, BS, 0, s>>>(a);
+      k_finalize<FIN_RECHECK><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    }
   }
 
   int run_chunk(int chunk) {
@@ -453,8 +459,7 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   SysScal* d_sc;
   int* d_act;
   if ((rc = ws(ctx, "f64_sc", nsys, &d_sc)) || (rc = ws(ctx, "act", nsys, &d_act)) ||
-      (rc = ws(ctx, "partials", size_t(nsys) * Gmax * 2 + 64, &a.partials)) ||
-      (rc = ws(ctx, "counter", 16, &a.counter)))
+      (rc = ws(ctx, "partials", size_t(nsys) * Gmax * 2 + 64, &a.partials)))
     return rc;
   const int rlen = o.maxit + 1;
   double* d_rerr = nullptr;
@@ -479,7 +484,10 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   a.act = d_act;
   a.nact = nsys;
   a.G = pick_G(ctx, nsys);
-  k_init<double, double><<<a.nact * a.G, BS, 0, ctx->stream>>>(a, o.tol);
+  a.tol = o.tol;
+  k_init<double, double><<<a.nact * a.G, BS, 0, ctx->stream>>>(a);
+  k_finalize<FIN_INIT><<<1, BS, 0, ctx->stream>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr,
+                                                  a.rerr_len, a.tol);
   MT_HIP_CHECK(ctx, hipGetLastError());
   if ((rc = read_scal(ctx, d_sc, nsys, h))) return rc;
   const int chunk = std::max(1, o.chunk);
@@ -552,10 +560,10 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   int* d_act;
   if ((rc = ws(ctx, "mx_so", nsys, &d_so)) || (rc = ws(ctx, "mx_si", nsys, &d_si)) ||
       (rc = ws(ctx, "act", nsys, &d_act)) ||
-      (rc = ws(ctx, "partials", size_t(nsys) * Gmax * 2 + 64, &a.partials)) ||
-      (rc = ws(ctx, "counter", 16, &a.counter)))
+      (rc = ws(ctx, "partials", size_t(nsys) * Gmax * 2 + 64, &a.partials)))
     return rc;
   a.rerr = nullptr;
+  a.tol = 0.0;
   a.rerr_len = 0;
   a.sc = d_si;
   a.act = d_act;
@@ -570,7 +578,6 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   oa.r32 = a.r;
   oa.d32 = d32;
   oa.partials = a.partials;
-  oa.counter = a.counter;
   oa.tol = o.tol;
   oa.inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
   oa.max_outer = o.max_outer > 0 ? o.max_outer : 30;
@@ -600,6 +607,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   oa.nact = nsys;
   oa.G = pick_G(ctx, nsys);
   k_outer_bnorm<<<oa.nact * oa.G, BS, 0, ctx->stream>>>(oa);
+  k_outer_fin<0><<<1, BS, 0, ctx->stream>>>(oa);
   MT_HIP_CHECK(ctx, hipGetLastError());
   if ((rc = read_scal(ctx, d_so, nsys, ho))) return rc;
   const int chunk = std::max(1, o.chunk);
@@ -610,6 +618,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     oa.nact = n;
     oa.G = pick_G(ctx, n);
     k_outer_resid<<<oa.nact * oa.G, BS, 0, ctx->stream>>>(oa);
+    k_outer_fin<1><<<1, BS, 0, ctx->stream>>>(oa);
     MT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = read_scal(ctx, d_so, nsys, ho))) return rc;
     std::vector<int> outer_act;
@@ -1111,10 +1120,8 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   SysScal* d_sc;
   int* d_act;
   Z* partials;
-  unsigned* counter;
   if ((rc = ws(ctx, "bn_sc", nsys, &d_sc)) || (rc = ws(ctx, "bn_act", nsys, &d_act)) ||
-      (rc = ws(ctx, "partials", size_t(nsys) * pick_G(ctx, 1) * 2 + 64, &partials)) ||
-      (rc = ws(ctx, "counter", 16, &counter)))
+      (rc = ws(ctx, "partials", size_t(nsys) * pick_G(ctx, 1) * 2 + 64, &partials)))
     return rc;
   MT_HIP_CHECK(ctx, hipMemcpyAsync(d_sc, h.data(), nsys * sizeof(SysScal), hipMemcpyHostToDevice, ctx->stream));
   MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, act.data(), nsys * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -1131,7 +1138,7 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
     std::memset(&a, 0, sizeof(a));
     a.A = view<T>(ctx, false);
     a.sc = d_sc; a.act = d_act; a.nact = nsys; a.G = G; a.pc = PCK_NONE; a.nE = g.nE;
-    a.partials = partials; a.counter = counter;
+    a.partials = partials;
     C2<T>* vin; C2<T>* vout; C2<T>* vrt;
     int r2;
     if ((r2 = ws(ctx, sizeof(T) == 8 ? "bn_in64" : "bn_in32", nv, &vin)) ||
@@ -1141,11 +1148,7 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
     a.ph = vin; a.v = vout; a.rt = vrt;
     k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);  // warm-up
     MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    for (int q = 0; q < reps; ++q) {
-      // restore RUN status clobbered by alpha == 0 breakdown on zero vectors
-      hipMemcpyAsync(d_sc, h.data(), nsys * sizeof(SysScal), hipMemcpyHostToDevice, ctx->stream);
-      k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);
-    }
+    for (int q = 0; q < reps; ++q) k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);
     MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     MT_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[1]));
     MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));

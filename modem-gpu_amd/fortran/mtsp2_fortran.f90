@@ -1,0 +1,266 @@
+! mtsp2_fortran -- ISO_C_BINDING interface of libmtsp2 for the reference's
+! Fortran host (ModEM SP2, f90/3D_MT/FWD_SP2).
+!
+! It plays the role cudaFortMap.f90 plays for the reference's CUDA path
+! (f90/3D_MT/FWD_SP2/cudaFortMap.f90:93-1408): bind(C) interfaces to the
+! C ABI of include/mtsp2.h, plus mtsp2_bicg, a subroutine with the argument
+! meaning of cuBiCG(b, x, KSPiter, device_idx) (solver.f90:931-965) that
+! takes the operator explicitly (AAii as the reference's 1-based CSR, the
+! frequency-free diagonal VomegaMuSig/omega and omega), keeps one library
+! context per device and re-uploads the operator only when it changes.
+module mtsp2_fortran
+   use iso_c_binding
+   implicit none
+   private
+
+   integer(c_int), parameter, public :: MT_OK = 0, MT_E_ARG = 1, MT_E_DEVICE = 2, &
+        MT_E_OOM = 3, MT_E_NAN_RHS = 4, MT_E_BREAKDOWN = 5, MT_E_MAXIT = 6, &
+        MT_E_STATE = 7, MT_E_STENCIL = 8
+   integer(c_int), parameter, public :: MT_PC_DILU = 0, MT_PC_JACOBI = 1, MT_PC_NONE = 2
+   integer(c_int), parameter, public :: MT_PREC_FP64 = 0, MT_PREC_MIXED = 1
+
+   type, bind(C), public :: mt_solve_opts
+      integer(c_int) :: maxit
+      real(c_double) :: tol
+      integer(c_int) :: precond, precision, restart, best_iterate
+      real(c_double) :: inner_tol
+      integer(c_int) :: max_outer, op_transpose, chunk, use_graph
+   end type
+
+   type, bind(C), public :: mt_sys_stats
+      integer(c_int) :: niter, nouter, status
+      real(c_double) :: relres
+   end type
+
+   public :: mt_default_opts, mt_ctx_create, mt_ctx_destroy, mt_device_count, mt_set_grid, &
+             mt_operator_set_csr, mt_bicg, mt_solve_batch, mt_apply, mt_precond_apply, &
+             mt_divcor, mt_pmultT, mt_last_error_string
+   public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release
+
+   interface
+      subroutine mt_default_opts(opts) bind(C, name='mt_default_opts')
+         import :: mt_solve_opts
+         type(mt_solve_opts), intent(out) :: opts
+      end subroutine
+      integer(c_int) function mt_ctx_create(device, ctx) bind(C, name='mt_ctx_create')
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr), intent(out) :: ctx
+      end function
+      integer(c_int) function mt_ctx_destroy(ctx) bind(C, name='mt_ctx_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      type(c_ptr) function mt_last_error(ctx) bind(C, name='mt_last_error')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end function
+      integer(c_int) function mt_device_count(n) bind(C, name='mt_device_count')
+         import :: c_int
+         integer(c_int), intent(out) :: n
+      end function
+      integer(c_int) function mt_set_grid(ctx, nx, ny, nz) bind(C, name='mt_set_grid')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nx, ny, nz
+      end function
+      integer(c_int) function mt_operator_set_csr(ctx, n, nnz, rowptr, col, base, val, vms1) &
+            bind(C, name='mt_operator_set_csr')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n, nnz, base
+         integer(c_int), intent(in) :: rowptr(*), col(*)
+         real(c_double), intent(in) :: val(*), vms1(*)
+      end function
+      integer(c_int) function mt_bicg(ctx, omega, b, x, maxit, tol, precond, niter, rerr, failed) &
+            bind(C, name='mt_bicg')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: omega, tol
+         complex(c_double_complex), intent(in) :: b(*)
+         complex(c_double_complex), intent(inout) :: x(*)
+         integer(c_int), value :: maxit, precond
+         integer(c_int), intent(out) :: niter, failed
+         real(c_double), intent(out) :: rerr(*)
+      end function
+      integer(c_int) function mt_solve_batch(ctx, nsys, omegas, b, x, opts, stats) &
+            bind(C, name='mt_solve_batch')
+         import :: c_int, c_ptr, c_double, c_double_complex, mt_solve_opts, mt_sys_stats
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nsys
+         real(c_double), intent(in) :: omegas(*)
+         complex(c_double_complex), intent(in) :: b(*)
+         complex(c_double_complex), intent(inout) :: x(*)
+         type(mt_solve_opts), intent(in) :: opts
+         type(mt_sys_stats), intent(out) :: stats(*)
+      end function
+      integer(c_int) function mt_apply(ctx, omega, op, x, y) bind(C, name='mt_apply')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: omega
+         integer(c_int), value :: op
+         complex(c_double_complex), intent(in) :: x(*)
+         complex(c_double_complex), intent(out) :: y(*)
+      end function
+      integer(c_int) function mt_precond_apply(ctx, omega, precond, x, y) &
+            bind(C, name='mt_precond_apply')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: omega
+         integer(c_int), value :: precond
+         complex(c_double_complex), intent(in) :: x(*)
+         complex(c_double_complex), intent(out) :: y(*)
+      end function
+      integer(c_int) function mt_divcor(ctx, omega, ei, phi0, maxit, tol, niter, relres) &
+            bind(C, name='mt_divcor')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: omega, tol
+         complex(c_double_complex), intent(inout) :: ei(*)
+         type(c_ptr), value :: phi0
+         integer(c_int), value :: maxit
+         integer(c_int), intent(out) :: niter
+         real(c_double), intent(out) :: relres
+      end function
+      integer(c_int) function mt_pmultT(ctx, omega, npol, e0, lam, dm, acc) bind(C, name='mt_pmultT')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: omega
+         integer(c_int), value :: npol, acc
+         complex(c_double_complex), intent(in) :: e0(*), lam(*)
+         real(c_double), intent(inout) :: dm(*)
+      end function
+      integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
+         import :: c_size_t, c_ptr
+         type(c_ptr), value :: s
+      end function
+   end interface
+
+   ! one context per device, with the identity of the operator last uploaded
+   integer, parameter :: MAXDEV = 16
+   type(c_ptr), save :: ctxs(0:MAXDEV-1) = c_null_ptr
+   integer(c_int64_t), save :: op_key(3, 0:MAXDEV-1) = 0
+
+contains
+
+   function mt_last_error_string(ctx) result(msg)
+      type(c_ptr), intent(in) :: ctx
+      character(len=:), allocatable :: msg
+      type(c_ptr) :: p
+      character(kind=c_char), pointer :: s(:)
+      integer :: i, n
+      p = mt_last_error(ctx)
+      n = int(c_strlen(p))
+      call c_f_pointer(p, s, [n])
+      allocate(character(len=n) :: msg)
+      do i = 1, n
+         msg(i:i) = s(i)
+      enddo
+   end function
+
+   ! Context for device_idx with the operator (AAii, vms1) uploaded.  The
+   ! operator is re-uploaded when its size or contents change (a new sigma).
+   subroutine get_ctx(device_idx, nx, ny, nz, row, col, val, vms1, ctx, ierr)
+      integer, intent(in)                 :: device_idx, nx, ny, nz
+      integer(c_int), intent(in)          :: row(:), col(:)
+      real(c_double), intent(in)          :: val(:), vms1(:)
+      type(c_ptr), intent(out)            :: ctx
+      integer, intent(out)                :: ierr
+      integer(c_int64_t) :: key(3)
+      integer(c_int) :: rc
+      real(c_double) :: h
+      integer :: i
+      ierr = MT_OK
+      if (device_idx < 0 .or. device_idx >= MAXDEV) then
+         ierr = MT_E_DEVICE
+         return
+      endif
+      if (.not. c_associated(ctxs(device_idx))) then
+         rc = mt_ctx_create(int(device_idx, c_int), ctxs(device_idx))
+         if (rc /= MT_OK) then
+            ierr = rc
+            return
+         endif
+      endif
+      ctx = ctxs(device_idx)
+      h = 0.0d0
+      do i = 1, size(val)
+         h = h + val(i) * dble(mod(i, 9973) + 1)
+      enddo
+      do i = 1, size(vms1)
+         h = h + vms1(i) * dble(mod(i, 7919) + 1)
+      enddo
+      key(1) = size(val)
+      key(2) = transfer(h, key(2))
+      key(3) = int(nx, c_int64_t) + 4096_c_int64_t * (int(ny, c_int64_t) + 4096_c_int64_t * nz)
+      if (all(key == op_key(:, device_idx))) return
+      rc = mt_set_grid(ctx, int(nx, c_int), int(ny, c_int), int(nz, c_int))
+      if (rc == MT_OK) rc = mt_operator_set_csr(ctx, int(size(row) - 1, c_int), int(size(col), c_int), &
+                                               row, col, 1_c_int, val, vms1)
+      if (rc /= MT_OK) then
+         ierr = rc
+         op_key(:, device_idx) = 0
+         return
+      endif
+      op_key(:, device_idx) = key
+   end subroutine
+
+   ! Drop-in for cuBiCG (solver.f90:931-2009).  Arguments follow solverControl_t
+   ! (solver.f90:25-36): on entry maxIt/tol, on exit niter, rerr(1:maxIt),
+   ! failed.  x holds x0 on entry and the solution (or, when not converged,
+   ! the smallest-residual iterate) on exit.  ierr /= 0 replaces the
+   ! reference's `stop` on device errors.
+   subroutine mtsp2_bicg(device_idx, nx, ny, nz, row, col, val, vms1, omega, b, x, &
+                         maxIt, tol, niter, rerr, failed, ierr, precond)
+      integer, intent(in)                     :: device_idx, nx, ny, nz, maxIt
+      integer(c_int), intent(in)              :: row(:), col(:)
+      real(c_double), intent(in)              :: val(:), vms1(:), omega, tol
+      complex(c_double_complex), intent(in)   :: b(:)
+      complex(c_double_complex), intent(inout):: x(:)
+      integer, intent(out)                    :: niter, ierr
+      real(c_double), intent(out)             :: rerr(:)
+      logical, intent(out)                    :: failed
+      integer, intent(in), optional           :: precond
+      type(c_ptr) :: ctx
+      integer(c_int) :: rc, nit, fl, pc
+      call get_ctx(device_idx, nx, ny, nz, row, col, val, vms1, ctx, ierr)
+      if (ierr /= MT_OK) return
+      pc = MT_PC_DILU
+      if (present(precond)) pc = int(precond, c_int)
+      rc = mt_bicg(ctx, omega, b, x, int(maxIt, c_int), tol, pc, nit, rerr, fl)
+      ierr = rc
+      niter = nit
+      failed = (fl /= 0)
+   end subroutine
+
+   ! Both polarizations and many frequencies at once (nsys systems sharing
+   ! AAii): b, x are (n_int, nsys).
+   subroutine mtsp2_solve_batch(device_idx, nx, ny, nz, row, col, val, vms1, omegas, b, x, &
+                                opts, stats, ierr)
+      integer, intent(in)                     :: device_idx, nx, ny, nz
+      integer(c_int), intent(in)              :: row(:), col(:)
+      real(c_double), intent(in)              :: val(:), vms1(:), omegas(:)
+      complex(c_double_complex), intent(in)   :: b(:, :)
+      complex(c_double_complex), intent(inout):: x(:, :)
+      type(mt_solve_opts), intent(in)         :: opts
+      type(mt_sys_stats), intent(out)         :: stats(:)
+      integer, intent(out)                    :: ierr
+      type(c_ptr) :: ctx
+      call get_ctx(device_idx, nx, ny, nz, row, col, val, vms1, ctx, ierr)
+      if (ierr /= MT_OK) return
+      ierr = mt_solve_batch(ctx, int(size(omegas), c_int), omegas, b, x, opts, stats)
+   end subroutine
+
+   subroutine mtsp2_release()
+      integer :: d
+      integer(c_int) :: rc
+      do d = 0, MAXDEV - 1
+         if (c_associated(ctxs(d))) then
+            rc = mt_ctx_destroy(ctxs(d))
+            ctxs(d) = c_null_ptr
+            op_key(:, d) = 0
+         endif
+      enddo
+   end subroutine
+
+end module mtsp2_fortran

@@ -130,7 +130,9 @@ def plane_wave(grid, omega, pol, rho=100.0):
     nXe, nYe, nZe = edge_counts(nx, ny, nz)
     zn = node_depths(grid)
     k = np.sqrt(1j * omega * MU0 / rho)
-    ez = np.where(zn >= 0, np.exp(1j * k * zn), 1 + 1j * k * zn)
+    ez = 1 + 1j * k * zn
+    below = zn >= 0
+    ez[below] = np.exp(1j * k * zn[below])
     e = np.zeros(nXe + nYe + nZe, np.complex128)
     if pol == 0:
         e[:nXe] = np.broadcast_to(ez[:, None, None], (nz + 1, ny + 1, nx)).ravel()
