@@ -187,6 +187,28 @@ int mt_divcor(mt_ctx *ctx, double omega, mt_zdouble *ei, const mt_zdouble *phi0,
 int mt_pmultT(mt_ctx *ctx, double omega, int npol, const mt_zdouble *e0_all,
               const mt_zdouble *lam_all, double *dm, int accumulate);
 
+/* Gradient of Re(sum_pol lam_pol^T A(omega) e_pol) w.r.t. the LOGE cell
+ * parameters (nx*ny*nzEarth, Fortran order), accumulated into dm.  For a
+ * misfit phi = 1/2 ||W (d - P e)||^2 with lam solving A^T lam = P^T conj(W^2
+ * (d - P e)) this is d phi / d m.  e_all: forward fields on all edges
+ * (npol*n_edge); lam: adjoint fields on interior edges (npol*n_int).
+ * mode MT_GRAD_REFERENCE: only the i*ISIGN*omega*mu0*sigma_E*V_E diagonal, the
+ *   term the reference's PmultT differentiates (SolverSens.f90:63-116);
+ * mode MT_GRAD_EXACT: plus the grad-div part of AAii through sigma_E and
+ *   sigma_N (GradDivSetup2, modelOperator3D.f90:916-1008): the exact discrete
+ *   adjoint of the CC-GD system, which central differences confirm.
+ * Has no counterpart entry point in the reference (its JmultT_TX,
+ * SensComp.f90:408-485, chains PmultT and dEdgeToModelParam in Fortran). */
+#define MT_GRAD_REFERENCE 0
+#define MT_GRAD_EXACT 1
+int mt_gradient(mt_ctx *ctx, double omega, int npol, const mt_zdouble *e_all,
+                const mt_zdouble *lam, double *dm, int mode);
+/* Host-only variant (assembles the model itself; for small problems/tests). */
+int mt_host_gradient(int nx, int ny, int nzAir, int nzEarth, const double *dx, const double *dy,
+                     const double *dz, const double *logcond, double air_logcond, double omega,
+                     int npol, const mt_zdouble *e_all, const mt_zdouble *lam, double *dm,
+                     int mode);
+
 /* ---- tuning / introspection ------------------------------------------------ */
 /* Runs the SpMV autotuner for nsys systems: times the solver's SpMV kernel for
  * several row-tile group counts per system and keeps the fastest; *variant

@@ -34,38 +34,29 @@ struct TileShape {
   unsigned int pts[kMaxTilePts];  // packed (li | lj<<8 | lk<<16) sorted by li+lj+lk
 };
 
+struct LatticeInfo {
+  int A, B, C;        // interior lattice dims
+  int oi, oj, ok;     // offset of the interior lattice in its full lattice
+  int a, b;           // full-lattice dims (i, j) for addressing
+  int64_t base;       // address of the full lattice's first point in the vector
+};
+
 template <class T, class V>
 struct DiluArgs {
-  const V* coef;
-  int64_t nE;
-  int nx, ny, nz;
-  int64_t nXe, nYe;
-  const int4* tiles;       // tiles of this level: {comp, ti, tj, tk}
+  const V* coef;           // [7 or 15][cstride]: slots 1..6 = a-1, a+1, b-1, b+1, c-1, c+1
+  int64_t cstride;         // coefficient slot stride (= per-frequency pivot stride)
+  int64_t vstride;         // per-system vector stride
+  LatticeInfo lat[3];      // the independent blocks (3 edge components, or 1 node lattice)
+  const int4* tiles;       // tiles of this level: {lattice, ti, tj, tk}
   int ntiles;
   const int* act;
   int nact;
   const SysScal* sc;
   const C2<T>* in;         // forward: right-hand side
   C2<T>* out;              // forward: y; backward: in place y -> z
-  const C2<T>* dinv;       // [nfreq][nE]
+  const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed
 };
-
-struct LatticeInfo {
-  int A, B, C;        // interior lattice dims
-  int oi, oj, ok;     // offset of interior lattice in edge lattice
-  int a, b;           // edge-lattice dims (i, j) for addressing
-  int64_t base;
-};
-
-__device__ __forceinline__ LatticeInfo lattice_of(int c, int nx, int ny, int nz, int64_t nXe,
-                                                  int64_t nYe) {
-  LatticeInfo L;
-  if (c == 0) { L.A = nx; L.B = ny - 1; L.C = nz - 1; L.oi = 0; L.oj = 1; L.ok = 1; L.a = nx; L.b = ny + 1; L.base = 0; }
-  else if (c == 1) { L.A = nx - 1; L.B = ny; L.C = nz - 1; L.oi = 1; L.oj = 0; L.ok = 1; L.a = nx + 1; L.b = ny; L.base = nXe; }
-  else { L.A = nx - 1; L.B = ny - 1; L.C = nz; L.oi = 1; L.oj = 1; L.ok = 0; L.a = nx + 1; L.b = ny + 1; L.base = nXe + nYe; }
-  return L;
-}
 
 __device__ __forceinline__ int64_t lat_e(const LatticeInfo& L, int i, int j, int k) {
   return L.base + (i + L.oi) + int64_t(L.a) * ((j + L.oj) + int64_t(L.b) * (k + L.ok));
@@ -85,11 +76,11 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
   const SysScal& sc = a.sc[sys];
   if (sc.status != a.status_req) return;
   const int4 tl = a.tiles[tix];
-  const LatticeInfo L = lattice_of(tl.x, a.nx, a.ny, a.nz, a.nXe, a.nYe);
+  const LatticeInfo L = a.lat[tl.x];
   const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
   const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
-  const int64_t o = (int64_t)sys * a.nE;
-  const C2<T>* dinv = a.dinv + (int64_t)sc.fidx * a.nE;
+  const int64_t o = (int64_t)sys * a.vstride;
+  const C2<T>* dinv = a.dinv + (int64_t)sc.fidx * a.cstride;
   const int s0 = FWD ? 1 : 2;  // lower slots 1,3,5 / upper slots 2,4,6
   // box coordinate b <-> local tile coordinate l: FWD: l = b - 1, BWD: l = b
   for (int q = threadIdx.x; q < NB; q += 64) {
@@ -106,7 +97,7 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
       if (inTile) {
         const int p = li + TX * (lj + TY * lk);
 #pragma unroll
-        for (int s = 0; s < 3; ++s) cf[s][p] = a.coef[(int64_t)(s0 + 2 * s) * a.nE + e];
+        for (int s = 0; s < 3; ++s) cf[s][p] = a.coef[(int64_t)(s0 + 2 * s) * a.cstride + e];
         if (!FWD) dv[p] = dd;
       }
     } else if (inTile) {

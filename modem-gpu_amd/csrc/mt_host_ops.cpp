@@ -160,6 +160,9 @@ Status assemble(HostOperator& op, int nx, int ny, int nzAir, int nzEarth, const 
   for (int64_t e = 0; e < nE; ++e) sigE[e] = numE[e] / denE[e];
   for (int64_t n = 0; n < nN; ++n) sigN[n] = numN[n] / denN[n];
   op.Vcell_ghost_edge = denE;
+  op.Vcell_ghost_node = denN;
+  op.sigN = sigN;
+  op.vnode = vnode;
   // UpdateCond zeroes sigma on boundary edges (modelOperator3D.f90:192)
   for (int e : op.bmap) sigE[e] = 0.0;
 
@@ -173,6 +176,9 @@ Status assemble(HostOperator& op, int nx, int ny, int nzAir, int nzEarth, const 
 
   op.vms1.assign(nE, 0.0);
   for (int e : op.imap) op.vms1[e] = kMu0 * sigE[e] * op.Vedge[e];
+  op.sigE = sigE;
+  op.dualA = dualA;
+  op.edgeL = edgeL;
 
   op.coef.assign(size_t(kSlots) * nE, 0.0);
   const SlotTable& ST = slots();
@@ -245,6 +251,8 @@ Status assemble(HostOperator& op, int nx, int ny, int nzAir, int nzEarth, const 
         bool earth = false;
         for (auto& x : ed) earth = earth || eearth[x.e];
         const int64_t n = nidx(i, j, k);
+        if (op.nodeEarth.empty()) op.nodeEarth.assign(nN, 0);
+        op.nodeEarth[n] = earth;
         // air part: DualFaceA G diag(1/Vnode) G^T DualFaceA
         // earth part: DualFaceA G diag(1/(sigN Vnode)) G^T diag(sigE DualFaceA)
         const double m2 = earth ? 1.0 / sigN[n] / vnode[n] : 1.0 / vnode[n];
@@ -408,6 +416,157 @@ void transpose_stencil(const HostOperator& op, std::vector<double>& coefT) {
       coefT[size_t(t) * nE + f] += v;
     }
   }
+}
+
+void divcor_node_stencil(const HostOperator& op, std::vector<double>& coefN,
+                         std::vector<double>& wE) {
+  const Grid& g = op.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int64_t nN = int64_t(nx + 1) * (ny + 1) * (nz + 1);
+  auto nidx = [&](int i, int j, int k) { return i + int64_t(nx + 1) * (j + int64_t(ny + 1) * k); };
+  std::vector<char> isInt(g.nE, 0);
+  for (int e : op.imap) isInt[e] = 1;
+  // weight of each interior edge in DivC: DualFaceA * sigma_E (VDs = VDiv diag(-sigma_E))
+  wE.assign(g.nE, 0.0);
+  for (int e : op.imap) wE[e] = op.dualA[e] * op.sigE[e];
+  coefN.assign(size_t(7) * nN, 0.0);
+  // edge e from node a to node b: Gai phi = (phi_b - phi_a)/l, VDs adds
+  // -G_en w_e (.) at both ends: row a gets +w/l (phi_b - phi_a), row b -w/l (phi_b - phi_a)
+  auto add_edge = [&](int64_t e, int64_t a, int64_t b, int sa, int sb, bool aInt, bool bInt) {
+    if (!isInt[e]) return;
+    const double c = wE[e] / op.edgeL[e];
+    if (aInt) {
+      coefN[a] -= c;                       // phi_a on row a
+      if (bInt) coefN[size_t(sa) * nN + a] += c;  // phi_b on row a
+    }
+    if (bInt) {
+      coefN[b] -= c;
+      if (aInt) coefN[size_t(sb) * nN + b] += c;
+    }
+  };
+  auto inner = [&](int i, int j, int k) {
+    return i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz;
+  };
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        if (i < nx) add_edge(g.eidx(0, i, j, k), nidx(i, j, k), nidx(i + 1, j, k), 2, 1, inner(i, j, k), inner(i + 1, j, k));
+        if (j < ny) add_edge(g.eidx(1, i, j, k), nidx(i, j, k), nidx(i, j + 1, k), 4, 3, inner(i, j, k), inner(i, j + 1, k));
+        if (k < nz) add_edge(g.eidx(2, i, j, k), nidx(i, j, k), nidx(i, j, k + 1), 6, 5, inner(i, j, k), inner(i, j, k + 1));
+      }
+}
+
+void divcor_inv_pivots(const HostOperator& op, const std::vector<double>& coefN,
+                       std::vector<double>& dinv) {
+  const Grid& g = op.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int64_t nN = int64_t(nx + 1) * (ny + 1) * (nz + 1);
+  dinv.assign(nN, 0.0);
+  const int64_t st[3] = {1, nx + 1, int64_t(nx + 1) * (ny + 1)};
+  for (int k = 1; k < nz; ++k)
+    for (int j = 1; j < ny; ++j)
+      for (int i = 1; i < nx; ++i) {
+        const int64_t n = i + st[1] * j + st[2] * k;
+        double p = coefN[n];
+        for (int q = 0; q < 3; ++q) {
+          const double alo = coefN[size_t(1 + 2 * q) * nN + n];
+          if (alo == 0.0) continue;
+          p -= alo * alo * dinv[n - st[q]];
+        }
+        dinv[n] = 1.0 / p;
+      }
+}
+
+void grad_assemble(const HostOperator& op, double omega, int npol, const zc* e_all,
+                   const zc* lam, int mode, double* dm) {
+  const Grid& g = op.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int64_t nE = g.nE, nN = int64_t(nx + 1) * (ny + 1) * (nz + 1);
+  auto nidx = [&](int i, int j, int k) { return i + int64_t(nx + 1) * (j + int64_t(ny + 1) * k); };
+  std::vector<double> tE(nE, 0.0), tN(nN, 0.0);
+  std::vector<zc> L(nE), E(nE), D(nN), S(nN);
+  const zc dcoef(0.0, kIsign * omega * kMu0);  // d A_ee / d sigma_e = i ISIGN w mu0 V_e
+  for (int pol = 0; pol < npol; ++pol) {
+    std::fill(L.begin(), L.end(), zc(0.0));
+    std::fill(E.begin(), E.end(), zc(0.0));
+    for (int64_t r = 0; r < g.nInt; ++r) {
+      const int e = op.imap[r];
+      L[e] = lam[size_t(pol) * g.nInt + r];
+      E[e] = e_all[size_t(pol) * nE + e];  // interior columns of AAii only
+    }
+    for (int e : op.imap) tE[e] += (dcoef * op.Vedge[e] * L[e] * E[e]).real();
+    if (mode == 0) continue;
+    // D_n = sum_a G_an A_a lam_a, S_n = sum_b G_bn sigma_b A_b e_b (interior edges)
+    std::fill(D.begin(), D.end(), zc(0.0));
+    std::fill(S.begin(), S.end(), zc(0.0));
+    for (int c = 0; c < 3; ++c) {
+      int a, b, d;
+      g.dims(c, a, b, d);
+      for (int k = 0; k < d; ++k)
+        for (int j = 0; j < b; ++j)
+          for (int i = 0; i < a; ++i) {
+            if (!g.interior(c, i, j, k)) continue;
+            const int64_t e = g.eidx(c, i, j, k);
+            const int64_t n0 = nidx(i, j, k);
+            const int64_t n1 = nidx(i + (c == 0), j + (c == 1), k + (c == 2));
+            const zc dl = op.dualA[e] * L[e], ds = op.sigE[e] * op.dualA[e] * E[e];
+            D[n0] -= dl; D[n1] += dl;
+            S[n0] -= ds; S[n1] += ds;
+          }
+    }
+    // node weights M2e = Nearth / (sigma_N Vnode) on interior earth nodes
+    for (int c = 0; c < 3; ++c) {
+      int a, b, d;
+      g.dims(c, a, b, d);
+      for (int k = 0; k < d; ++k)
+        for (int j = 0; j < b; ++j)
+          for (int i = 0; i < a; ++i) {
+            if (!g.interior(c, i, j, k)) continue;
+            const int64_t e = g.eidx(c, i, j, k);
+            const int64_t n0 = nidx(i, j, k);
+            const int64_t n1 = nidx(i + (c == 0), j + (c == 1), k + (c == 2));
+            zc acc(0.0);
+            if (op.nodeEarth[n0]) acc -= D[n0] / (op.sigN[n0] * op.vnode[n0]);
+            if (op.nodeEarth[n1]) acc += D[n1] / (op.sigN[n1] * op.vnode[n1]);
+            tE[e] += (op.dualA[e] * E[e] * acc).real();
+          }
+    }
+    for (int64_t n = 0; n < nN; ++n)
+      if (op.nodeEarth[n])
+        tN[n] += (-D[n] * S[n] / (op.sigN[n] * op.sigN[n] * op.vnode[n])).real();
+  }
+  // chain rule to the cells: sigma_E and sigma_N are volume-weighted averages
+  // of exp(m) over adjacent cells (ModelParamToEdge/Node, ModelMap.inc:88-277)
+  for (int k = op.nzAir; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int64_t q = i + int64_t(nx) * (j + int64_t(ny) * (k - op.nzAir));
+        const double w = op.dx[i] * op.dy[j] * op.dz[k] * std::exp(op.logcond[q]);
+        double acc = 0.0;
+        for (int dj = 0; dj < 2; ++dj)
+          for (int dk = 0; dk < 2; ++dk) {
+            const int64_t e = g.eidx(0, i, j + dj, k + dk);
+            acc += tE[e] / op.Vcell_ghost_edge[e];
+          }
+        for (int di = 0; di < 2; ++di)
+          for (int dk = 0; dk < 2; ++dk) {
+            const int64_t e = g.eidx(1, i + di, j, k + dk);
+            acc += tE[e] / op.Vcell_ghost_edge[e];
+          }
+        for (int di = 0; di < 2; ++di)
+          for (int dj = 0; dj < 2; ++dj) {
+            const int64_t e = g.eidx(2, i + di, j + dj, k);
+            acc += tE[e] / op.Vcell_ghost_edge[e];
+          }
+        if (mode != 0)
+          for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+              for (int dk = 0; dk < 2; ++dk) {
+                const int64_t n = nidx(i + di, j + dj, k + dk);
+                acc += tN[n] / op.Vcell_ghost_node[n];
+              }
+        dm[q] += w * acc;
+      }
 }
 
 void edge_to_model(const HostOperator& op, const double* e, double* dm) {

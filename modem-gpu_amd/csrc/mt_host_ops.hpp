@@ -34,7 +34,39 @@ struct HostOperator {
   std::vector<double> logcond;         // nx*ny*nzEarth, Fortran order
   double airLog = 0.0;
   std::vector<double> Vcell_ghost_edge;  // Cell2Edge(V_C) on all edges
+  std::vector<double> sigE;            // [nE] edge conductivity (0 on boundary edges)
+  std::vector<double> dualA;           // [nE] DualFaceA
+  std::vector<double> edgeL;           // [nE] EdgeL
+  std::vector<double> sigN;            // [nN] node conductivity
+  std::vector<double> vnode;           // [nN] Vnode
+  std::vector<char> nodeEarth;         // [nN] earth class of interior nodes (airNIndex)
+  std::vector<double> Vcell_ghost_node;  // [nN] sum of adjacent cell volumes
 };
+
+// Gradient of Re(sum_pol lam_pol^T A(sigma) e_pol) w.r.t. the LOGE cell
+// parameters, for one frequency: e_all = forward fields on all edges
+// (npol x nE), lam = adjoint fields on interior edges (npol x nInt).
+// mode 0: the reference's PmultT term only (SolverSens.f90:63-116), i.e. the
+//         derivative of the i*ISIGN*omega*mu0*sigma_E*V_E diagonal;
+// mode 1: plus the derivative of the grad-div part of AAii (GradDivSetup2,
+//         modelOperator3D.f90:916-1008) through sigma_E and sigma_N -- the
+//         exact discrete adjoint of the CC-GD system.
+// dm (nx*ny*nzEarth, Fortran order) is accumulated into.
+void grad_assemble(const HostOperator& op, double omega, int npol, const zc* e_all,
+                   const zc* lam, int mode, double* dm);
+
+// Divergence-correction operators on the padded node lattice
+// ((nx+1)(ny+1)(nz+1) nodes, boundary entries zero):
+//   VDs e   (DivC, modelOperator3D.f90:784-795) = -sum_e G_en DualFaceA_e sigma_e e_e
+//   Gai phi (Grad, :797-805)                    = (phi_end - phi_start) / EdgeL on interior edges
+//   VDsG = VDs Gai restricted to interior nodes (DivCorSetup, :670-705): a
+//   symmetric 7-point node stencil, stored [7][nN] (self, i-1, i+1, j-1, j+1, k-1, k+1).
+void divcor_node_stencil(const HostOperator& op, std::vector<double>& coefN,
+                         std::vector<double>& wE);
+// Dilu_Real (spOpTools.f90:1903-1948) of VDsG on the interior-node lattice:
+// returns 1/p on all nodes (zero on boundary nodes).
+void divcor_inv_pivots(const HostOperator& op, const std::vector<double>& coefN,
+                       std::vector<double>& dinv);
 
 void build_index_maps(HostOperator& op);
 

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "mt_dilu.hpp"
+#include "mt_divcor.hpp"
 #include "mt_host_ops.hpp"
 #include "mt_internal.h"
 #include "mt_kernels.hpp"
@@ -35,9 +36,22 @@ struct DeviceOperator {
 struct TilePlan {
   int tx = 8, ty = 8, tz = 8;
   int nlev = 0;
+  int nlat = 0;
+  LatticeInfo lat[3];
   std::vector<int> lvl_off;   // host: start of each tile level in tiles[]
   int4* tiles = nullptr;      // device
   TileShape* shape = nullptr; // device
+};
+
+// Divergence-correction state (node lattice), built on first use.
+struct DivcorData {
+  bool ready = false;
+  int64_t nN = 0;
+  double* coefN = nullptr;  // [7][nN] VDsG stencil
+  double* wE = nullptr;     // [nE] DualFaceA*sigma_E on interior edges
+  double* invL = nullptr;   // [nE] 1/EdgeL
+  Z* dinv = nullptr;        // [nN] 1/pivot (real, stored complex)
+  TilePlan tiles;
 };
 
 struct mt_ctx {
@@ -47,6 +61,7 @@ struct mt_ctx {
   HostOperator hop;
   DeviceOperator dop;
   TilePlan tiles;
+  DivcorData dc;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // block-DILU inverse pivots per (frequency, precision, transpose), on the
@@ -137,20 +152,19 @@ static StencilView<V> view(const mt_ctx* ctx, bool transpose) {
   return A;
 }
 
-// Tile plan for the DILU sweeps: the tiles of all three blocks grouped by
-// tile level ti+tj+tk.
-static int build_tile_plan(mt_ctx* ctx) {
-  TilePlan& tp = ctx->tiles;
+// Tile plan for the DILU sweeps: the tiles of all lattices grouped by tile
+// level ti+tj+tk.
+static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInfo* lat) {
   hipFree(tp.tiles);
   hipFree(tp.shape);
   tp.tiles = nullptr;
   tp.shape = nullptr;
-  const Grid& g = ctx->hop.g;
-  const int dims[3][3] = {{g.nx, g.ny - 1, g.nz - 1}, {g.nx - 1, g.ny, g.nz - 1}, {g.nx - 1, g.ny - 1, g.nz}};
+  tp.nlat = nlat;
   std::vector<std::vector<int4>> byLev;
-  for (int c = 0; c < 3; ++c) {
-    const int NTx = (dims[c][0] + tp.tx - 1) / tp.tx, NTy = (dims[c][1] + tp.ty - 1) / tp.ty,
-              NTz = (dims[c][2] + tp.tz - 1) / tp.tz;
+  for (int c = 0; c < nlat; ++c) {
+    tp.lat[c] = lat[c];
+    const int NTx = (lat[c].A + tp.tx - 1) / tp.tx, NTy = (lat[c].B + tp.ty - 1) / tp.ty,
+              NTz = (lat[c].C + tp.tz - 1) / tp.tz;
     for (int tk = 0; tk < NTz; ++tk)
       for (int tj = 0; tj < NTy; ++tj)
         for (int ti = 0; ti < NTx; ++ti) {
@@ -187,6 +201,13 @@ static int build_tile_plan(mt_ctx* ctx) {
   return MT_OK;
 }
 
+// The three interior edge lattices (x, y, z blocks of PC_setup).
+static void edge_lattices(const Grid& g, LatticeInfo lat[3]) {
+  lat[0] = {g.nx, g.ny - 1, g.nz - 1, 0, 1, 1, g.nx, g.ny + 1, 0};
+  lat[1] = {g.nx - 1, g.ny, g.nz - 1, 1, 0, 1, g.nx + 1, g.ny, g.nXe};
+  lat[2] = {g.nx - 1, g.ny - 1, g.nz, 1, 1, 0, g.nx + 1, g.ny + 1, g.nXe + g.nYe};
+}
+
 static void free_workspaces(mt_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->ws) hipFree(kv.second.first);
@@ -201,6 +222,9 @@ static int upload_operator(mt_ctx* ctx) {
   // Padded vectors rely on zero boundary entries and a zero tail; buffers laid
   // out for another grid would carry stale values there, so drop them all.
   free_workspaces(ctx);
+  hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
+  hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
+  ctx->dc = DivcorData();
   DeviceOperator& d = ctx->dop;
   const size_t nc = size_t(kSlots) * g.nE;
   std::vector<float> tmp(nc);
@@ -218,7 +242,9 @@ static int upload_operator(mt_ctx* ctx) {
   MT_HIP_CHECK(ctx, hipMemcpy(d.imap, ctx->hop.imap.data(), g.nInt * sizeof(int), hipMemcpyHostToDevice));
   build_offsets(g, d.off);
   d.ready = true;
-  return build_tile_plan(ctx);
+  LatticeInfo lat[3];
+  edge_lattices(g, lat);
+  return build_tile_plan(ctx, ctx->tiles, 3, lat);
 }
 
 static int ensure_transpose(mt_ctx* ctx) {
@@ -288,15 +314,17 @@ static int build_dinv(mt_ctx* ctx, int pc, bool transpose, const std::vector<dou
   return MT_OK;
 }
 
-// DILU apply for the active systems with status `req`: out = U^-1 L^-1 in.
+// DILU apply for the active systems with status `req`: out = U^-1 L^-1 in,
+// over the lattices of `tp` with coefficients coef[slot*cstride + addr].
 template <class T, class V>
-static void dilu_apply(mt_ctx* ctx, bool transpose, const int* act, int nact, const SysScal* sc,
-                       const C2<T>* in, C2<T>* out, const C2<T>* dinv, int req) {
-  const Grid& g = ctx->hop.g;
-  TilePlan& tp = ctx->tiles;
+static void dilu_sweeps(mt_ctx* ctx, const TilePlan& tp, const V* coef, int64_t cstride,
+                        int64_t vstride, const int* act, int nact, const SysScal* sc,
+                        const C2<T>* in, C2<T>* out, const C2<T>* dinv, int req) {
   DiluArgs<T, V> a;
-  a.coef = view<V>(ctx, transpose).coef;
-  a.nE = g.nE; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.nXe = g.nXe; a.nYe = g.nYe;
+  a.coef = coef;
+  a.cstride = cstride;
+  a.vstride = vstride;
+  for (int c = 0; c < 3; ++c) a.lat[c] = tp.lat[c];
   a.act = act; a.nact = nact; a.sc = sc; a.in = in; a.out = out; a.dinv = dinv;
   a.status_req = req;
   for (int L = 0; L < tp.nlev; ++L) {
@@ -310,6 +338,14 @@ static void dilu_apply(mt_ctx* ctx, bool transpose, const int* act, int nact, co
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
     k_dilu_tile<T, V, 8, 8, 8, false><<<a.ntiles * nact, 64, 0, ctx->stream>>>(a, tp.shape);
   }
+}
+
+template <class T, class V>
+static void dilu_apply(mt_ctx* ctx, bool transpose, const int* act, int nact, const SysScal* sc,
+                       const C2<T>* in, C2<T>* out, const C2<T>* dinv, int req) {
+  const Grid& g = ctx->hop.g;
+  dilu_sweeps<T, V>(ctx, ctx->tiles, view<V>(ctx, transpose).coef, g.nE, g.nE, act, nact, sc,
+                    in, out, dinv, req);
 }
 
 // ======================================================= BiCGStab engine
@@ -751,6 +787,8 @@ int mt_ctx_destroy(mt_ctx* ctx) {
   free_dop(ctx->dop);
   hipFree(ctx->tiles.tiles);
   hipFree(ctx->tiles.shape);
+  hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
+  hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -1094,10 +1132,140 @@ int mt_pmultT(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e0, const m
   return MT_OK;
 }
 
+static int ensure_divcor(mt_ctx* ctx) {
+  DivcorData& dc = ctx->dc;
+  if (dc.ready) return MT_OK;
+  const HostOperator& H = ctx->hop;
+  const Grid& g = H.g;
+  std::vector<double> coefN, wE, dinv;
+  divcor_node_stencil(H, coefN, wE);
+  divcor_inv_pivots(H, coefN, dinv);
+  dc.nN = int64_t(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  std::vector<double> invL(g.nE);
+  for (int64_t e = 0; e < g.nE; ++e) invL[e] = 1.0 / H.edgeL[e];
+  std::vector<Z> dz(dc.nN);
+  for (int64_t n = 0; n < dc.nN; ++n) dz[n] = {dinv[n], 0.0};
+  MT_HIP_CHECK(ctx, hipMalloc(&dc.coefN, coefN.size() * sizeof(double)));
+  MT_HIP_CHECK(ctx, hipMalloc(&dc.wE, g.nE * sizeof(double)));
+  MT_HIP_CHECK(ctx, hipMalloc(&dc.invL, g.nE * sizeof(double)));
+  MT_HIP_CHECK(ctx, hipMalloc(&dc.dinv, dc.nN * sizeof(Z)));
+  MT_HIP_CHECK(ctx, hipMemcpy(dc.coefN, coefN.data(), coefN.size() * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dc.wE, wE.data(), g.nE * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dc.invL, invL.data(), g.nE * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dc.dinv, dz.data(), dc.nN * sizeof(Z), hipMemcpyHostToDevice));
+  LatticeInfo lat = {g.nx - 1, g.ny - 1, g.nz - 1, 1, 1, 1, g.nx + 1, g.ny + 1, 0};
+  int rc = build_tile_plan(ctx, dc.tiles, 1, &lat);
+  if (rc) return rc;
+  dc.ready = true;
+  return MT_OK;
+}
+
+// SdivCorr (EMsolve3D.f90:507-613): phi = VDsG^-1 (VDs ei - phi0) by PCG with
+// the node DILU, then ei -= Gai phi.  VDs does not depend on omega (the
+// reference's VomegaMuSig/(mu0 omega)/V_E is sigma_E).
+int mt_gradient(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e_all,
+                const mt_zdouble* lam, double* dm, int mode) {
+  if (!ctx || !e_all || !lam || !dm || npol < 1 || mode < 0 || mode > 1)
+    return mt_fail(ctx, MT_E_ARG, "mt_gradient: bad arguments");
+  if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_gradient needs mt_assemble");
+  grad_assemble(ctx->hop, omega, npol, reinterpret_cast<const zc*>(e_all),
+                reinterpret_cast<const zc*>(lam), mode, dm);
+  return MT_OK;
+}
+
+int mt_host_gradient(int nx, int ny, int nzAir, int nzEarth, const double* dx, const double* dy,
+                     const double* dz, const double* logcond, double air_log, double omega,
+                     int npol, const mt_zdouble* e_all, const mt_zdouble* lam, double* dm,
+                     int mode) {
+  if (!dx || !dy || !dz || !logcond || !e_all || !lam || !dm || npol < 1 || mode < 0 || mode > 1)
+    return MT_E_ARG;
+  HostOperator H;
+  Status st = assemble(H, nx, ny, nzAir, nzEarth, dx, dy, dz, logcond, air_log);
+  if (st.code) return st.code;
+  grad_assemble(H, omega, npol, reinterpret_cast<const zc*>(e_all),
+                reinterpret_cast<const zc*>(lam), mode, dm);
+  return MT_OK;
+}
+
 int mt_divcor(mt_ctx* ctx, double omega, mt_zdouble* ei, const mt_zdouble* phi0, int maxit,
               double tol, int* niter, double* relres) {
-  (void)omega; (void)ei; (void)phi0; (void)maxit; (void)tol; (void)niter; (void)relres;
-  return mt_fail(ctx, MT_E_STATE, "mt_divcor: not built yet");
+  (void)omega;
+  if (!ctx || !ei || maxit < 1) return mt_fail(ctx, MT_E_ARG, "mt_divcor: bad arguments");
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_divcor needs mt_assemble");
+  hipSetDevice(ctx->device);
+  if ((rc = ensure_divcor(ctx))) return rc;
+  const Grid& g = ctx->hop.g;
+  DivcorData& dc = ctx->dc;
+  NodeArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.coefN = dc.coefN; a.wE = dc.wE; a.invL = dc.invL;
+  a.nN = dc.nN; a.nE = g.nE; a.nXe = g.nXe; a.nYe = g.nYe; a.nXi = g.nXi; a.nYi = g.nYi;
+  a.nInt = g.nInt; a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+  a.G = 512;
+  a.tol = tol;
+  const size_t nNp = size_t(dc.nN + g.tail());
+  if ((rc = ws(ctx, "dc_e", g.vlen(1), &a.e)) || (rc = ws(ctx, "dc_b", nNp, &a.b)) ||
+      (rc = ws(ctx, "dc_x", nNp, &a.x)) || (rc = ws(ctx, "dc_r", nNp, &a.r)) ||
+      (rc = ws(ctx, "dc_s", nNp, &a.s)) || (rc = ws(ctx, "dc_p", nNp, &a.p)) ||
+      (rc = ws(ctx, "dc_q", nNp, &a.q)) || (rc = ws(ctx, "dc_sc", 1, &a.sc)) ||
+      (rc = ws(ctx, "partials", size_t(pick_G(ctx, 1)) * 2 + 1024, &a.partials)) ||
+      (rc = ws(ctx, "dc_rerr", size_t(maxit) + 1, &a.rerr)))
+    return rc;
+  int* d_act;
+  if ((rc = ws(ctx, "dc_act", 1, &d_act))) return rc;
+  MT_HIP_CHECK(ctx, hipMemsetAsync(d_act, 0, sizeof(int), ctx->stream));
+  if ((rc = upload_padded(ctx, 1, ei, a.e, "io_tmp"))) return rc;
+  Z* phi0d = nullptr;
+  if (phi0) {
+    // phi0 on interior nodes (NODEi order) -> padded node vector
+    if ((rc = ws(ctx, "dc_phi0", nNp, &phi0d))) return rc;
+    std::vector<Z> h(dc.nN, Z{0.0, 0.0});
+    int64_t q = 0;
+    for (int k = 1; k < g.nz; ++k)
+      for (int j = 1; j < g.ny; ++j)
+        for (int i = 1; i < g.nx; ++i, ++q)
+          h[i + int64_t(g.nx + 1) * (j + int64_t(g.ny + 1) * k)] = {phi0[q].real(), phi0[q].imag()};
+    MT_HIP_CHECK(ctx, hipMemcpy(phi0d, h.data(), dc.nN * sizeof(Z), hipMemcpyHostToDevice));
+  }
+  a.phi0 = phi0d;
+  SysScal h;
+  std::memset(&h, 0, sizeof(h));
+  h.maxit = maxit;
+  h.status = ST_RUN;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(a.sc, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+  const int nb = a.G;
+  k_divc<<<nb, BS, 0, ctx->stream>>>(a, a.b);
+  k_pcg_init<<<nb, BS, 0, ctx->stream>>>(a);
+  k_pcg_fin<PF_INIT><<<1, BS, 0, ctx->stream>>>(a);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  std::vector<SysScal> hs(1);
+  if ((rc = read_scal(ctx, a.sc, 1, hs))) return rc;
+  if (hs[0].bnorm == 0.0) {  // divergence already zero: no correction (SdivCorr)
+    if (niter) *niter = 0;
+    if (relres) *relres = 0.0;
+    return MT_OK;
+  }
+  while (hs[0].status == ST_RUN) {
+    for (int c = 0; c < 10; ++c) {
+      dilu_sweeps<double, double>(ctx, dc.tiles, dc.coefN, dc.nN, dc.nN, d_act, 1, a.sc, a.r, a.s,
+                                  dc.dinv, ST_RUN);
+      k_pcg_rs<<<nb, BS, 0, ctx->stream>>>(a);
+      k_pcg_fin<PF_DELTA><<<1, BS, 0, ctx->stream>>>(a);
+      k_pcg_pq<<<nb, BS, 0, ctx->stream>>>(a);
+      k_pcg_fin<PF_ALPHA><<<1, BS, 0, ctx->stream>>>(a);
+      k_pcg_xr<<<nb, BS, 0, ctx->stream>>>(a);
+      k_pcg_fin<PF_R><<<1, BS, 0, ctx->stream>>>(a);
+    }
+    MT_HIP_CHECK(ctx, hipGetLastError());
+    if ((rc = read_scal(ctx, a.sc, 1, hs))) return rc;
+  }
+  k_grad_sub<<<1024, BS, 0, ctx->stream>>>(a, view<double>(ctx, false));
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  if (niter) *niter = hs[0].iter;
+  if (relres) *relres = hs[0].rnorm / hs[0].bnorm;
+  return download_padded(ctx, 1, a.e, ei, "io_tmp");
 }
 
 int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per_launch,

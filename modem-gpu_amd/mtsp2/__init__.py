@@ -78,6 +78,8 @@ _SIGS = {
     "mt_post_trn": (_I, [_P, _PZ, _PZ]),
     "mt_divcor": (_I, [_P, _D, _PZ, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
     "mt_pmultT": (_I, [_P, _D, _I, _PZ, _PZ, _PD, _I]),
+    "mt_gradient": (_I, [_P, _D, _I, _PZ, _PZ, _PD, _I]),
+    "mt_host_gradient": (_I, [_I, _I, _I, _I, _PD, _PD, _PD, _PD, _D, _D, _I, _PZ, _PZ, _PD, _I]),
     "mt_autotune": (_I, [_P, _I, C.POINTER(_I)]),
     "mt_set_tuning": (_I, [_P, _I]),
     "mt_bench_spmv": (_I, [_P, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
@@ -155,6 +157,27 @@ def host_dilu_pivots(grid_cells, row, col, val, vms1, omega, base=0):
     if rc:
         raise MTError(rc, "mt_host_dilu_pivots failed")
     return out
+
+
+GRAD_REFERENCE, GRAD_EXACT = 0, 1
+
+
+def host_gradient(dx, dy, dz, nzAir, logcond, omega, e_all, lam, mode=GRAD_EXACT,
+                  air_log=np.log(1e-10)):
+    """Host-only gradient d/dm Re(sum lam^T A e) (see mt_gradient)."""
+    dx = np.ascontiguousarray(dx, np.float64)
+    dy = np.ascontiguousarray(dy, np.float64)
+    dz = np.ascontiguousarray(dz, np.float64)
+    lc = np.asarray(logcond, np.float64)
+    e = np.ascontiguousarray(np.atleast_2d(e_all), np.complex128)
+    la = np.ascontiguousarray(np.atleast_2d(lam), np.complex128)
+    dm = np.zeros(lc.size)
+    rc = lib().mt_host_gradient(len(dx), len(dy), int(nzAir), len(dz) - int(nzAir), dx, dy, dz,
+                                np.ascontiguousarray(lc.ravel(order="F")), float(air_log),
+                                float(omega), e.shape[0], e, la, dm, int(mode))
+    if rc:
+        raise MTError(rc, "mt_host_gradient failed")
+    return dm.reshape(lc.shape, order="F")
 
 
 class Context:
@@ -304,6 +327,14 @@ class Context:
         out = np.zeros(int(np.prod(shape))) if dm is None else np.ascontiguousarray(
             np.asarray(dm, np.float64).ravel(order="F"))
         self._chk(lib().mt_pmultT(self._h, float(omega), e0.shape[0], e0, lam, out, int(acc)))
+        return out.reshape(shape, order="F")
+
+    def gradient(self, omega, e_all, lam, shape, mode=GRAD_EXACT, dm=None):
+        e = np.ascontiguousarray(np.atleast_2d(e_all), np.complex128)
+        la = np.ascontiguousarray(np.atleast_2d(lam), np.complex128)
+        out = np.zeros(int(np.prod(shape))) if dm is None else np.ascontiguousarray(
+            np.asarray(dm, np.float64).ravel(order="F")).copy()
+        self._chk(lib().mt_gradient(self._h, float(omega), e.shape[0], e, la, out, int(mode)))
         return out.reshape(shape, order="F")
 
     def divcor(self, omega, ei, maxit=100, tol=1e-7):

@@ -120,3 +120,31 @@ def test_graph_and_eager_agree(gpu_ctx):
     Xe, se = gpu_ctx.solve_batch(om, B, X0, tol=1e-9, use_graph=0, chunk=5)
     assert [s["niter"] for s in sg] == [s["niter"] for s in se]
     assert np.array_equal(Xg, Xe)
+
+
+def test_divcor_matches_oracle(gpu_ctx):
+    """SdivCorr on the GPU (DivC, node-DILU PCG, Grad update) against the
+    oracle's DivC / PCG / Grad, which are bit-exact with the reference."""
+    print("case divcor", flush=True)
+    o, g = _setup(gpu_ctx, 8, nx=14, ny=12, nzAir=4, nzEarth=10)
+    w = 2 * np.pi * 1.0
+    o.update_freq(w)
+    o.divcor_setup()
+    rng = np.random.default_rng(9)
+    ei = _cvec(rng, o.nInt)
+    phi_rhs = np.zeros(o.nNi, complex)
+    import ctypes
+    sp2oracle.lib().sp2o_divc(o._h, ei, phi_rhs)
+    phi, info = o.pcg(phi_rhs, maxit=200, tol=1e-9)
+    phi_all = np.zeros(o.nNode, complex)
+    phi_all[o.index()[2]] = phi
+    ge = np.zeros(o.nInt, complex)
+    sp2oracle.lib().sp2o_grad(o._h, phi_all, ge)
+    e_ref = ei - ge
+    eg, nit, rr = gpu_ctx.divcor(w, ei, maxit=200, tol=1e-9)
+    assert abs(nit - info["niter"]) <= 2
+    assert rr <= 1e-9
+    assert rel(eg, e_ref) <= 1e-8
+    d_after = np.zeros(o.nNi, complex)
+    sp2oracle.lib().sp2o_divc(o._h, np.ascontiguousarray(eg), d_after)
+    assert np.linalg.norm(d_after) <= 1e-6 * np.linalg.norm(phi_rhs)
