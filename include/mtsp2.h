@@ -47,6 +47,10 @@ typedef double _Complex mt_zdouble;
 #define MT_PC_DILU 0      /* block diagonal-ILU, the reference's PC_setup      */
 #define MT_PC_JACOBI 1
 #define MT_PC_NONE 2
+#define MT_PC_DILU_RB 3   /* block DILU in red-black tile order: 2 launches per
+                             sweep instead of one per tile level; changes the
+                             iterates (not in the reference), meant for the
+                             inner solves of MT_PREC_MIXED                    */
 #define MT_PREC_FP64 0
 #define MT_PREC_MIXED 1   /* FP32 inner BiCGStab + FP64 defect correction      */
 
@@ -222,9 +226,21 @@ int mt_set_tuning(mt_ctx *ctx, int groups_per_system);
  * (nnz*s_v + n*s_d + nsys*3*n*s_c). */
 int mt_bench_spmv(mt_ctx *ctx, int nsys, int precision, int reps, double *ms_per_launch,
                   double *bytes_per_launch);
-/* Kernel timing of the last mt_solve_* call: count and summed milliseconds of
- * the dominant SpMV kernel, measured with HIP events on the solver stream. */
-int mt_last_spmv_timing(const mt_ctx *ctx, int *launches, double *ms, double *bytes_per_launch);
+/* Kernel timing of the BiCGStab engine with HIP events on the solver stream,
+ * summed over every solve since the last mt_timing_enable (which resets it;
+ * on = 0 turns the events off).  family MT_KT_SPMV: one event pair around each
+ * fused SpMV+dot launch; MT_KT_DILU_FWD / MT_KT_DILU_BWD: one pair around each
+ * forward / backward block-DILU sweep (one launch per tile level, so the time
+ * includes the gaps between them).  launches: kernel launches covered; ms:
+ * summed event time; bytes: algorithmic bytes of those launches (SpMV: stored
+ * couplings + diagonal per launch, x, y and the dot partner per system; DILU:
+ * three coupling slots per sweep, input, output and pivots per system). */
+#define MT_KT_SPMV 0
+#define MT_KT_DILU_FWD 1
+#define MT_KT_DILU_BWD 2
+int mt_timing_enable(mt_ctx *ctx, int on);
+int mt_kernel_timing(const mt_ctx *ctx, int family, long long *launches, double *ms,
+                     double *bytes);
 
 #ifdef __cplusplus
 }

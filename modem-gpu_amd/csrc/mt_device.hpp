@@ -102,6 +102,80 @@ __device__ __forceinline__ RowInfo decode_row(const StencilView<V>& A, int64_t r
   return ri;
 }
 
+// Compile-time component tag for the row loops below.
+template <int C>
+struct Comp {
+  static constexpr int value = C;
+};
+
+// Row r, counted from the first interior row of component C, -> RowInfo.
+template <int C, class V>
+__device__ __forceinline__ RowInfo decode_row_c(const StencilView<V>& A, int64_t r) {
+  RowInfo ri;
+  int i, j, k;
+  const int nx = A.nx, ny = A.ny;
+  uint32_t q = uint32_t(r);
+  if (C == 0) {
+    const uint32_t ai = uint32_t(nx), bi = uint32_t(ny - 1);
+    i = q % ai; q /= ai; j = 1 + q % bi; k = 1 + q / bi;
+  } else if (C == 1) {
+    const uint32_t ai = uint32_t(nx - 1), bi = uint32_t(ny);
+    i = 1 + q % ai; q /= ai; j = q % bi; k = 1 + q / bi;
+  } else {
+    const uint32_t ai = uint32_t(nx - 1), bi = uint32_t(ny - 1);
+    i = 1 + q % ai; q /= ai; j = 1 + q % bi; k = q / bi;
+  }
+  const int64_t ex = i + int64_t(nx) * (j + int64_t(ny + 1) * k);
+  const int64_t ey = A.nXe + i + int64_t(nx + 1) * (j + int64_t(ny) * k);
+  const int64_t ez = A.nXe + A.nYe + i + int64_t(nx + 1) * (j + int64_t(ny + 1) * k);
+  ri.c = C;
+  if (C == 0) { ri.e = ex; ri.b1 = ey; ri.b2 = ez; }
+  else if (C == 1) { ri.e = ey; ri.b1 = ex; ri.b2 = ez; }
+  else { ri.e = ez; ri.b1 = ex; ri.b2 = ey; }
+  return ri;
+}
+
+// body(ri, Comp<C>()) for the interior rows r = s + t0 + n*step of each
+// component's part [s, e) of [lo, hi): inside the body the component, and so
+// the neighbour offsets, are compile-time (uniform kernel-argument reads).
+template <class V, class F>
+__device__ __forceinline__ void for_rows(const StencilView<V>& A, int64_t lo, int64_t hi,
+                                         int64_t t0, int64_t step, F&& body) {
+  const int64_t b1 = A.nXi, b2 = A.nXi + A.nYi;
+  for (int64_t r = lo + t0; r < min(hi, b1); r += step) body(decode_row_c<0>(A, r), Comp<0>());
+  for (int64_t r = max(lo, b1) + t0; r < min(hi, b2); r += step)
+    body(decode_row_c<1>(A, r - b1), Comp<1>());
+  for (int64_t r = max(lo, b2) + t0; r < hi; r += step) body(decode_row_c<2>(A, r - b2), Comp<2>());
+}
+
+// y_row = sum_s coef[s][e] x[nb_s] for a row of component C
+template <int C, class V, class T>
+__device__ __forceinline__ C2<T> stencil_row_c(const StencilView<V>& A, const RowInfo& ri,
+                                               const C2<T>* __restrict__ x) {
+  const V* __restrict__ cf = A.coef + ri.e;
+  const int64_t nE = A.nE;
+  C2<T> acc = cscale(T(cf[0]), x[ri.e]);
+#pragma unroll
+  for (int s = 1; s < 7; ++s) {
+    const C2<T> xv = x[ri.e + A.off[C][s]];
+    const T c = T(cf[s * nE]);
+    acc.x += c * xv.x; acc.y += c * xv.y;
+  }
+#pragma unroll
+  for (int s = 7; s < 11; ++s) {
+    const C2<T> xv = x[ri.b1 + A.off[C][s]];
+    const T c = T(cf[s * nE]);
+    acc.x += c * xv.x; acc.y += c * xv.y;
+  }
+#pragma unroll
+  for (int s = 11; s < 15; ++s) {
+    const C2<T> xv = x[ri.b2 + A.off[C][s]];
+    const T c = T(cf[s * nE]);
+    acc.x += c * xv.x; acc.y += c * xv.y;
+  }
+  return acc;
+}
+
 // y_row = sum_s coef[s][e] x[nb_s]  (real stencil part only)
 template <class V, class T>
 __device__ __forceinline__ C2<T> stencil_row(const StencilView<V>& A, const RowInfo& ri,
@@ -202,18 +276,19 @@ __device__ __forceinline__ void chunk_range(int64_t n, int G, int g, int64_t& lo
 // needs a device-scope fence per block, which on gfx950 writes back the
 // whole L2 of the XCD each time.)
 
-// Reduce partials[(si * G + g) * NV + q] over g for every system, in the last
-// block, in fixed g order (deterministic); result to red[si * NV + q] (LDS).
-template <int NV>
-__device__ void reduce_partials(const Z* partials, int nact, int G, Z* red) {
-  for (int cq = threadIdx.x; cq < nact * NV; cq += blockDim.x) {
-    const int si = cq / NV, q = cq % NV;
-    Z s = {0.0, 0.0};
-    const Z* p = partials + (int64_t)si * G * NV + q;
-    for (int g = 0; g < G; ++g) s = cadd(s, p[(int64_t)g * NV]);
-    red[cq] = s;
+// Sum over g of partials[(si * G + g) * NV + q] for one system, by the whole
+// block in a fixed order (deterministic); result valid in thread 0.
+template <int NV, int BS>
+__device__ __forceinline__ void reduce_sys(const Z* partials, int si, int G, Z (&v)[NV],
+                                           Z* lds /* >= NV*BS/64 */) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = {0.0, 0.0};
+  const Z* p = partials + (int64_t)si * G * NV;
+  for (int g = threadIdx.x; g < G; g += BS) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) v[q] = cadd(v[q], p[(int64_t)g * NV + q]);
   }
-  __syncthreads();
+  block_sum<NV, BS>(v, lds);
 }
 
 }  // namespace mt

@@ -63,75 +63,127 @@ __device__ __forceinline__ int64_t lat_e(const LatticeInfo& L, int i, int j, int
 }
 
 // FWD=true: lower solve with halo on the low side; false: upper solve.
+// The tile's inputs (box values, pivots, three coefficients per point) are
+// all requested from memory before the first is used, so a tile pays one
+// memory latency, not one per box row; then the TX+TY+TZ-2 hyperplanes are
+// swept out of LDS and the tile is written back in box order.
+//   forward LDS: wb = dinv*y on the whole box (halo from memory, tile points as
+//                they are solved), rb = r -> y on the tile, dv = dinv on the tile
+//   backward LDS: yb = y -> z on the tile and z on the halo, dv on the tile
 template <class T, class V, int TX, int TY, int TZ, bool FWD>
 __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileShape* shp) {
   constexpr int BX = TX + 1, BY = TY + 1, BZ = TZ + 1, NB = BX * BY * BZ, NP = TX * TY * TZ;
-  __shared__ C2<T> yb[NB];
-  __shared__ C2<T> dv[FWD ? NB : NP];
+  constexpr int NQ = (NB + 63) / 64;
+  constexpr int NLEV = TX + TY + TZ - 2;
+  __shared__ C2<T> yb[NB];   // FWD: w = dinv*y; BWD: y -> z
+  __shared__ C2<T> rb[FWD ? NP : 1];
+  __shared__ C2<T> dv[NP];
   __shared__ V cf[3][NP];
   __shared__ unsigned int spts[NP];
-  __shared__ int slvl[TX + TY + TZ];
+  __shared__ int slvl[NLEV + 1];
+  const int lane = threadIdx.x;
   const int tix = blockIdx.x / a.nact, si = blockIdx.x - tix * a.nact;
   const int sys = a.act[si];
-  const SysScal& sc = a.sc[sys];
-  if (sc.status != a.status_req) return;
+  if (a.sc[sys].status != a.status_req) return;
   const int4 tl = a.tiles[tix];
   const LatticeInfo L = a.lat[tl.x];
   const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
   const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
   const int64_t o = (int64_t)sys * a.vstride;
-  const C2<T>* dinv = a.dinv + (int64_t)sc.fidx * a.cstride;
+  // uniform base pointers, held in registers (a per-lane choice between two
+  // kernel-argument fields would be re-read from the argument segment)
+  const C2<T>* __restrict__ pin = a.in + o;
+  const C2<T>* __restrict__ pout = a.out + o;
+  const C2<T>* __restrict__ dinv = a.dinv + (int64_t)a.sc[sys].fidx * a.cstride;
   const int s0 = FWD ? 1 : 2;  // lower slots 1,3,5 / upper slots 2,4,6
-  // box coordinate b <-> local tile coordinate l: FWD: l = b - 1, BWD: l = b
-  for (int q = threadIdx.x; q < NB; q += 64) {
+  const V* __restrict__ cfp[3] = {a.coef + (int64_t)s0 * a.cstride,
+                                  a.coef + (int64_t)(s0 + 2) * a.cstride,
+                                  a.coef + (int64_t)(s0 + 4) * a.cstride};
+  // ---- issue every load of the tile
+  C2<T> yv[NQ], dd[NQ];
+  V c3[NQ][3];
+#pragma unroll
+  for (int u = 0; u < NQ; ++u) {
+    const int q = lane + 64 * u;
     const int bi = q % BX, bj = (q / BX) % BY, bk = q / (BX * BY);
+    // box coordinate b <-> local tile coordinate l: FWD: l = b - 1, BWD: l = b
     const int li = FWD ? bi - 1 : bi, lj = FWD ? bj - 1 : bj, lk = FWD ? bk - 1 : bk;
-    const bool inTile = li >= 0 && lj >= 0 && lk >= 0 && li < TX && lj < TY && lk < TZ;
+    const bool inTile = li >= 0 && lj >= 0 && lk >= 0 && li < ex && lj < ey && lk < ez;
     const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
-    const bool inLat = gi >= 0 && gj >= 0 && gk >= 0 && gi < L.A && gj < L.B && gk < L.C;
-    C2<T> yv = {T(0), T(0)}, dd = {T(0), T(0)};
+    const bool inLat = q < NB && gi >= 0 && gj >= 0 && gk >= 0 && gi < L.A && gj < L.B && gk < L.C;
+    yv[u] = {T(0), T(0)};
+    dd[u] = {T(0), T(0)};
+    c3[u][0] = c3[u][1] = c3[u][2] = V(0);
     if (inLat) {
       const int64_t e = lat_e(L, gi, gj, gk);
-      yv = inTile ? (FWD ? a.in[o + e] : a.out[o + e]) : a.out[o + e];
-      dd = dinv[e];
+      const C2<T>* src = (FWD && inTile) ? pin : pout;
+      yv[u] = src[e];
+      if (FWD || inTile) dd[u] = dinv[e];
       if (inTile) {
-        const int p = li + TX * (lj + TY * lk);
 #pragma unroll
-        for (int s = 0; s < 3; ++s) cf[s][p] = a.coef[(int64_t)(s0 + 2 * s) * a.cstride + e];
-        if (!FWD) dv[p] = dd;
+        for (int s = 0; s < 3; ++s) c3[u][s] = cfp[s][e];
       }
-    } else if (inTile) {
+    }
+  }
+  unsigned int pk[(NP + 63) / 64];
+#pragma unroll
+  for (int u = 0; u < (NP + 63) / 64; ++u) {
+    const int p = lane + 64 * u;
+    pk[u] = p < NP ? shp->pts[p] : 0u;
+  }
+  const int lv = lane <= NLEV ? shp->lvl[lane] : 0;
+  // ---- stage into LDS
+#pragma unroll
+  for (int u = 0; u < NQ; ++u) {
+    const int q = lane + 64 * u;
+    if (q >= NB) break;
+    const int bi = q % BX, bj = (q / BX) % BY, bk = q / (BX * BY);
+    const int li = FWD ? bi - 1 : bi, lj = FWD ? bj - 1 : bj, lk = FWD ? bk - 1 : bk;
+    const bool inBoxTile = li >= 0 && lj >= 0 && lk >= 0 && li < TX && lj < TY && lk < TZ;
+    if (inBoxTile) {
       const int p = li + TX * (lj + TY * lk);
 #pragma unroll
-      for (int s = 0; s < 3; ++s) cf[s][p] = V(0);
-      if (!FWD) dv[p] = dd;
+      for (int s = 0; s < 3; ++s) cf[s][p] = c3[u][s];
+      dv[p] = dd[u];
+      if (FWD) {
+        rb[p] = yv[u];
+        yb[q] = {T(0), T(0)};
+      } else {
+        yb[q] = yv[u];
+      }
+    } else {
+      // halo: FWD keeps w = dinv*y of the (final) upstream point, BWD its z
+      yb[q] = FWD ? cmul(dd[u], yv[u]) : yv[u];
     }
-    yb[q] = yv;
-    if (FWD) dv[q] = dd;
   }
-  for (int p = threadIdx.x; p < NP; p += 64) spts[p] = shp->pts[p];
-  for (int p = threadIdx.x; p < TX + TY + TZ - 1; p += 64) slvl[p] = shp->lvl[p];
+#pragma unroll
+  for (int u = 0; u < (NP + 63) / 64; ++u) {
+    const int p = lane + 64 * u;
+    if (p < NP) spts[p] = pk[u];
+  }
+  if (lane <= NLEV) slvl[lane] = lv;
   __syncthreads();
-  const int nlev = TX + TY + TZ - 2;
-  for (int hh = 0; hh < nlev; ++hh) {
-    const int h = FWD ? hh : nlev - 1 - hh;
-    for (int pp = slvl[h] + threadIdx.x; pp < slvl[h + 1]; pp += 64) {
-      const unsigned int pk = spts[pp];
-      const int li = pk & 0xff, lj = (pk >> 8) & 0xff, lk = pk >> 16;
+  // ---- hyperplane sweep
+  for (int hh = 0; hh < NLEV; ++hh) {
+    const int h = FWD ? hh : NLEV - 1 - hh;
+    for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
+      const unsigned int pt = spts[pp];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = pt >> 16;
       if (li >= ex || lj >= ey || lk >= ez) continue;
       const int p = li + TX * (lj + TY * lk);
       if (FWD) {
         const int q = (li + 1) + BX * ((lj + 1) + BY * (lk + 1));
-        C2<T> y = yb[q];
+        C2<T> y = rb[p];
         const int qn[3] = {q - 1, q - BX, q - BX * BY};
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
           const T c = T(cf[s][p]);
-          const C2<T> w = cmul(dv[qn[s]], yb[qn[s]]);
+          const C2<T> w = yb[qn[s]];
           y.x -= c * w.x;
           y.y -= c * w.y;
         }
-        yb[q] = y;
+        rb[p] = y;
+        yb[q] = cmul(dv[p], y);
       } else {
         const int q = li + BX * (lj + BY * lk);
         C2<T> y = yb[q];
@@ -148,12 +200,371 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
     }
     __syncthreads();
   }
-  for (int p = threadIdx.x; p < NP; p += 64) {
+  // ---- write the tile back
+  for (int p = lane; p < NP; p += 64) {
     const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
     if (li >= ex || lj >= ey || lk >= ez) continue;
-    const int q = FWD ? (li + 1) + BX * ((lj + 1) + BY * (lk + 1)) : li + BX * (lj + BY * lk);
-    a.out[o + lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = yb[q];
+    C2<T>* __restrict__ dst = a.out + o;
+    dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = FWD ? rb[p] : yb[li + BX * (lj + BY * lk)];
   }
+}
+
+// ------------------------------------------------- red-black tile ordering
+// Block DILU in a second ordering, for the inner FP32 solves of the mixed
+// solver (whose FP64 defect correction makes any fixed preconditioner
+// admissible): the tiles of each lattice are coloured red/black by the parity
+// of ti+tj+tk, all red tiles come first, then all black tiles, and inside a
+// tile the points keep the lexicographic order.  Red tiles only couple to
+// black tiles, so each colour is one launch: a sweep takes 2 launches instead
+// of one per tile level.  With this ordering
+//   lower(e) = in-tile lower neighbours, plus (black e) its face neighbours in
+//              the adjacent red tiles;
+//   upper(e) = in-tile upper neighbours, plus (red e) its face neighbours in
+//              the adjacent black tiles,
+// and the pivots, the forward and the backward sweep are those of DILU with
+// these sets.  HALO=true is the colour whose cross-face neighbours are final
+// (black in the forward sweep and in the pivots, red in the backward sweep):
+// their contributions are folded into the tile's right-hand side (or
+// diagonal) when it is loaded, and the tile is then swept like an isolated one.
+template <class T, class V, int TX, int TY, int TZ, bool FWD, bool HALO>
+__global__ __launch_bounds__(64) void k_dilu_rb(DiluArgs<T, V> a, const TileShape* shp) {
+  constexpr int BX = TX + 1, BY = TY + 1, BZ = TZ + 1, NB = BX * BY * BZ, NP = TX * TY * TZ;
+  constexpr int NU = (NP + 63) / 64;
+  constexpr int NLEV = TX + TY + TZ - 2;
+  __shared__ C2<T> yb[NB];   // FWD: w = dinv*y; BWD: y -> z; the pad layer stays zero
+  __shared__ C2<T> rb[FWD ? NP : 1];
+  __shared__ C2<T> dv[NP];
+  __shared__ V cf[3][NP];
+  __shared__ unsigned int spts[NP];
+  __shared__ int slvl[NLEV + 1];
+  const int lane = threadIdx.x;
+  const int tix = blockIdx.x / a.nact, si = blockIdx.x - tix * a.nact;
+  const int sys = a.act[si];
+  if (a.sc[sys].status != a.status_req) return;
+  const int4 tl = a.tiles[tix];
+  const LatticeInfo L = a.lat[tl.x];
+  const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
+  const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
+  const int64_t o = (int64_t)sys * a.vstride;
+  const C2<T>* __restrict__ pin = a.in + o;
+  const C2<T>* __restrict__ pout = a.out + o;
+  const C2<T>* __restrict__ dinv = a.dinv + (int64_t)a.sc[sys].fidx * a.cstride;
+  const V* __restrict__ cfs = a.coef;
+  const int64_t cst = a.cstride;
+  const int64_t sj = L.a, sk = int64_t(L.a) * L.b;  // lattice strides
+  // ---- loads: tile values, pivots, in-tile couplings, cross-face terms
+  C2<T> rv[NU], dd[NU];
+  V c3[NU][3];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int p = lane + 64 * u;
+    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
+    const bool inTile = p < NP && li < ex && lj < ey && lk < ez;
+    rv[u] = {T(0), T(0)};
+    dd[u] = {T(0), T(0)};
+    c3[u][0] = c3[u][1] = c3[u][2] = V(0);
+    if (inTile) {
+      const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
+      const int64_t e = lat_e(L, gi, gj, gk);
+      rv[u] = (FWD ? pin : pout)[e];
+      dd[u] = dinv[e];
+      // in-tile couplings: lower (slots 1,3,5) forward, upper (2,4,6) backward
+      const bool has0 = FWD ? li > 0 : li + 1 < ex;
+      const bool has1 = FWD ? lj > 0 : lj + 1 < ey;
+      const bool has2 = FWD ? lk > 0 : lk + 1 < ez;
+      const int s0 = FWD ? 1 : 2;
+      if (has0) c3[u][0] = cfs[(int64_t)s0 * cst + e];
+      if (has1) c3[u][1] = cfs[(int64_t)(s0 + 2) * cst + e];
+      if (has2) c3[u][2] = cfs[(int64_t)(s0 + 4) * cst + e];
+      if (HALO) {
+        // neighbours across the tile faces (the other colour, final)
+        const bool lo0 = li == 0 && gi > 0, hi0 = li + 1 == ex && gi + 1 < L.A;
+        const bool lo1 = lj == 0 && gj > 0, hi1 = lj + 1 == ey && gj + 1 < L.B;
+        const bool lo2 = lk == 0 && gk > 0, hi2 = lk + 1 == ez && gk + 1 < L.C;
+        const bool fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
+        const int64_t st[6] = {-1, 1, -sj, sj, -sk, sk};
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          if (!fl[f]) continue;
+          const T c = T(cfs[(int64_t)(1 + f) * cst + e]);
+          const C2<T> v = FWD ? cmul(dinv[e + st[f]], pout[e + st[f]]) : pout[e + st[f]];
+          rv[u].x -= c * v.x;
+          rv[u].y -= c * v.y;
+        }
+      }
+    }
+  }
+  for (int q = lane; q < NB; q += 64) yb[q] = {T(0), T(0)};
+  for (int p = lane; p < NP; p += 64) spts[p] = shp->pts[p];
+  if (lane <= NLEV) slvl[lane] = shp->lvl[lane];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int p = lane + 64 * u;
+    if (p >= NP) break;
+    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) cf[s][p] = c3[u][s];
+    dv[p] = dd[u];
+    if (FWD) rb[p] = rv[u];
+    else yb[li + BX * (lj + BY * lk)] = rv[u];
+  }
+  __syncthreads();
+  // ---- in-tile sweep
+  for (int hh = 0; hh < NLEV; ++hh) {
+    const int h = FWD ? hh : NLEV - 1 - hh;
+    for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
+      const unsigned int pt = spts[pp];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = pt >> 16;
+      if (li >= ex || lj >= ey || lk >= ez) continue;
+      const int p = li + TX * (lj + TY * lk);
+      if (FWD) {
+        const int q = (li + 1) + BX * ((lj + 1) + BY * (lk + 1));
+        C2<T> y = rb[p];
+        const int qn[3] = {q - 1, q - BX, q - BX * BY};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T c = T(cf[s][p]);
+          const C2<T> w = yb[qn[s]];
+          y.x -= c * w.x;
+          y.y -= c * w.y;
+        }
+        rb[p] = y;
+        yb[q] = cmul(dv[p], y);
+      } else {
+        const int q = li + BX * (lj + BY * lk);
+        C2<T> y = yb[q];
+        const int qn[3] = {q + 1, q + BX, q + BX * BY};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T c = T(cf[s][p]);
+          const C2<T> z = yb[qn[s]];
+          y.x -= c * z.x;
+          y.y -= c * z.y;
+        }
+        yb[q] = cmul(dv[p], y);
+      }
+    }
+    __syncthreads();
+  }
+  C2<T>* __restrict__ dst = a.out + o;
+  for (int p = lane; p < NP; p += 64) {
+    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
+    if (li >= ex || lj >= ey || lk >= ez) continue;
+    dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = FWD ? rb[p] : yb[li + BX * (lj + BY * lk)];
+  }
+}
+
+// ------------------------------------------------------------- pivots
+// PC_setup's block-DILU pivots on the device (Dilu_Cmplx_AS, spOpTools.f90:
+// 1996-2055, on the x-x, y-y, z-z blocks of A(omega)):
+//   p_e = a_ee + i*ISIGN*omega*vms1_e - sum_{l lower} a_el a_le / p_l,  dinv_e = 1/p_e
+// The recurrence has the dependencies of the forward sweep, so it runs on the
+// same tile levels (one launch per level, all frequencies at once).  Every
+// point combines its three lower neighbours in the host routine's order.
+struct PivotArgs {
+  const double* coef;      // [15][cstride] stencil slots (1,3,5 lower; 2,4,6 upper)
+  int64_t cstride;
+  const double* vms1;      // [cstride]
+  LatticeInfo lat[3];
+  const int4* tiles;
+  int ntiles;
+  const double* omegas;    // [nf]
+  int nf;
+  Z* dinv;                 // [nf][cstride]
+};
+
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(64) void k_dilu_pivots(PivotArgs a, const TileShape* shp) {
+#pragma clang fp contract(off)
+  constexpr int BX = TX + 1, BY = TY + 1, BZ = TZ + 1, NB = BX * BY * BZ, NP = TX * TY * TZ;
+  constexpr int NQ = (NB + 63) / 64;
+  constexpr int NLEV = TX + TY + TZ - 2;
+  __shared__ Z pv[NB];          // dinv of every box point (halo: final)
+  __shared__ double cu[3][NB];  // upper couplings a_le of every box point
+  __shared__ double cl[3][NP];  // lower couplings a_el of the tile points
+  __shared__ Z dg[NP];          // diagonal of the tile points
+  __shared__ unsigned int spts[NP];
+  __shared__ int slvl[NLEV + 1];
+  const int lane = threadIdx.x;
+  const int tix = blockIdx.x / a.nf, f = blockIdx.x - tix * a.nf;
+  const int4 tl = a.tiles[tix];
+  const LatticeInfo L = a.lat[tl.x];
+  const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
+  const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
+  const double wsg = kIsignD * a.omegas[f];
+  Z* __restrict__ dv = a.dinv + (int64_t)f * a.cstride;
+  const double* __restrict__ cf = a.coef;
+  const double* __restrict__ vm = a.vms1;
+  for (int u = 0; u < NQ; ++u) {
+    const int q = lane + 64 * u;
+    if (q >= NB) break;
+    const int bi = q % BX, bj = (q / BX) % BY, bk = q / (BX * BY);
+    const int li = bi - 1, lj = bj - 1, lk = bk - 1;
+    const bool inBoxTile = li >= 0 && lj >= 0 && lk >= 0 && li < TX && lj < TY && lk < TZ;
+    const bool inTile = inBoxTile && li < ex && lj < ey && lk < ez;
+    const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
+    const bool inLat = gi >= 0 && gj >= 0 && gk >= 0 && gi < L.A && gj < L.B && gk < L.C;
+    Z pvq = {0.0, 0.0};
+    double up[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+    Z d = {0.0, 0.0};
+    if (inLat) {
+      const int64_t e = lat_e(L, gi, gj, gk);
+      if (!inTile) pvq = dv[e];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) up[s] = cf[(int64_t)(2 + 2 * s) * a.cstride + e];
+      if (inTile) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) lo[s] = cf[(int64_t)(1 + 2 * s) * a.cstride + e];
+        d = {cf[e], wsg * vm[e]};
+      }
+    }
+    pv[q] = pvq;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) cu[s][q] = up[s];
+    if (inBoxTile) {
+      const int p = li + TX * (lj + TY * lk);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) cl[s][p] = lo[s];
+      dg[p] = d;
+    }
+  }
+  for (int p = lane; p < NP; p += 64) spts[p] = shp->pts[p];
+  if (lane <= NLEV) slvl[lane] = shp->lvl[lane];
+  __syncthreads();
+  for (int h = 0; h < NLEV; ++h) {
+    for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
+      const unsigned int pt = spts[pp];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = pt >> 16;
+      if (li >= ex || lj >= ey || lk >= ez) continue;
+      const int p = li + TX * (lj + TY * lk);
+      const int q = (li + 1) + BX * ((lj + 1) + BY * (lk + 1));
+      const int qn[3] = {q - 1, q - BX, q - BX * BY};
+      Z piv = dg[p];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const double alo = cl[s][p];
+        if (alo == 0.0) continue;
+        const double t = alo * cu[s][qn[s]];
+        const Z w = pv[qn[s]];
+        piv.x = piv.x - t * w.x;
+        piv.y = piv.y - t * w.y;
+      }
+      pv[q] = zdiv({1.0, 0.0}, piv);
+    }
+    __syncthreads();
+  }
+  for (int p = lane; p < NP; p += 64) {
+    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
+    if (li >= ex || lj >= ey || lk >= ez) continue;
+    dv[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = pv[(li + 1) + BX * ((lj + 1) + BY * (lk + 1))];
+  }
+}
+
+// Pivots in the red-black tile ordering: red tiles (HALO=false) are
+// factorised on their own; a black tile (HALO=true) first takes the
+// contributions a_el a_le / p_l of its face neighbours in the (final) red
+// tiles off its diagonal, then runs the in-tile recurrence.
+template <int TX, int TY, int TZ, bool HALO>
+__global__ __launch_bounds__(64) void k_dilu_pivots_rb(PivotArgs a, const TileShape* shp) {
+#pragma clang fp contract(off)
+  constexpr int NP = TX * TY * TZ;
+  constexpr int NLEV = TX + TY + TZ - 2;
+  __shared__ Z pv[NP];
+  __shared__ double cu[3][NP];  // upper couplings of the tile points
+  __shared__ double cl[3][NP];  // in-tile lower couplings
+  __shared__ Z dg[NP];
+  __shared__ unsigned int spts[NP];
+  __shared__ int slvl[NLEV + 1];
+  const int lane = threadIdx.x;
+  const int tix = blockIdx.x / a.nf, f = blockIdx.x - tix * a.nf;
+  const int4 tl = a.tiles[tix];
+  const LatticeInfo L = a.lat[tl.x];
+  const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
+  const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
+  const double wsg = kIsignD * a.omegas[f];
+  Z* __restrict__ dvo = a.dinv + (int64_t)f * a.cstride;
+  const double* __restrict__ cf = a.coef;
+  const int64_t cst = a.cstride;
+  const int64_t sj = L.a, sk = int64_t(L.a) * L.b;
+  for (int p = lane; p < NP; p += 64) {
+    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
+    const bool inTile = li < ex && lj < ey && lk < ez;
+    Z d = {0.0, 0.0};
+    double up[3] = {0.0, 0.0, 0.0}, lo[3] = {0.0, 0.0, 0.0};
+    if (inTile) {
+      const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
+      const int64_t e = lat_e(L, gi, gj, gk);
+      d = {cf[e], wsg * a.vms1[e]};
+#pragma unroll
+      for (int s = 0; s < 3; ++s) up[s] = cf[(int64_t)(2 + 2 * s) * cst + e];
+      if (li > 0) lo[0] = cf[1 * cst + e];
+      if (lj > 0) lo[1] = cf[3 * cst + e];
+      if (lk > 0) lo[2] = cf[5 * cst + e];
+      if (HALO) {
+        const bool fl[6] = {li == 0 && gi > 0, li + 1 == ex && gi + 1 < L.A,
+                            lj == 0 && gj > 0, lj + 1 == ey && gj + 1 < L.B,
+                            lk == 0 && gk > 0, lk + 1 == ez && gk + 1 < L.C};
+        const int64_t st[6] = {-1, 1, -sj, sj, -sk, sk};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+          if (!fl[q]) continue;
+          const double ael = cf[(int64_t)(1 + q) * cst + e];
+          if (ael == 0.0) continue;
+          // the coupling back from l to e is l's slot in the opposite direction
+          const int back = (q & 1) ? q : q + 2;  // 1-based slot: low<->high
+          const double ale = cf[(int64_t)back * cst + e + st[q]];
+          const double t = ael * ale;
+          const Z w = dvo[e + st[q]];
+          d.x = d.x - t * w.x;
+          d.y = d.y - t * w.y;
+        }
+      }
+    }
+    dg[p] = d;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      cu[s][p] = up[s];
+      cl[s][p] = lo[s];
+    }
+    pv[p] = {0.0, 0.0};
+  }
+  for (int p = lane; p < NP; p += 64) spts[p] = shp->pts[p];
+  if (lane <= NLEV) slvl[lane] = shp->lvl[lane];
+  __syncthreads();
+  const int stp[3] = {1, TX, TX * TY};
+  for (int h = 0; h < NLEV; ++h) {
+    for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
+      const unsigned int pt = spts[pp];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = pt >> 16;
+      if (li >= ex || lj >= ey || lk >= ez) continue;
+      const int p = li + TX * (lj + TY * lk);
+      Z piv = dg[p];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const double alo = cl[s][p];
+        if (alo == 0.0) continue;
+        const double t = alo * cu[s][p - stp[s]];
+        const Z w = pv[p - stp[s]];
+        piv.x = piv.x - t * w.x;
+        piv.y = piv.y - t * w.y;
+      }
+      pv[p] = zdiv({1.0, 0.0}, piv);
+    }
+    __syncthreads();
+  }
+  for (int p = lane; p < NP; p += 64) {
+    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
+    if (li >= ex || lj >= ey || lk >= ez) continue;
+    dvo[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = pv[p];
+  }
+}
+
+template <class T>
+__global__ void k_cast_pivots(const Z* __restrict__ in, C2<T>* __restrict__ out, int64_t n) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x)
+    out[q] = ccast<T>(in[q]);
 }
 
 }  // namespace mt

@@ -184,10 +184,9 @@ __global__ __launch_bounds__(BS) void k_pcg_xr(NodeArgs a) {
 // PCG scalar logic of solver.f90:57-136 (loop while rerr(i) > tol and i < maxit)
 template <int KIND>
 __global__ __launch_bounds__(BS) void k_pcg_fin(NodeArgs a) {
-  __shared__ Z red[64 * 2];
-  const int zero = 0;
-  reduce_partials<2>(a.partials, 1, a.G, red);
-  (void)zero;
+  __shared__ Z lds[2 * BS / 64];
+  Z red[2];
+  reduce_sys<2, BS>(a.partials, 0, a.G, red, lds);
   if (threadIdx.x != 0) return;
   SysScal& sc = *a.sc;
   if (KIND == PF_INIT) {

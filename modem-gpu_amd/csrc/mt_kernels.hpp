@@ -24,7 +24,7 @@
 namespace mt {
 
 constexpr int BS = 256;
-enum : int { PCK_DILU = 0, PCK_JACOBI = 1, PCK_NONE = 2 };
+enum : int { PCK_DILU = 0, PCK_JACOBI = 1, PCK_NONE = 2, PCK_DILU_RB = 3 };
 enum : int { FIN_INIT = 0, FIN_ALPHA = 1, FIN_OMEGA = 2, FIN_RUPDATE = 3, FIN_RECHECK = 4 };
 
 template <class T, class V>
@@ -50,6 +50,18 @@ __device__ __forceinline__ C2<T> apply_row(const StencilView<V>& A, const RowInf
                                            const C2<T>* __restrict__ x, T dscale) {
   C2<T> y = stencil_row<V, T>(A, ri, x);
   // i*ISIGN*omega*vms1*x == (ISIGN*omega*vms1) * (i x)
+  const T d = dscale * T(A.vms1[ri.e]);
+  const C2<T> xe = x[ri.e];
+  y.x += -d * xe.y;
+  y.y += d * xe.x;
+  return y;
+}
+
+// Same with the component C known at compile time.
+template <int C, class T, class V>
+__device__ __forceinline__ C2<T> apply_row_c(const StencilView<V>& A, const RowInfo& ri,
+                                             const C2<T>* __restrict__ x, T dscale) {
+  C2<T> y = stencil_row_c<C, V, T>(A, ri, x);
   const T d = dscale * T(A.vms1[ri.e]);
   const C2<T> xe = x[ri.e];
   y.x += -d * xe.y;
@@ -130,39 +142,40 @@ __device__ inline void fin_init(SysScal& sc, Z rr, Z bb, double tol, double* rer
   sc.status = ST_RUN;
 }
 
-// One block: reduce the partials of the kernel just run, in fixed order, and
-// update the solver scalars of every active system.
+// One block per active system (grid = nact): reduce that system's partials of
+// the kernel just run, in fixed order, and update its solver scalars.
 template <int KIND>
 __global__ __launch_bounds__(BS) void k_finalize(SysScal* scs, const int* act, int nact, int G,
                                                  const Z* partials, double* rerr_base,
                                                  int rerr_len, double tol) {
-  __shared__ Z red[64 * 2];
-  reduce_partials<2>(partials, nact, G, red);
-  for (int q = threadIdx.x; q < nact; q += blockDim.x) {
-    const int s = act[q];
-    SysScal& sc = scs[s];
-    double* rerr = rerr_base ? rerr_base + (int64_t)s * rerr_len : nullptr;
-    const Z a0 = red[q * 2], a1 = red[q * 2 + 1];
-    if (KIND == FIN_INIT) {
-      fin_init(sc, a0, a1, tol, rerr);
-    } else if (KIND == FIN_ALPHA) {
-      if (sc.status != ST_RUN) continue;
-      sc.alpha = zdiv(sc.rho, a0);
-      if ((sc.alpha.x == 0.0 && sc.alpha.y == 0.0) || isnan(sc.alpha.x) || isnan(sc.alpha.y))
-        sc.status = ST_BREAK;
-    } else if (KIND == FIN_OMEGA) {
-      if (sc.status != ST_RUN) continue;
-      sc.omg = zdiv(a1, a0);
-      sc.best_due = 0;  // consumed by k_supdate of this iteration
-      if ((sc.omg.x == 0.0 && sc.omg.y == 0.0) || isnan(sc.omg.x) || isnan(sc.omg.y))
-        sc.status = ST_BREAK;
-    } else if (KIND == FIN_RUPDATE) {
-      if (sc.status != ST_RUN) continue;
-      fin_rupdate(sc, a0, a1, rerr);
-    } else if (KIND == FIN_RECHECK) {
-      if (sc.status != ST_RECHECK) continue;
-      fin_recheck(sc, a0, a1, rerr);
-    }
+  __shared__ Z lds[2 * BS / 64];
+  const int q = blockIdx.x;
+  Z red[2];
+  reduce_sys<2, BS>(partials, q, G, red, lds);
+  if (threadIdx.x != 0) return;
+  const int s = act[q];
+  SysScal& sc = scs[s];
+  double* rerr = rerr_base ? rerr_base + (int64_t)s * rerr_len : nullptr;
+  const Z a0 = red[0], a1 = red[1];
+  if (KIND == FIN_INIT) {
+    fin_init(sc, a0, a1, tol, rerr);
+  } else if (KIND == FIN_ALPHA) {
+    if (sc.status != ST_RUN) return;
+    sc.alpha = zdiv(sc.rho, a0);
+    if ((sc.alpha.x == 0.0 && sc.alpha.y == 0.0) || isnan(sc.alpha.x) || isnan(sc.alpha.y))
+      sc.status = ST_BREAK;
+  } else if (KIND == FIN_OMEGA) {
+    if (sc.status != ST_RUN) return;
+    sc.omg = zdiv(a1, a0);
+    sc.best_due = 0;  // consumed by k_supdate of this iteration
+    if ((sc.omg.x == 0.0 && sc.omg.y == 0.0) || isnan(sc.omg.x) || isnan(sc.omg.y))
+      sc.status = ST_BREAK;
+  } else if (KIND == FIN_RUPDATE) {
+    if (sc.status != ST_RUN) return;
+    fin_rupdate(sc, a0, a1, rerr);
+  } else if (KIND == FIN_RECHECK) {
+    if (sc.status != ST_RECHECK) return;
+    fin_recheck(sc, a0, a1, rerr);
   }
 }
 
@@ -177,18 +190,21 @@ __global__ __launch_bounds__(BS) void k_init(IterArgs<T, V> a) {
   const int64_t o = (int64_t)sys * a.nE;
   const T dsc = T(kIsignD * a.sc[sys].freq_omega);
   Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
-  const C2<T>* x = a.x + o;
+  const C2<T>* __restrict__ x = a.x + o;
+  const C2<T>* __restrict__ b = a.b + o;
+  C2<T>* __restrict__ r = a.r + o;
+  C2<T>* __restrict__ xb = a.xbest ? a.xbest + o : nullptr;
   int64_t lo, hi;
   chunk_range(a.A.nInt, a.G, g, lo, hi);
-  for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
-    const RowInfo ri = decode_row(a.A, rr);
-    const C2<T> bv = a.b[o + ri.e];
-    const C2<T> y = csub(bv, apply_row<T, V>(a.A, ri, x, dsc));
-    a.r[o + ri.e] = y;
-    if (a.xbest) a.xbest[o + ri.e] = x[ri.e];
+  for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
+    constexpr int C = decltype(cc)::value;
+    const C2<T> bv = b[ri.e];
+    const C2<T> y = csub(bv, apply_row_c<C, T, V>(a.A, ri, x, dsc));
+    r[ri.e] = y;
+    if (xb) xb[ri.e] = x[ri.e];
     acc[0].x += double(y.x) * y.x + double(y.y) * y.y;
     acc[1].x += double(bv.x) * bv.x + double(bv.y) * bv.y;
-  }
+  });
   store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
@@ -232,25 +248,25 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
   if (sc0.status == ST_RUN) {
     const int64_t o = (int64_t)sys * a.nE;
     const T dsc = T(kIsignD * sc0.freq_omega);
-    const C2<T>* in = (WHICH == 1 ? a.ph : a.sh) + o;
-    C2<T>* out = (WHICH == 1 ? a.v : a.t) + o;
+    const C2<T>* __restrict__ in = (WHICH == 1 ? a.ph : a.sh) + o;
+    C2<T>* __restrict__ out = (WHICH == 1 ? a.v : a.t) + o;
+    const C2<T>* __restrict__ pw = (WHICH == 1 ? a.rt : a.s) + o;  // dot partner
     int64_t lo, hi;
     chunk_range(a.A.nInt, a.G, g, lo, hi);
-    for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
-      const RowInfo ri = decode_row(a.A, rr);
-      const C2<T> y = apply_row<T, V>(a.A, ri, in, dsc);
+    for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
+      constexpr int C = decltype(cc)::value;
+      const C2<T> w = pw[ri.e];
+      const C2<T> y = apply_row_c<C, T, V>(a.A, ri, in, dsc);
       out[ri.e] = y;
       if (WHICH == 1) {
-        const C2<T> w = a.rt[o + ri.e];
         acc[0].x += double(w.x) * y.x + double(w.y) * y.y;
         acc[0].y += double(w.x) * y.y - double(w.y) * y.x;
       } else {
-        const C2<T> w = a.s[o + ri.e];
         acc[0].x += double(y.x) * y.x + double(y.y) * y.y;
         acc[1].x += double(y.x) * w.x + double(y.y) * w.y;
         acc[1].y += double(y.x) * w.y - double(y.y) * w.x;
       }
-    }
+    });
   }
   store_partials(a.partials, si, a.G, g, acc, lds);
 }
@@ -317,18 +333,21 @@ __global__ __launch_bounds__(BS) void k_recheck(IterArgs<T, V> a) {
   if (sc0.status == ST_RECHECK) {
     const int64_t o = (int64_t)sys * a.nE;
     const T dsc = T(kIsignD * sc0.freq_omega);
-    const C2<T>* x = a.x + o;
+    const C2<T>* __restrict__ x = a.x + o;
+    const C2<T>* __restrict__ b = a.b + o;
+    const C2<T>* __restrict__ rt = a.rt + o;
+    C2<T>* __restrict__ r = a.r + o;
     int64_t lo, hi;
     chunk_range(a.A.nInt, a.G, g, lo, hi);
-    for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
-      const RowInfo ri = decode_row(a.A, rr);
-      const C2<T> y = csub(a.b[o + ri.e], apply_row<T, V>(a.A, ri, x, dsc));
-      a.r[o + ri.e] = y;
-      const C2<T> w = a.rt[o + ri.e];
+    for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
+      constexpr int C = decltype(cc)::value;
+      const C2<T> w = rt[ri.e];
+      const C2<T> y = csub(b[ri.e], apply_row_c<C, T, V>(a.A, ri, x, dsc));
+      r[ri.e] = y;
       acc[0].x += double(w.x) * y.x + double(w.y) * y.y;
       acc[0].y += double(w.x) * y.y - double(w.y) * y.x;
       acc[1].x += double(y.x) * y.x + double(y.y) * y.y;
-    }
+    });
   }
   store_partials(a.partials, si, a.G, g, acc, lds);
 }
@@ -339,11 +358,13 @@ template <class T, class V>
 __global__ __launch_bounds__(BS) void k_apply(StencilView<V> A, const C2<T>* x, C2<T>* y,
                                               double omega) {
   const T dsc = T(kIsignD * omega);
-  for (int64_t rr = (int64_t)blockIdx.x * BS + threadIdx.x; rr < A.nInt;
-       rr += (int64_t)gridDim.x * BS) {
-    const RowInfo ri = decode_row(A, rr);
-    y[ri.e] = apply_row<T, V>(A, ri, x, dsc);
-  }
+  const C2<T>* __restrict__ xr = x;
+  C2<T>* __restrict__ yr = y;
+  for_rows(A, 0, A.nInt, (int64_t)blockIdx.x * BS + threadIdx.x, (int64_t)gridDim.x * BS,
+           [&](const RowInfo& ri, auto cc) {
+             constexpr int C = decltype(cc)::value;
+             yr[ri.e] = apply_row_c<C, T, V>(A, ri, xr, dsc);
+           });
 }
 
 // --------------------------------------------------- interior <-> padded
@@ -429,30 +450,38 @@ __global__ __launch_bounds__(BS) void k_outer_resid(OuterArgs a) {
   if (so0.status == ST_RUN) {
     const int64_t o = (int64_t)sys * a.nE;
     const double dsc = kIsignD * so0.freq_omega, sc = so0.scale;
+    const Z* __restrict__ x = a.x + o;
+    const Z* __restrict__ b = a.b + o;
+    C2<float>* __restrict__ r32 = a.r32 + o;
+    C2<float>* __restrict__ d32 = a.d32 + o;
     int64_t lo, hi;
     chunk_range(a.A.nInt, a.G, g, lo, hi);
-    for (int64_t rr = lo + threadIdx.x; rr < hi; rr += BS) {
-      const RowInfo ri = decode_row(a.A, rr);
-      const Z y = csub(a.b[o + ri.e], apply_row<double, double>(a.A, ri, a.x + o, dsc));
-      a.r32[o + ri.e] = {float(sc * y.x), float(sc * y.y)};
-      a.d32[o + ri.e] = {0.0f, 0.0f};
+    for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
+      constexpr int C = decltype(cc)::value;
+      const Z y = csub(b[ri.e], apply_row_c<C, double, double>(a.A, ri, x, dsc));
+      r32[ri.e] = {float(sc * y.x), float(sc * y.y)};
+      d32[ri.e] = {0.0f, 0.0f};
       acc[0].x += y.x * y.x + y.y * y.y;
-    }
+    });
   }
   store_partials(a.partials, si, a.G, g, acc, lds);
 }
 
+// One block per active system (grid = nact).
 // STAGE 0: bnorm, btol, FP32 scaling (a power of two near 1/||b||).
 // STAGE 1: outer convergence and, for systems still refining, the inner set-up.
 template <int STAGE>
 __global__ __launch_bounds__(BS) void k_outer_fin(OuterArgs a) {
-  __shared__ Z red[64 * 2];
-  reduce_partials<2>(a.partials, a.nact, a.G, red);
-  for (int q = threadIdx.x; q < a.nact; q += blockDim.x) {
+  __shared__ Z lds[2 * BS / 64];
+  const int q = blockIdx.x;
+  Z red[2];
+  reduce_sys<2, BS>(a.partials, q, a.G, red, lds);
+  if (threadIdx.x != 0) return;
+  {
     const int s = a.act[q];
     SysScal& so = a.so[s];
     if (STAGE == 0) {
-      const double bn = sqrt(red[q * 2].x);
+      const double bn = sqrt(red[0].x);
       so.bnorm = bn;
       so.btol = a.tol * bn;
       so.scale = (bn > 0.0 && isfinite(bn)) ? ldexp(1.0, -ilogb(bn)) : 1.0;
@@ -460,17 +489,17 @@ __global__ __launch_bounds__(BS) void k_outer_fin(OuterArgs a) {
       so.tot_inner = 0;
       so.status = isnan(bn) ? ST_BREAK : (bn == 0.0 ? ST_CONV : ST_RUN);
       if (isnan(bn)) so.iter = -1;
-      continue;
+      return;
     }
     SysScal& sn = a.si[s];
-    if (so.status != ST_RUN) { sn.status = ST_CONV; continue; }
-    const double rn = sqrt(red[q * 2].x);
+    if (so.status != ST_RUN) { sn.status = ST_CONV; return; }
+    const double rn = sqrt(red[0].x);
     so.rnorm = rn;
     if (a.rerr && so.iter < a.rerr_len) a.rerr[(int64_t)s * a.rerr_len + so.iter] = rn / so.bnorm;
     so.iter += 1;
-    if (rn < so.btol) { so.status = ST_CONV; sn.status = ST_CONV; continue; }
+    if (rn < so.btol) { so.status = ST_CONV; sn.status = ST_CONV; return; }
     if (so.iter > a.max_outer || so.tot_inner >= a.maxit_total) {
-      so.status = ST_MAXIT; sn.status = ST_CONV; continue;
+      so.status = ST_MAXIT; sn.status = ST_CONV; return;
     }
     const double bi = rn * so.scale;
     sn.bnorm = bi;

@@ -61,18 +61,17 @@ struct mt_ctx {
   HostOperator hop;
   DeviceOperator dop;
   TilePlan tiles;
+  TilePlan tiles_rb;  // red-black tile ordering of the same lattices
   DivcorData dc;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  // block-DILU inverse pivots per (frequency, precision, transpose), on the
-  // device: PC_setup runs once per frequency, as UpdateFreq does
-  // (modelOperator3D.f90:155-164), not once per solve
-  std::map<std::tuple<uint64_t, int, int>, void*> pivots;
-  // last solve timing of the SpMV kernels
-  int spmv_launches = 0;
-  double spmv_ms = 0.0;
-  double spmv_bytes = 0.0;
   int G_override = 0;
+  // kernel timing (mt_timing_enable): HIP events on `stream` around every
+  // SpMV launch and every DILU sweep of the BiCGStab engine
+  bool timing = false;
+  struct KTime { long long launches = 0; double ms = 0.0, bytes = 0.0; } kt[3];
+  std::vector<hipEvent_t> tev;  // event pool
+  int64_t nnz_stencil = -1;     // stored AAii couplings (cached)
 };
 
 static int mt_fail(mt_ctx* ctx, int code, const std::string& msg) {
@@ -154,7 +153,9 @@ static StencilView<V> view(const mt_ctx* ctx, bool transpose) {
 
 // Tile plan for the DILU sweeps: the tiles of all lattices grouped by tile
 // level ti+tj+tk.
-static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInfo* lat) {
+// rb = true: two groups instead, the red (ti+tj+tk even) and the black tiles.
+static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInfo* lat,
+                           bool rb = false) {
   hipFree(tp.tiles);
   hipFree(tp.shape);
   tp.tiles = nullptr;
@@ -168,7 +169,7 @@ static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInf
     for (int tk = 0; tk < NTz; ++tk)
       for (int tj = 0; tj < NTy; ++tj)
         for (int ti = 0; ti < NTx; ++ti) {
-          const int L = ti + tj + tk;
+          const int L = rb ? ((ti + tj + tk) & 1) : ti + tj + tk;
           if ((int)byLev.size() <= L) byLev.resize(L + 1);
           byLev[L].push_back(make_int4(c, ti, tj, tk));
         }
@@ -212,8 +213,6 @@ static void free_workspaces(mt_ctx* ctx) {
   hipStreamSynchronize(ctx->stream);
   for (auto& kv : ctx->ws) hipFree(kv.second.first);
   ctx->ws.clear();
-  for (auto& kv : ctx->pivots) hipFree(kv.second);
-  ctx->pivots.clear();
 }
 
 static int upload_operator(mt_ctx* ctx) {
@@ -244,7 +243,9 @@ static int upload_operator(mt_ctx* ctx) {
   d.ready = true;
   LatticeInfo lat[3];
   edge_lattices(g, lat);
-  return build_tile_plan(ctx, ctx->tiles, 3, lat);
+  int rc = build_tile_plan(ctx, ctx->tiles, 3, lat);
+  if (rc) return rc;
+  return build_tile_plan(ctx, ctx->tiles_rb, 3, lat, true);
 }
 
 static int ensure_transpose(mt_ctx* ctx) {
@@ -282,44 +283,55 @@ static int build_dinv(mt_ctx* ctx, int pc, bool transpose, const std::vector<dou
     MT_HIP_CHECK(ctx, hipGetLastError());
     return MT_OK;
   }
-  // DILU pivots on the host (sequential recurrence), uploaded per frequency
-  HostOperator tmpT;
-  const HostOperator* src = &ctx->hop;
-  if (transpose) {
-    tmpT.g = ctx->hop.g;
-    tmpT.imap = ctx->hop.imap;
-    tmpT.vms1 = ctx->hop.vms1;
-    transpose_stencil(ctx->hop, tmpT.coef);
-    src = &tmpT;
+  // DILU pivots on the device, in FP64, for all frequencies in one
+  // level-scheduled sweep (PC_setup per frequency, as UpdateFreq does)
+  const bool rb = pc == PCK_DILU_RB;
+  const TilePlan& tp = rb ? ctx->tiles_rb : ctx->tiles;
+  double* dfreq;
+  if ((rc = ws(ctx, "freqs", nf, &dfreq))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(dfreq, freqs.data(), nf * sizeof(double), hipMemcpyHostToDevice,
+                                   ctx->stream));
+  Z* p64;
+  if (sizeof(T) == 8) {
+    p64 = reinterpret_cast<Z*>(*out);
+  } else if ((rc = ws(ctx, "piv64", size_t(nf) * g.nE, &p64))) {
+    return rc;
   }
-  std::vector<zc> dv;
-  std::vector<C2<T>> hv;
-  for (int f = 0; f < nf; ++f) {
-    uint64_t bits;
-    std::memcpy(&bits, &freqs[f], sizeof(bits));
-    const auto key = std::make_tuple(bits, int(sizeof(T)), int(transpose));
-    auto it = ctx->pivots.find(key);
-    if (it == ctx->pivots.end()) {
-      dilu_inv_pivots(*src, freqs[f], dv);
-      hv.resize(g.nE);
-      for (int64_t e = 0; e < g.nE; ++e) hv[e] = {T(dv[e].real()), T(dv[e].imag())};
-      void* d = nullptr;
-      MT_HIP_CHECK(ctx, hipMalloc(&d, g.nE * sizeof(C2<T>)));
-      MT_HIP_CHECK(ctx, hipMemcpy(d, hv.data(), g.nE * sizeof(C2<T>), hipMemcpyHostToDevice));
-      it = ctx->pivots.emplace(key, d).first;
-    }
-    MT_HIP_CHECK(ctx, hipMemcpyAsync(*out + size_t(f) * g.nE, it->second, g.nE * sizeof(C2<T>),
-                                     hipMemcpyDeviceToDevice, ctx->stream));
+  PivotArgs pa;
+  pa.coef = transpose ? ctx->dop.coefT64 : ctx->dop.coef64;
+  pa.cstride = g.nE;
+  pa.vms1 = ctx->dop.vms64;
+  for (int c = 0; c < 3; ++c) pa.lat[c] = tp.lat[c];
+  pa.omegas = dfreq;
+  pa.nf = nf;
+  pa.dinv = p64;
+  for (int L = 0; L < tp.nlev; ++L) {
+    pa.tiles = tp.tiles + tp.lvl_off[L];
+    pa.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
+    if (pa.ntiles == 0) continue;
+    if (!rb)
+      k_dilu_pivots<8, 8, 8><<<pa.ntiles * nf, 64, 0, ctx->stream>>>(pa, tp.shape);
+    else if (L == 0)
+      k_dilu_pivots_rb<8, 8, 8, false><<<pa.ntiles * nf, 64, 0, ctx->stream>>>(pa, tp.shape);
+    else
+      k_dilu_pivots_rb<8, 8, 8, true><<<pa.ntiles * nf, 64, 0, ctx->stream>>>(pa, tp.shape);
   }
+  if (sizeof(T) != 8) {
+    const int64_t n = int64_t(nf) * g.nE;
+    k_cast_pivots<T><<<int(std::min<int64_t>(4096, (n + 255) / 256)), 256, 0, ctx->stream>>>(p64, *out, n);
+  }
+  MT_HIP_CHECK(ctx, hipGetLastError());
   return MT_OK;
 }
 
 // DILU apply for the active systems with status `req`: out = U^-1 L^-1 in,
-// over the lattices of `tp` with coefficients coef[slot*cstride + addr].
+// over the lattices of `tp` with coefficients coef[slot*cstride + addr]: the
+// forward (lower) sweep over the tile levels, then the backward (upper) sweep
+// in place.
 template <class T, class V>
-static void dilu_sweeps(mt_ctx* ctx, const TilePlan& tp, const V* coef, int64_t cstride,
-                        int64_t vstride, const int* act, int nact, const SysScal* sc,
-                        const C2<T>* in, C2<T>* out, const C2<T>* dinv, int req) {
+static DiluArgs<T, V> dilu_args(const TilePlan& tp, const V* coef, int64_t cstride, int64_t vstride,
+                                const int* act, int nact, const SysScal* sc, const C2<T>* in,
+                                C2<T>* out, const C2<T>* dinv, int req) {
   DiluArgs<T, V> a;
   a.coef = coef;
   a.cstride = cstride;
@@ -327,17 +339,63 @@ static void dilu_sweeps(mt_ctx* ctx, const TilePlan& tp, const V* coef, int64_t 
   for (int c = 0; c < 3; ++c) a.lat[c] = tp.lat[c];
   a.act = act; a.nact = nact; a.sc = sc; a.in = in; a.out = out; a.dinv = dinv;
   a.status_req = req;
+  a.tiles = nullptr;
+  a.ntiles = 0;
+  return a;
+}
+
+template <class T, class V>
+static void dilu_fwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
   for (int L = 0; L < tp.nlev; ++L) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
-    k_dilu_tile<T, V, 8, 8, 8, true><<<a.ntiles * nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    k_dilu_tile<T, V, 8, 8, 8, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
   }
-  a.in = out;
+}
+
+template <class T, class V>
+static void dilu_bwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
+  a.in = a.out;
   for (int L = tp.nlev - 1; L >= 0; --L) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
-    k_dilu_tile<T, V, 8, 8, 8, false><<<a.ntiles * nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    k_dilu_tile<T, V, 8, 8, 8, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
   }
+}
+
+// Red-black tile ordering: forward red then black (with the red faces),
+// backward black then red (with the black faces).
+template <class T, class V>
+static void dilu_rb_fwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
+  for (int L = 0; L < tp.nlev; ++L) {
+    a.tiles = tp.tiles + tp.lvl_off[L];
+    a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
+    if (a.ntiles == 0) continue;
+    if (L == 0) k_dilu_rb<T, V, 8, 8, 8, true, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    else k_dilu_rb<T, V, 8, 8, 8, true, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+  }
+}
+
+template <class T, class V>
+static void dilu_rb_bwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
+  a.in = a.out;
+  for (int L = tp.nlev - 1; L >= 0; --L) {
+    a.tiles = tp.tiles + tp.lvl_off[L];
+    a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
+    if (a.ntiles == 0) continue;
+    if (L == tp.nlev - 1) k_dilu_rb<T, V, 8, 8, 8, false, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    else k_dilu_rb<T, V, 8, 8, 8, false, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+  }
+}
+
+template <class T, class V>
+static void dilu_sweeps(mt_ctx* ctx, const TilePlan& tp, const V* coef, int64_t cstride,
+                        int64_t vstride, const int* act, int nact, const SysScal* sc,
+                        const C2<T>* in, C2<T>* out, const C2<T>* dinv, int req) {
+  const DiluArgs<T, V> a =
+      dilu_args<T, V>(tp, coef, cstride, vstride, act, nact, sc, in, out, dinv, req);
+  dilu_fwd<T, V>(ctx, tp, a);
+  dilu_bwd<T, V>(ctx, tp, a);
 }
 
 template <class T, class V>
@@ -346,6 +404,17 @@ static void dilu_apply(mt_ctx* ctx, bool transpose, const int* act, int nact, co
   const Grid& g = ctx->hop.g;
   dilu_sweeps<T, V>(ctx, ctx->tiles, view<V>(ctx, transpose).coef, g.nE, g.nE, act, nact, sc,
                     in, out, dinv, req);
+}
+
+static int64_t stencil_nnz(mt_ctx* ctx) {
+  if (ctx->nnz_stencil < 0) {
+    const Grid& g = ctx->hop.g;
+    int64_t z = 0;
+    for (int e : ctx->hop.imap)
+      for (int s = 0; s < kSlots; ++s) z += ctx->hop.coef[size_t(s) * g.nE + e] != 0.0;
+    ctx->nnz_stencil = z;
+  }
+  return ctx->nnz_stencil;
 }
 
 // ======================================================= BiCGStab engine
@@ -366,28 +435,87 @@ struct Engine {
   bool recheck = true;  // FP64 solves re-check the true residual; inner solves do not
   hipGraphExec_t gexec = nullptr;
   int g_nact = -1, g_chunk = -1;
+  // timing: event pairs (family, first, second) recorded by one chunk; with a
+  // graph they are recorded by the graph's event nodes on every replay
+  struct TPair { int fam, e0, e1; };
+  std::vector<TPair> tpairs;
+  int tnext = 0;
+  std::vector<int> last_iter;
+
+  int trec() {
+    hipEventRecord(ctx->tev[tnext], ctx->stream);
+    return tnext++;
+  }
+  void sweep(bool fwd, const DiluArgs<T, V>& d) {
+    if (a.pc == PCK_DILU_RB) {
+      if (fwd) dilu_rb_fwd<T, V>(ctx, ctx->tiles_rb, d);
+      else dilu_rb_bwd<T, V>(ctx, ctx->tiles_rb, d);
+    } else {
+      if (fwd) dilu_fwd<T, V>(ctx, ctx->tiles, d);
+      else dilu_bwd<T, V>(ctx, ctx->tiles, d);
+    }
+  }
+  void precond(const C2<T>* in, C2<T>* out) {
+    if (a.pc != PCK_DILU && a.pc != PCK_DILU_RB) return;
+    const Grid& g = ctx->hop.g;
+    const TilePlan& tp = a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles;
+    const DiluArgs<T, V> d = dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE,
+                                             a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN);
+    if (!ctx->timing) {
+      sweep(true, d);
+      sweep(false, d);
+      return;
+    }
+    const int e0 = trec();
+    sweep(true, d);
+    const int e1 = trec();
+    sweep(false, d);
+    const int e2 = trec();
+    tpairs.push_back({MT_KT_DILU_FWD, e0, e1});
+    tpairs.push_back({MT_KT_DILU_BWD, e1, e2});
+  }
+  template <int WHICH>
+  void spmv(int nblk) {
+    const int e0 = ctx->timing ? trec() : -1;
+    k_spmv_dot<T, V, WHICH><<<nblk, BS, 0, ctx->stream>>>(a);
+    if (ctx->timing) tpairs.push_back({MT_KT_SPMV, e0, trec()});
+  }
 
   void launch_iteration() {
     const int nblk = a.nact * a.G;
     hipStream_t s = ctx->stream;
     k_pupdate<T, V><<<nblk, BS, 0, s>>>(a);
-    if (a.pc == PCK_DILU) dilu_apply<T, V>(ctx, transpose, a.act, a.nact, a.sc, a.p, a.ph, a.dinv, ST_RUN);
-    k_spmv_dot<T, V, 1><<<nblk, BS, 0, s>>>(a);
-    k_finalize<FIN_ALPHA><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    precond(a.p, a.ph);
+    spmv<1>(nblk);
+    k_finalize<FIN_ALPHA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     k_supdate<T, V><<<nblk, BS, 0, s>>>(a);
-    if (a.pc == PCK_DILU) dilu_apply<T, V>(ctx, transpose, a.act, a.nact, a.sc, a.s, a.sh, a.dinv, ST_RUN);
-    k_spmv_dot<T, V, 2><<<nblk, BS, 0, s>>>(a);
-    k_finalize<FIN_OMEGA><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    precond(a.s, a.sh);
+    spmv<2>(nblk);
+    k_finalize<FIN_OMEGA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     k_rupdate<T, V><<<nblk, BS, 0, s>>>(a);
-    k_finalize<FIN_RUPDATE><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    k_finalize<FIN_RUPDATE><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     if (recheck) {
       k_recheck<T, V><<<nblk, BS, 0, s>>>(a);
-      k_finalize<FIN_RECHECK><<<1, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+      k_finalize<FIN_RECHECK><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     }
   }
 
+  // event pool for one chunk: 2 SpMV + 2 DILU applications per iteration
+  int ensure_events(int chunk) {
+    const size_t need = size_t(chunk) * 10 + 4;
+    while (ctx->tev.size() < need) {
+      hipEvent_t e;
+      MT_HIP_CHECK(ctx, hipEventCreate(&e));
+      ctx->tev.push_back(e);
+    }
+    return MT_OK;
+  }
+
   int run_chunk(int chunk) {
+    if (ctx->timing) { int rc = ensure_events(chunk); if (rc) return rc; }
     if (!use_graph) {
+      tpairs.clear();
+      tnext = 0;
       for (int c = 0; c < chunk; ++c) launch_iteration();
       MT_HIP_CHECK(ctx, hipGetLastError());
       return MT_OK;
@@ -398,6 +526,8 @@ struct Engine {
     }
     if (!gexec) {
       hipGraph_t graph;
+      tpairs.clear();
+      tnext = 0;
       MT_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
       for (int c = 0; c < chunk; ++c) launch_iteration();
       MT_HIP_CHECK(ctx, hipStreamEndCapture(ctx->stream, &graph));
@@ -407,6 +537,42 @@ struct Engine {
       g_chunk = chunk;
     }
     MT_HIP_CHECK(ctx, hipGraphLaunch(gexec, ctx->stream));
+    return MT_OK;
+  }
+
+  // Iteration baseline of every system (host copy of the scalars).
+  void begin(const std::vector<SysScal>& h) {
+    last_iter.resize(h.size());
+    for (size_t q = 0; q < h.size(); ++q) last_iter[q] = h[q].iter;
+  }
+  // After a chunk and the host read of the scalars: add the event times of the
+  // chunk and the algorithmic bytes of the iterations it ran.  Per iteration
+  // a system takes part in 2 SpMV launches and 2 DILU applications.
+  int account(const std::vector<SysScal>& h) {
+    if (!ctx->timing) return MT_OK;
+    long long units = 0;
+    for (size_t q = 0; q < h.size() && q < last_iter.size(); ++q) {
+      units += 2LL * std::max(0, h[q].iter - last_iter[q]);
+      last_iter[q] = h[q].iter;
+    }
+    const Grid& g = ctx->hop.g;
+    const double sv = sizeof(V), sc2 = 2.0 * sizeof(T), n = double(g.nInt);
+    int cnt[3] = {0, 0, 0};
+    for (const TPair& tp : tpairs) {
+      float ms = 0.f;
+      MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->tev[tp.e0], ctx->tev[tp.e1]));
+      mt_ctx::KTime& k = ctx->kt[tp.fam];
+      k.ms += ms;
+      k.launches += tp.fam == MT_KT_SPMV ? 1 : (a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles).nlev;
+      ++cnt[tp.fam];
+    }
+    // shared per launch (SpMV: stored couplings + frequency-free diagonal) or
+    // per sweep (DILU: the three coupling slots), plus per system-unit
+    // vectors: SpMV x, y, and the dot partner; DILU input, output, pivots
+    ctx->kt[MT_KT_SPMV].bytes += cnt[MT_KT_SPMV] * (double(stencil_nnz(ctx)) * sv + n * sv) +
+                                 (cnt[MT_KT_SPMV] ? units * 3.0 * n * sc2 : 0.0);
+    for (int f = MT_KT_DILU_FWD; f <= MT_KT_DILU_BWD; ++f)
+      ctx->kt[f].bytes += cnt[f] * 3.0 * n * sv + (cnt[f] ? units * 3.0 * n * sc2 : 0.0);
     return MT_OK;
   }
   ~Engine() {
@@ -445,10 +611,12 @@ static int compact(mt_ctx* ctx, const std::vector<SysScal>& h, int* d_act, std::
   std::vector<int> na;
   for (int s : act)
     if (keep(h[s].status)) na.push_back(s);
-  if (na.size() != act.size() && !na.empty()) {
-    hipMemcpyAsync(d_act, na.data(), na.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
-  }
-  act = na;
+  const bool changed = na.size() != act.size();
+  act.swap(na);
+  // the source is the caller's list, which outlives the copy (every caller
+  // synchronises the stream before it compacts again)
+  if (changed && !act.empty())
+    hipMemcpyAsync(d_act, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream);
   return int(act.size());
 }
 static bool keep_running(int st) { return st == ST_RUN || st == ST_RECHECK; }
@@ -522,10 +690,11 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   a.G = pick_G(ctx, nsys);
   a.tol = o.tol;
   k_init<double, double><<<a.nact * a.G, BS, 0, ctx->stream>>>(a);
-  k_finalize<FIN_INIT><<<1, BS, 0, ctx->stream>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr,
+  k_finalize<FIN_INIT><<<a.nact, BS, 0, ctx->stream>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr,
                                                   a.rerr_len, a.tol);
   MT_HIP_CHECK(ctx, hipGetLastError());
   if ((rc = read_scal(ctx, d_sc, nsys, h))) return rc;
+  E.begin(h);
   const int chunk = std::max(1, o.chunk);
   while (true) {
     const int n = compact(ctx, h, d_act, act, keep_running);
@@ -536,6 +705,7 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     }
     if ((rc = E.run_chunk(chunk))) return rc;
     if ((rc = read_scal(ctx, d_sc, nsys, h))) return rc;
+    if ((rc = E.account(h))) return rc;
   }
   // best iterate for systems that did not converge (solver.f90:1911-1919)
   for (int s = 0; s < nsys; ++s) {
@@ -643,7 +813,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   oa.nact = nsys;
   oa.G = pick_G(ctx, nsys);
   k_outer_bnorm<<<oa.nact * oa.G, BS, 0, ctx->stream>>>(oa);
-  k_outer_fin<0><<<1, BS, 0, ctx->stream>>>(oa);
+  k_outer_fin<0><<<oa.nact, BS, 0, ctx->stream>>>(oa);
   MT_HIP_CHECK(ctx, hipGetLastError());
   if ((rc = read_scal(ctx, d_so, nsys, ho))) return rc;
   const int chunk = std::max(1, o.chunk);
@@ -654,7 +824,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     oa.nact = n;
     oa.G = pick_G(ctx, n);
     k_outer_resid<<<oa.nact * oa.G, BS, 0, ctx->stream>>>(oa);
-    k_outer_fin<1><<<1, BS, 0, ctx->stream>>>(oa);
+    k_outer_fin<1><<<oa.nact, BS, 0, ctx->stream>>>(oa);
     MT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = read_scal(ctx, d_so, nsys, ho))) return rc;
     std::vector<int> outer_act;
@@ -663,6 +833,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     if (outer_act.empty()) break;
     // inner: FP32 BiCGStab on the refining systems until each stops
     if ((rc = read_scal(ctx, d_si, nsys, hi))) return rc;
+    E.begin(hi);
     std::vector<int> iact = outer_act;
     MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, iact.data(), iact.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     a.nact = int(iact.size());
@@ -670,6 +841,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     while (true) {
       if ((rc = E.run_chunk(chunk))) return rc;
       if ((rc = read_scal(ctx, d_si, nsys, hi))) return rc;
+      if ((rc = E.account(hi))) return rc;
       const int ni = compact(ctx, hi, d_act, iact, keep_running);
       if (ni == 0) break;
       if (ni != a.nact) { a.nact = ni; a.G = pick_G(ctx, ni); }
@@ -787,9 +959,12 @@ int mt_ctx_destroy(mt_ctx* ctx) {
   free_dop(ctx->dop);
   hipFree(ctx->tiles.tiles);
   hipFree(ctx->tiles.shape);
+  hipFree(ctx->tiles_rb.tiles);
+  hipFree(ctx->tiles_rb.shape);
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
+  for (auto& e : ctx->tev) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return MT_OK;
@@ -893,7 +1068,8 @@ int mt_bicg(mt_ctx* ctx, double omega, const mt_zdouble* b, mt_zdouble* x, int m
             int precond, int* niter, double* rerr, int* failed) {
   int rc = need_op(ctx);
   if (rc) return rc;
-  if (!b || !x || maxit < 1) return mt_fail(ctx, MT_E_ARG, "mt_bicg: bad arguments");
+  if (!b || !x || maxit < 1 || precond < MT_PC_DILU || precond > MT_PC_DILU_RB)
+    return mt_fail(ctx, MT_E_ARG, "mt_bicg: bad arguments");
   hipSetDevice(ctx->device);
   const Grid& g = ctx->hop.g;
   mt_solve_opts o;
@@ -923,6 +1099,10 @@ static int solve_batch_common(mt_ctx* ctx, int nsys, const double* omegas, Z* db
   mt_solve_opts o;
   if (opts) o = *opts; else mt_default_opts(&o);
   if (o.maxit < 1 || nsys < 1 || nsys > 64) return mt_fail(ctx, MT_E_ARG, "solve: 1 <= nsys <= 64, maxit >= 1");
+  if (o.precond < MT_PC_DILU || o.precond > MT_PC_DILU_RB)
+    return mt_fail(ctx, MT_E_ARG, "solve: unknown preconditioner");
+  if (o.precision != MT_PREC_FP64 && o.precision != MT_PREC_MIXED)
+    return mt_fail(ctx, MT_E_ARG, "solve: unknown precision");
   std::vector<SysScal> fin;
   int rc;
   if (o.precision == MT_PREC_MIXED) {
@@ -1031,6 +1211,8 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
   int rc = need_op(ctx);
   if (rc) return rc;
   if (!x || !y) return mt_fail(ctx, MT_E_ARG, "mt_precond_apply: null vector");
+  if (precond < MT_PC_DILU || precond > MT_PC_DILU_RB)
+    return mt_fail(ctx, MT_E_ARG, "mt_precond_apply: unknown preconditioner");
   hipSetDevice(ctx->device);
   const Grid& g = ctx->hop.g;
   std::vector<double> freqs(1, omega);
@@ -1050,6 +1232,11 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
   MT_HIP_CHECK(ctx, hipMemsetAsync(d_act, 0, sizeof(int), ctx->stream));
   if (precond == MT_PC_DILU) {
     dilu_apply<double, double>(ctx, false, d_act, 1, d_sc, dx, dy, dinv, ST_RUN);
+  } else if (precond == MT_PC_DILU_RB) {
+    const DiluArgs<double, double> d = dilu_args<double, double>(
+        ctx->tiles_rb, ctx->dop.coef64, g.nE, g.nE, d_act, 1, d_sc, dx, dy, dinv, ST_RUN);
+    dilu_rb_fwd<double, double>(ctx, ctx->tiles_rb, d);
+    dilu_rb_bwd<double, double>(ctx, ctx->tiles_rb, d);
   } else if (precond == MT_PC_JACOBI) {
     // y = dinv .* x via the p-update kernel path is overkill; do it on the host view
     std::vector<Z> hd(g.nE), hx(g.nE), hy(g.nE);
@@ -1355,11 +1542,20 @@ int mt_autotune(mt_ctx* ctx, int nsys, int* variant) {
   return MT_OK;
 }
 
-int mt_last_spmv_timing(const mt_ctx* ctx, int* launches, double* ms, double* bytes_per_launch) {
+int mt_timing_enable(mt_ctx* ctx, int on) {
   if (!ctx) return MT_E_ARG;
-  if (launches) *launches = ctx->spmv_launches;
-  if (ms) *ms = ctx->spmv_ms;
-  if (bytes_per_launch) *bytes_per_launch = ctx->spmv_bytes;
+  ctx->timing = on != 0;
+  for (auto& k : ctx->kt) k = mt_ctx::KTime();
+  return MT_OK;
+}
+
+int mt_kernel_timing(const mt_ctx* ctx, int family, long long* launches, double* ms,
+                     double* bytes) {
+  if (!ctx || family < 0 || family > 2) return MT_E_ARG;
+  const mt_ctx::KTime& k = ctx->kt[family];
+  if (launches) *launches = k.launches;
+  if (ms) *ms = k.ms;
+  if (bytes) *bytes = k.bytes;
   return MT_OK;
 }
 

@@ -21,8 +21,9 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmtsp2.so")
 
 MT_OK, MT_E_ARG, MT_E_DEVICE, MT_E_OOM, MT_E_NAN_RHS = 0, 1, 2, 3, 4
 MT_E_BREAKDOWN, MT_E_MAXIT, MT_E_STATE, MT_E_STENCIL = 5, 6, 7, 8
-PC_DILU, PC_JACOBI, PC_NONE = 0, 1, 2
+PC_DILU, PC_JACOBI, PC_NONE, PC_DILU_RB = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
+KT_SPMV, KT_DILU_FWD, KT_DILU_BWD = 0, 1, 2
 
 
 class MTError(RuntimeError):
@@ -83,7 +84,8 @@ _SIGS = {
     "mt_autotune": (_I, [_P, _I, C.POINTER(_I)]),
     "mt_set_tuning": (_I, [_P, _I]),
     "mt_bench_spmv": (_I, [_P, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
-    "mt_last_spmv_timing": (_I, [_P, C.POINTER(_I), C.POINTER(_D), C.POINTER(_D)]),
+    "mt_timing_enable": (_I, [_P, _I]),
+    "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
 }
 
 
@@ -352,6 +354,17 @@ class Context:
 
     def set_tuning(self, groups):
         self._chk(lib().mt_set_tuning(self._h, int(groups)))
+
+    def timing_enable(self, on=True):
+        self._chk(lib().mt_timing_enable(self._h, int(bool(on))))
+
+    def kernel_timing(self, family):
+        """(launches, ms, algorithmic bytes) of KT_SPMV / KT_DILU_FWD / KT_DILU_BWD
+        since timing_enable()."""
+        n, ms, by = C.c_longlong(), _D(), _D()
+        self._chk(lib().mt_kernel_timing(self._h, int(family), C.byref(n), C.byref(ms),
+                                         C.byref(by)))
+        return n.value, ms.value, by.value
 
     def bench_spmv(self, nsys, precision=PREC_FP64, reps=20):
         ms, by = _D(), _D()

@@ -12,7 +12,8 @@ import pytest
 
 import mtsp2
 import sp2oracle
-from conftest import rel, small_problem
+from conftest import (block_diag_operator, dilu_factor, dilu_mult, rb_tile_keys, rel,
+                      small_problem)
 from mtsp2 import problem as P
 
 pytestmark = pytest.mark.gpu
@@ -148,3 +149,66 @@ def test_divcor_matches_oracle(gpu_ctx):
     d_after = np.zeros(o.nNi, complex)
     sp2oracle.lib().sp2o_divc(o._h, np.ascontiguousarray(eg), d_after)
     assert np.linalg.norm(d_after) <= 1e-6 * np.linalg.norm(phi_rhs)
+
+
+def test_kernel_timing_graph_and_eager(gpu_ctx):
+    """HIP-event kernel timing: the same launches and bytes are counted with
+    hipGraph replay and with eager launches, and the times are positive."""
+    print("case kernel timing", flush=True)
+    o, g = _setup(gpu_ctx, 7, nx=12, ny=11, nzAir=4, nzEarth=8)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [0.1, 10.0])
+    res = {}
+    for graph in (1, 0):
+        gpu_ctx.timing_enable(True)
+        X, st = gpu_ctx.solve_batch(om, B, X0, tol=1e-8, precision=mtsp2.PREC_MIXED,
+                                    use_graph=graph, chunk=4)
+        res[graph] = [gpu_ctx.kernel_timing(f) for f in
+                      (mtsp2.KT_SPMV, mtsp2.KT_DILU_FWD, mtsp2.KT_DILU_BWD)]
+        gpu_ctx.timing_enable(False)
+        assert all(s["status"] == 0 for s in st)
+    for f in range(3):
+        (n1, ms1, b1), (n0, ms0, b0) = res[1][f], res[0][f]
+        assert n1 == n0 > 0 and b1 == b0 > 0
+        assert ms1 > 0 and ms0 > 0
+    # after disabling, nothing more is counted
+    gpu_ctx.solve_batch(om, B, X0, tol=1e-8, precision=mtsp2.PREC_MIXED)
+    assert gpu_ctx.kernel_timing(mtsp2.KT_SPMV) == (0, 0.0, 0.0)
+
+
+def test_dilu_rb_apply_inverts_its_factorisation(gpu_ctx):
+    """MT_PC_DILU_RB applies M^-1 with M = (P+E) P^-1 (P+F), E and F the
+    strictly lower and upper parts of the x-x, y-y, z-z blocks of A(omega) in
+    the red-black tile ordering and P the DILU pivots of that ordering
+    (dilu_factor, pinned on the natural ordering against the oracle's
+    PC_Lsolve/PC_Usolve in tests/test_oracle_invariants.py): M z = x."""
+    print("case dilu-rb apply", flush=True)
+    dx, dy, dz, nza, lc = small_problem(12, nx=20, ny=18, nzAir=4, nzEarth=10)
+    o = sp2oracle.Problem(dx, dy, dz, nza, lc)
+    gpu_ctx.assemble(dx, dy, dz, nza, lc)
+    w = 2 * np.pi * 3.0
+    key = rb_tile_keys(len(dx), len(dy), len(dz))
+    piv, lower, upper = dilu_factor(block_diag_operator(o, w), key)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal(o.nInt) + 1j * rng.standard_normal(o.nInt)
+    z = gpu_ctx.precond_apply(w, x, mtsp2.PC_DILU_RB)
+    assert rel(dilu_mult(piv, lower, upper, z), x) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [mtsp2.PREC_FP64, mtsp2.PREC_MIXED])
+def test_batch_dilu_rb_converges(gpu_ctx, precision):
+    """Solves preconditioned with the red-black tile DILU reach the FP64 true
+    residual tolerance and agree with the oracle's solution."""
+    print(f"case batch dilu-rb precision={precision}", flush=True)
+    o, g = _setup(gpu_ctx, 13, nx=20, ny=18, nzAir=4, nzEarth=10)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [1e-2, 10.0])
+    X, stats = gpu_ctx.solve_batch(om, B, X0, tol=1e-10, maxit=3000, precision=precision,
+                                   precond=mtsp2.PC_DILU_RB)
+    for s in range(len(om)):
+        o.update_freq(om[s])
+        assert stats[s]["status"] == 0, stats[s]
+        r = B[s] - o.mult_aii(X[s])
+        assert np.linalg.norm(r) / np.linalg.norm(B[s]) <= 1e-10 * 1.0001
+        xo, info = o.bicg(B[s], X0[s], maxit=3000, tol=1e-10, gpu_semantics=True)
+        assert rel(X[s], xo) <= 1e-7

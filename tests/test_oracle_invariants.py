@@ -86,6 +86,28 @@ def test_dilu_reproduces_block_pattern(prob):
     assert np.max(abs(got - Ac.data[keep])) <= 1e-12 * abs(Ac.data).max()
 
 
+def test_dilu_restatement_matches_oracle_in_natural_order(prob):
+    """The numpy DILU-in-any-ordering restatement (tests/conftest.py, used to
+    check the red-black tile DILU on the GPU) reproduces, with the natural
+    (EDGEi) ordering, the oracle's U pivots and its PC_Lsolve/PC_Usolve: the
+    oracle's M^-1 x is mapped back to x by the restated M."""
+    from conftest import block_diag_operator, dilu_factor, dilu_mult
+    w = 2 * np.pi * 0.7
+    B = block_diag_operator(prob, w)
+    piv, lower, upper = dilu_factor(B, np.arange(prob.nInt))
+    U = S(prob.lu(1))
+    assert rel(piv, U.diagonal()) <= 1e-13
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(prob.nInt) + 1j * rng.standard_normal(prob.nInt)
+    z = prob.pc_usolve(prob.pc_lsolve(x))
+    assert rel(dilu_mult(piv, lower, upper, z), x) <= 1e-12
+    # a plausible slip -- a lower coupling dropped -- is caught
+    bad = lower.tolil()
+    r0 = bad.rows[prob.nInt // 2][0]
+    bad[prob.nInt // 2, r0] = 0.0
+    assert rel(dilu_mult(piv, bad.tocsr(), upper, z), x) > 1e-9
+
+
 def test_bicg_matches_dense_solve():
     dx, dy, dz, nza, lc = small_problem(22, nx=6, ny=5, nzAir=3, nzEarth=4)
     p = sp2oracle.Problem(dx, dy, dz, nza, lc)
