@@ -234,7 +234,9 @@ int mt_bench_spmv(mt_ctx *ctx, int nsys, int precision, int reps, double *ms_per
  * includes the gaps between them).  launches: kernel launches covered; ms:
  * summed event time; bytes: algorithmic bytes of those launches (SpMV: stored
  * couplings + diagonal per launch, x, y and the dot partner per system; DILU:
- * three coupling slots per sweep, input, output and pivots per system). */
+ * three coupling slots per sweep, input, output and pivots per system).
+ * While timing is on, solves launch their kernels eagerly (use_graph is
+ * ignored): HIP does not time events recorded by graph nodes. */
 #define MT_KT_SPMV 0
 #define MT_KT_DILU_FWD 1
 #define MT_KT_DILU_BWD 2

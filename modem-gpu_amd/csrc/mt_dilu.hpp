@@ -53,6 +53,7 @@ struct DiluArgs {
   int nact;
   const SysScal* sc;
   const C2<T>* in;         // forward: right-hand side
+  const C2<T>* in2;        // forward, when set: right-hand side in - alpha*in2
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed
@@ -93,6 +94,8 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
   // uniform base pointers, held in registers (a per-lane choice between two
   // kernel-argument fields would be re-read from the argument segment)
   const C2<T>* __restrict__ pin = a.in + o;
+  const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
+  const C2<T> al = ccast<T>(a.sc[sys].alpha);
   const C2<T>* __restrict__ pout = a.out + o;
   const C2<T>* __restrict__ dinv = a.dinv + (int64_t)a.sc[sys].fidx * a.cstride;
   const int s0 = FWD ? 1 : 2;  // lower slots 1,3,5 / upper slots 2,4,6
@@ -118,6 +121,7 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
       const int64_t e = lat_e(L, gi, gj, gk);
       const C2<T>* src = (FWD && inTile) ? pin : pout;
       yv[u] = src[e];
+      if (FWD && inTile && pin2) yv[u] = csub(yv[u], cmul(al, pin2[e]));
       if (FWD || inTile) dd[u] = dinv[e];
       if (inTile) {
 #pragma unroll
@@ -247,6 +251,8 @@ __global__ __launch_bounds__(64) void k_dilu_rb(DiluArgs<T, V> a, const TileShap
   const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
   const int64_t o = (int64_t)sys * a.vstride;
   const C2<T>* __restrict__ pin = a.in + o;
+  const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
+  const C2<T> al = ccast<T>(a.sc[sys].alpha);
   const C2<T>* __restrict__ pout = a.out + o;
   const C2<T>* __restrict__ dinv = a.dinv + (int64_t)a.sc[sys].fidx * a.cstride;
   const V* __restrict__ cfs = a.coef;
@@ -267,6 +273,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb(DiluArgs<T, V> a, const TileShap
       const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
       const int64_t e = lat_e(L, gi, gj, gk);
       rv[u] = (FWD ? pin : pout)[e];
+      if (FWD && pin2) rv[u] = csub(rv[u], cmul(al, pin2[e]));
       dd[u] = dinv[e];
       // in-tile couplings: lower (slots 1,3,5) forward, upper (2,4,6) backward
       const bool has0 = FWD ? li > 0 : li + 1 < ex;

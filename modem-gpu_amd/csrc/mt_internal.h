@@ -125,6 +125,7 @@ struct DevBuf {
   do {                                                                           \
     hipError_t e_ = (call);                                                      \
     if (e_ != hipSuccess) {                                                      \
+      (void)hipGetLastError(); /* clear it: the next call must not report it */  \
       return mt_fail((ctx), MT_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     }                                                                            \
   } while (0)

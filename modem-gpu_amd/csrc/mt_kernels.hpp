@@ -11,6 +11,8 @@
 //   k_spmv_dot<2> t = A sh, partials of <t,t>, <t,s> -> k_finalize<FIN_OMEGA>
 //   k_rupdate     x += omega sh, r = s - omega t, partials of <rt,r>, <r,r>
 //                                                    -> k_finalize<FIN_RUPDATE>
+//   (mixed inner solves with DILU: k_supdate is folded into the
+//    preconditioner input, the <t,s> dot and k_rupdate_fused)
 //   k_recheck     only systems whose recursive residual met tol:
 //                 r = b - A x (solver.f90:1855-1895)  -> k_finalize<FIN_RECHECK>
 // The finalize kernels hold all solver scalar logic (convergence, breakdown,
@@ -43,6 +45,10 @@ struct IterArgs {
   double* rerr;          // [nsys][rerr_len]
   int rerr_len;
   double tol;
+  // fused = 1 (inner solves of the mixed solver with a DILU preconditioner, no
+  // best iterate): s = r - alpha v is never stored; the preconditioner, the
+  // <t,s> dot and k_rupdate_fused recompute it
+  int fused = 0;
 };
 
 template <class T, class V>
@@ -250,12 +256,17 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
     const T dsc = T(kIsignD * sc0.freq_omega);
     const C2<T>* __restrict__ in = (WHICH == 1 ? a.ph : a.sh) + o;
     C2<T>* __restrict__ out = (WHICH == 1 ? a.v : a.t) + o;
-    const C2<T>* __restrict__ pw = (WHICH == 1 ? a.rt : a.s) + o;  // dot partner
+    // dot partner: rt, or s = r - alpha v (stored, or recomputed when fused)
+    const bool fz = WHICH == 2 && a.fused;
+    const C2<T>* __restrict__ pw = (WHICH == 1 ? a.rt : (fz ? a.r : a.s)) + o;
+    const C2<T>* __restrict__ pv = a.v + o;
+    const C2<T> al = ccast<T>(sc0.alpha);
     int64_t lo, hi;
     chunk_range(a.A.nInt, a.G, g, lo, hi);
     for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
       constexpr int C = decltype(cc)::value;
-      const C2<T> w = pw[ri.e];
+      C2<T> w = pw[ri.e];
+      if (fz) w = csub(w, cmul(al, pv[ri.e]));
       const C2<T> y = apply_row_c<C, T, V>(a.A, ri, in, dsc);
       out[ri.e] = y;
       if (WHICH == 1) {
@@ -316,6 +327,43 @@ __global__ __launch_bounds__(BS) void k_rupdate(IterArgs<T, V> a) {
       acc[0].x += double(w.x) * r.x + double(w.y) * r.y;
       acc[0].y += double(w.x) * r.y - double(w.y) * r.x;
       acc[1].x += double(r.x) * r.x + double(r.y) * r.y;
+    }
+  }
+  store_partials(a.partials, si, a.G, g, acc, lds);
+}
+
+// Fused form for the mixed inner solves: x += alpha ph + omega sh,
+// r = (r - alpha v) - omega t, partials of <rt,r>, <r,r>.  The same
+// operations, in the same order, as k_supdate followed by k_rupdate.
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_rupdate_fused(IterArgs<T, V> a) {
+  __shared__ Z lds[2 * BS / 64];
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc0 = a.sc[sys];
+  Z acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+  if (sc0.status == ST_RUN) {
+    const int64_t o = (int64_t)sys * a.nE;
+    const C2<T> al = ccast<T>(sc0.alpha), om = ccast<T>(sc0.omg);
+    C2<T>* __restrict__ x = a.x + o;
+    C2<T>* __restrict__ r = a.r + o;
+    const C2<T>* __restrict__ ph = a.ph + o;
+    const C2<T>* __restrict__ sh = a.sh + o;
+    const C2<T>* __restrict__ v = a.v + o;
+    const C2<T>* __restrict__ t = a.t + o;
+    const C2<T>* __restrict__ rt = a.rt + o;
+    int64_t lo, hi;
+    chunk_range(a.nE, a.G, g, lo, hi);
+#pragma unroll 2
+    for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
+      x[e] = cadd(cadd(x[e], cmul(al, ph[e])), cmul(om, sh[e]));
+      const C2<T> rn = csub(csub(r[e], cmul(al, v[e])), cmul(om, t[e]));
+      r[e] = rn;
+      const C2<T> w = rt[e];
+      acc[0].x += double(w.x) * rn.x + double(w.y) * rn.y;
+      acc[0].y += double(w.x) * rn.y - double(w.y) * rn.x;
+      acc[1].x += double(rn.x) * rn.x + double(rn.y) * rn.y;
     }
   }
   store_partials(a.partials, si, a.G, g, acc, lds);

@@ -338,6 +338,7 @@ static DiluArgs<T, V> dilu_args(const TilePlan& tp, const V* coef, int64_t cstri
   a.vstride = vstride;
   for (int c = 0; c < 3; ++c) a.lat[c] = tp.lat[c];
   a.act = act; a.nact = nact; a.sc = sc; a.in = in; a.out = out; a.dinv = dinv;
+  a.in2 = nullptr;
   a.status_req = req;
   a.tiles = nullptr;
   a.ntiles = 0;
@@ -455,12 +456,14 @@ struct Engine {
       else dilu_bwd<T, V>(ctx, ctx->tiles, d);
     }
   }
-  void precond(const C2<T>* in, C2<T>* out) {
+  // out = M^-1 (in - alpha*in2) (in2 may be null)
+  void precond(const C2<T>* in, C2<T>* out, const C2<T>* in2 = nullptr) {
     if (a.pc != PCK_DILU && a.pc != PCK_DILU_RB) return;
     const Grid& g = ctx->hop.g;
     const TilePlan& tp = a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles;
-    const DiluArgs<T, V> d = dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE,
-                                             a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN);
+    DiluArgs<T, V> d = dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE,
+                                       a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN);
+    d.in2 = in2;
     if (!ctx->timing) {
       sweep(true, d);
       sweep(false, d);
@@ -488,11 +491,16 @@ struct Engine {
     precond(a.p, a.ph);
     spmv<1>(nblk);
     k_finalize<FIN_ALPHA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
-    k_supdate<T, V><<<nblk, BS, 0, s>>>(a);
-    precond(a.s, a.sh);
+    if (a.fused) {
+      precond(a.r, a.sh, a.v);  // sh = M^-1 (r - alpha v)
+    } else {
+      k_supdate<T, V><<<nblk, BS, 0, s>>>(a);
+      precond(a.s, a.sh);
+    }
     spmv<2>(nblk);
     k_finalize<FIN_OMEGA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
-    k_rupdate<T, V><<<nblk, BS, 0, s>>>(a);
+    if (a.fused) k_rupdate_fused<T, V><<<nblk, BS, 0, s>>>(a);
+    else k_rupdate<T, V><<<nblk, BS, 0, s>>>(a);
     k_finalize<FIN_RUPDATE><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
     if (recheck) {
       k_recheck<T, V><<<nblk, BS, 0, s>>>(a);
@@ -513,7 +521,8 @@ struct Engine {
 
   int run_chunk(int chunk) {
     if (ctx->timing) { int rc = ensure_events(chunk); if (rc) return rc; }
-    if (!use_graph) {
+    // HIP cannot time events recorded by graph nodes: timing launches eagerly
+    if (!use_graph || ctx->timing) {
       tpairs.clear();
       tnext = 0;
       for (int c = 0; c < chunk; ++c) launch_iteration();
@@ -758,6 +767,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   a.x = d32;
   a.xbest = nullptr;
   a.b = nullptr;
+  a.fused = (o.precond == PCK_DILU || o.precond == PCK_DILU_RB) ? 1 : 0;
   C2<float>* dinv = nullptr;
   if ((rc = build_dinv<float>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   a.dinv = dinv;
