@@ -7,11 +7,15 @@ relative residual <= 1e-7 for every system.
 
 A step is one pass of the hot path over the batch: the 16 forward systems
 solved together in the mixed-precision batched solver (FP32 BiCGStab inner
-iterations, FP64 defect correction, block-DILU, hipGraph-replayed chunks), with
+iterations preconditioned with the red-black tile block-DILU, FP64 defect
+correction, DILU pivots computed on the device inside the step), with
 right-hand sides and initial guesses already resident in HBM.  value =
-solves/s over the timed steps.  Beside it: the plain FP64 GPU solver (one
-system at a time, the drop-in cuBiCG semantics), the fused SpMV kernel's HBM
-roofline from HIP events, and the reference CPU path timed on a bounded sample.
+solves/s over the timed steps, which run with HIP events around every SpMV
+launch and preconditioner application (the roofline figures come from them).
+Beside it: the same steps replayed as hipGraphs, the plain FP64 GPU solver
+(one system at a time, block-DILU in the reference ordering: the drop-in
+cuBiCG semantics), the isolated SpMV kernel, and the reference CPU path timed
+on a bounded sample.
 """
 import argparse
 import json
@@ -75,7 +79,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-7)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--precond", type=int, default=0)
+    ap.add_argument("--precond", type=int, default=3,
+                    help="preconditioner of the mixed solver's FP32 inner solves "
+                         "(0 block-DILU in the reference ordering, 3 red-black tile block-DILU); "
+                         "the plain FP64 baseline always uses the reference ordering")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
     args = ap.parse_args()
@@ -124,7 +131,7 @@ def main():
         st = step()
     el = time.time() - t0
     kt = {name: ctx.kernel_timing(fam) for name, fam in
-          (("spmv", mtsp2.KT_SPMV), ("dilu_fwd", mtsp2.KT_DILU_FWD), ("dilu_bwd", mtsp2.KT_DILU_BWD))}
+          (("spmv", mtsp2.KT_SPMV), ("precond", mtsp2.KT_PRECOND))}
     ctx.timing_enable(False)
     for name, (nl, kms, kby) in kt.items():
         if nl:
@@ -157,7 +164,7 @@ def main():
             xs = d_x + s * n * 16
             ctx.dev_copy(xs, d_x0 + s * n * 16, n * 16)
             r = ctx.solve_batch_dev(om[s:s + 1], bs, xs, tol=args.tol, maxit=4000,
-                                    precision=mtsp2.PREC_FP64, precond=args.precond,
+                                    precision=mtsp2.PREC_FP64, precond=mtsp2.PC_DILU,
                                     chunk=10, use_graph=opts["use_graph"])
             its.append(r[0]["niter"])
         fp64 = nsys / (time.time() - t0)
@@ -166,9 +173,10 @@ def main():
 
     dom = max(kt, key=lambda k: kt[k][1])
     nl, kms, kby = kt[dom]
+    pc_kernels = {0: "k_dilu_tile<float,float,8,8,8,true|false> (block-DILU level sweeps)",
+                  3: "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD> (red-black tile block-DILU)"}
     kernels = {"spmv": "k_spmv_dot<float,float,1|2> (v = A x with its dot products)",
-               "dilu_fwd": "k_dilu_tile<float,float,8,8,8,true> (block-DILU forward sweep)",
-               "dilu_bwd": "k_dilu_tile<float,float,8,8,8,false> (block-DILU backward sweep)"}
+               "precond": pc_kernels.get(args.precond, "preconditioner")}
     achieved = kby / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": kernels[dom],

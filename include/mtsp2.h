@@ -229,17 +229,17 @@ int mt_bench_spmv(mt_ctx *ctx, int nsys, int precision, int reps, double *ms_per
 /* Kernel timing of the BiCGStab engine with HIP events on the solver stream,
  * summed over every solve since the last mt_timing_enable (which resets it;
  * on = 0 turns the events off).  family MT_KT_SPMV: one event pair around each
- * fused SpMV+dot launch; MT_KT_DILU_FWD / MT_KT_DILU_BWD: one pair around each
- * forward / backward block-DILU sweep (one launch per tile level, so the time
- * includes the gaps between them).  launches: kernel launches covered; ms:
- * summed event time; bytes: algorithmic bytes of those launches (SpMV: stored
- * couplings + diagonal per launch, x, y and the dot partner per system; DILU:
- * three coupling slots per sweep, input, output and pivots per system).
- * While timing is on, solves launch their kernels eagerly (use_graph is
- * ignored): HIP does not time events recorded by graph nodes. */
+ * fused SpMV+dot launch; MT_KT_PRECOND: one pair around each preconditioner
+ * application (block DILU: one launch per tile level and sweep; red-black
+ * tile DILU: three launches), so its time includes the gaps between those
+ * launches.  launches: kernel launches covered; ms: summed event time; bytes:
+ * algorithmic bytes of those launches (SpMV: stored couplings + diagonal per
+ * launch, x, y and the dot partner per system; preconditioner: the six
+ * lattice couplings per application, right-hand side(s), result and pivots
+ * per system).  While timing is on, solves launch their kernels eagerly
+ * (use_graph is ignored): HIP does not time events recorded by graph nodes. */
 #define MT_KT_SPMV 0
-#define MT_KT_DILU_FWD 1
-#define MT_KT_DILU_BWD 2
+#define MT_KT_PRECOND 1
 int mt_timing_enable(mt_ctx *ctx, int on);
 int mt_kernel_timing(const mt_ctx *ctx, int family, long long *launches, double *ms,
                      double *bytes);

@@ -362,6 +362,280 @@ __global__ __launch_bounds__(64) void k_dilu_rb(DiluArgs<T, V> a, const TileShap
   }
 }
 
+// ------------------------------------------- red-black sweeps, persistent
+// The red-black application as three launches over 8x8x8 tiles:
+//   RB_RED_FWD   red tiles, forward:  y = L^-1 r on each tile alone;
+//   RB_BLACK_FB  black tiles: forward with the red face neighbours (all six
+//                faces are lower for a black point) and, since a black
+//                point's upper neighbours all lie in its own tile, the
+//                backward sweep of the same tile right after, in LDS;
+//   RB_RED_BWD   red tiles, backward with the black face neighbours (upper).
+// Work items are (tile of the colour, active system) pairs, systems fastest.
+// The items are cut into 8 contiguous ranges, one per XCD (workgroups are
+// dispatched round-robin over the XCDs), and the blocks of an XCD walk its
+// range, so concurrent blocks of one XCD work on the same tiles for different
+// systems and share the tile's couplings in that XCD's L2.  A block is
+// persistent: while it sweeps one item out of LDS, the loads of its next item
+// are in flight.  The box holds the tile and a one-point halo on all six
+// faces (box stride 1, 10, 100).
+enum : int { RB_RED_FWD = 0, RB_BLACK_FB = 1, RB_RED_BWD = 2 };
+
+template <class T, class V, int MODE>
+__global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileShape* shp) {
+  constexpr int TX = 8, NP = 512, NU = 8, NLEV = 22, BX = 10, NB = 1000;
+  constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
+  constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
+  constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
+  __shared__ C2<T> W[NB];                     // box: w = dinv*y (forward), y -> z (backward)
+  __shared__ C2<T> RB[FWDP ? NP : 1];         // rhs -> y on the tile
+  __shared__ C2<T> DV[NP];
+  __shared__ V CL[FWDP ? 3 : 1][FWDP ? NP : 1];  // lower couplings used by the forward sweep
+  __shared__ V CH[BWDP ? 3 : 1][BWDP ? NP : 1];  // upper couplings used by the backward sweep
+  __shared__ int s_sys[64], s_ok[64], s_fidx[64];
+  __shared__ C2<T> s_al[64];
+  const int lane = threadIdx.x;
+  const int nwork = a.ntiles * a.nact;
+  // the active systems (at most 64): index, status, frequency slot, alpha
+  if (lane < a.nact) {
+    const int sy = a.act[lane];
+    const SysScal& sc = a.sc[sy];
+    s_sys[lane] = sy;
+    s_ok[lane] = sc.status == a.status_req;
+    s_fidx[lane] = sc.fidx;
+    s_al[lane] = ccast<T>(sc.alpha);
+  }
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, gx = gridDim.x >> 3;
+  const int chunk = (nwork + 7) / 8;
+  const int w0 = xcd * chunk + lb, wend = min(nwork, (xcd + 1) * chunk);
+  // this lane's point on every hyperplane of the tile (at most 48 per plane)
+  unsigned int pk[NLEV];
+#pragma unroll
+  for (int h = 0; h < NLEV; ++h) {
+    const int b = shp->lvl[h], e = shp->lvl[h + 1];
+    pk[h] = (b + lane < e) ? shp->pts[b + lane] : 0xffffffffu;
+  }
+  for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
+  __syncthreads();
+  const V* __restrict__ cfs = a.coef;
+  const int64_t cst = a.cstride;
+  // ---- prefetch registers (one item)
+  C2<T> rv[NU], v2[NU], dd[NU], fv[6], fd[6];
+  V cl[NU][3], ch[NU][3];
+  auto item_ok = [&](int w) { return w < wend && s_ok[w % a.nact]; };
+  int4 tl_next = make_int4(0, 0, 0, 0);
+  auto issue = [&](int w) {
+    const int tix = w / a.nact, si = w - tix * a.nact, sys = s_sys[si];
+    const int4 tl = a.tiles[tix];
+    tl_next = tl;
+    const LatticeInfo L = a.lat[tl.x];
+    const int gi0 = tl.y * TX, gj0 = tl.z * TX, gk0 = tl.w * TX;
+    const int ex = min(TX, L.A - gi0), ey = min(TX, L.B - gj0), ez = min(TX, L.C - gk0);
+    const int64_t o = (int64_t)sys * a.vstride;
+    const C2<T>* __restrict__ pin = a.in + o;
+    const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
+    const C2<T>* __restrict__ pout = a.out + o;
+    const C2<T>* __restrict__ dinv = a.dinv + (int64_t)s_fidx[si] * cst;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      rv[u] = v2[u] = dd[u] = {T(0), T(0)};
+#pragma unroll
+      for (int s = 0; s < 3; ++s) cl[u][s] = ch[u][s] = V(0);
+      if (li < ex && lj < ey && lk < ez) {
+        const int64_t e = lat_e(L, gi0 + li, gj0 + lj, gk0 + lk);
+        rv[u] = FWDP ? pin[e] : pout[e];
+        if (FWDP && pin2) v2[u] = pin2[e];
+        dd[u] = dinv[e];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          if (FWDP || MODE == RB_RED_BWD) cl[u][s] = cfs[(int64_t)(1 + 2 * s) * cst + e];
+          if (BWDP) ch[u][s] = cfs[(int64_t)(2 + 2 * s) * cst + e];
+        }
+      }
+    }
+    if (HALO) {
+      // face f: lane (fa, fb) of the 8x8 face; the tile point next to the
+      // face must be in the lattice, and its neighbour across the face too
+      const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int ti, tj, tk;   // tile point next to the face
+        if (dir == 0) { ti = hi ? 7 : 0; tj = fa; tk = fb; }
+        else if (dir == 1) { ti = fa; tj = hi ? 7 : 0; tk = fb; }
+        else { ti = fa; tj = fb; tk = hi ? 7 : 0; }
+        const int ni = gi0 + ti + (dir == 0 ? (hi ? 1 : -1) : 0);
+        const int nj = gj0 + tj + (dir == 1 ? (hi ? 1 : -1) : 0);
+        const int nk = gk0 + tk + (dir == 2 ? (hi ? 1 : -1) : 0);
+        fv[f] = fd[f] = {T(0), T(0)};
+        const bool ok = ti < ex && tj < ey && tk < ez && ni >= 0 && nj >= 0 && nk >= 0 &&
+                        ni < L.A && nj < L.B && nk < L.C;
+        if (ok) {
+          const int64_t l = lat_e(L, ni, nj, nk);
+          fv[f] = pout[l];
+          if (MODE == RB_BLACK_FB) fd[f] = dinv[l];
+        }
+      }
+    }
+  };
+  int w = w0;
+  while (w < wend && !item_ok(w)) w += gx;
+  if (w < wend) issue(w);
+  for (; w < wend;) {
+    const int tix = w / a.nact, si = w - tix * a.nact, sys = s_sys[si];
+    const int4 tl = tl_next;
+    const LatticeInfo L = a.lat[tl.x];
+    const int gi0 = tl.y * TX, gj0 = tl.z * TX, gk0 = tl.w * TX;
+    const int ex = min(TX, L.A - gi0), ey = min(TX, L.B - gj0), ez = min(TX, L.C - gk0);
+    const int64_t o = (int64_t)sys * a.vstride;
+    const C2<T> al = s_al[si];
+    // ---- stage the item into LDS
+    if (HALO) {
+      const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int q;
+        if (dir == 0) q = (hi ? 9 : 0) + BX * ((fa + 1) + BX * (fb + 1));
+        else if (dir == 1) q = (fa + 1) + BX * ((hi ? 9 : 0) + BX * (fb + 1));
+        else q = (fa + 1) + BX * ((fb + 1) + BX * (hi ? 9 : 0));
+        W[q] = MODE == RB_BLACK_FB ? cmul(fd[f], fv[f]) : fv[f];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+      const bool in0 = li > 0, in1 = lj > 0, in2 = lk > 0;                      // lower in tile
+      const bool up0 = li + 1 < ex, up1 = lj + 1 < ey, up2 = lk + 1 < ez;      // upper in tile
+      C2<T> r = rv[u];
+      if (FWDP && a.in2) r = csub(r, cmul(al, v2[u]));
+      DV[p] = dd[u];
+      if (FWDP) {
+        RB[p] = r;
+        // red: face neighbours are upper, so only in-tile lower couplings;
+        // black: every lower-side coupling (in tile or across the face)
+        CL[0][p] = (MODE == RB_BLACK_FB || in0) ? cl[u][0] : V(0);
+        CL[1][p] = (MODE == RB_BLACK_FB || in1) ? cl[u][1] : V(0);
+        CL[2][p] = (MODE == RB_BLACK_FB || in2) ? cl[u][2] : V(0);
+      } else {
+        W[q] = r;
+      }
+      if (BWDP) {
+        // black: upper neighbours in the tile only; red: in tile or across
+        CH[0][p] = (MODE == RB_RED_BWD || up0) ? ch[u][0] : V(0);
+        CH[1][p] = (MODE == RB_RED_BWD || up1) ? ch[u][1] : V(0);
+        CH[2][p] = (MODE == RB_RED_BWD || up2) ? ch[u][2] : V(0);
+      }
+    }
+    __syncthreads();
+    // fold the face terms the sweeps do not reach: black forward, the high
+    // faces (red neighbours above); red backward, the low faces (black below)
+    if (HALO) {
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int p = lane + 64 * u;
+        const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+        if (!(li < ex && lj < ey && lk < ez)) continue;
+        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        if (MODE == RB_BLACK_FB) {
+          C2<T> r = RB[p];
+          const bool f0 = li == TX - 1, f1 = lj == TX - 1, f2 = lk == TX - 1;
+          const int st[3] = {1, BX, BX * BX};
+          const bool fl[3] = {f0, f1, f2};
+#pragma unroll
+          for (int s = 0; s < 3; ++s) {
+            if (!fl[s]) continue;
+            const T c = T(ch[u][s]);
+            const C2<T> v = W[q + st[s]];
+            r.x -= c * v.x;
+            r.y -= c * v.y;
+          }
+          RB[p] = r;
+        } else {
+          C2<T> y = W[q];
+          const int st[3] = {1, BX, BX * BX};
+          const bool fl[3] = {li == 0, lj == 0, lk == 0};
+#pragma unroll
+          for (int s = 0; s < 3; ++s) {
+            if (!fl[s]) continue;
+            const T c = T(cl[u][s]);
+            const C2<T> v = W[q - st[s]];
+            y.x -= c * v.x;
+            y.y -= c * v.y;
+          }
+          W[q] = y;
+        }
+      }
+      __syncthreads();
+    }
+    // ---- the next item's loads go out now and land during the sweeps
+    int wn = w + gx;
+    while (wn < wend && !item_ok(wn)) wn += gx;
+    if (wn < wend) issue(wn);
+    // ---- sweeps
+    if (FWDP) {
+#pragma unroll
+      for (int h = 0; h < NLEV; ++h) {
+        const unsigned int pt = pk[h];
+        const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+        if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
+          const int p = li + TX * (lj + TX * lk);
+          const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+          C2<T> y = RB[p];
+          const int qn[3] = {q - 1, q - BX, q - BX * BX};
+#pragma unroll
+          for (int s = 0; s < 3; ++s) {
+            const T c = T(CL[s][p]);
+            const C2<T> v = W[qn[s]];
+            y.x -= c * v.x;
+            y.y -= c * v.y;
+          }
+          RB[p] = y;
+          W[q] = cmul(DV[p], y);
+        }
+        __syncthreads();
+      }
+    }
+    if (BWDP) {
+#pragma unroll
+      for (int h = NLEV - 1; h >= 0; --h) {
+        const unsigned int pt = pk[h];
+        const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+        if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
+          const int p = li + TX * (lj + TX * lk);
+          const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+          C2<T> y = MODE == RB_BLACK_FB ? RB[p] : W[q];
+          const int qn[3] = {q + 1, q + BX, q + BX * BX};
+#pragma unroll
+          for (int s = 0; s < 3; ++s) {
+            const T c = T(CH[s][p]);
+            const C2<T> z = W[qn[s]];
+            y.x -= c * z.x;
+            y.y -= c * z.y;
+          }
+          W[q] = cmul(DV[p], y);
+        }
+        __syncthreads();
+      }
+    }
+    // ---- write the tile back (red forward: y; otherwise z)
+    C2<T>* __restrict__ dst = a.out + o;
+    for (int p = lane; p < NP; p += 64) {
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      if (!(li < ex && lj < ey && lk < ez)) continue;
+      const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+      dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = MODE == RB_RED_FWD ? RB[p] : W[q];
+    }
+    __syncthreads();
+    w = wn;
+  }
+}
+
 // ------------------------------------------------------------- pivots
 // PC_setup's block-DILU pivots on the device (Dilu_Cmplx_AS, spOpTools.f90:
 // 1996-2055, on the x-x, y-y, z-z blocks of A(omega)):

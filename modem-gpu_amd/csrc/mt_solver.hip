@@ -66,10 +66,11 @@ struct mt_ctx {
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int G_override = 0;
+  int num_cu = 256;             // compute units of the device
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
   // SpMV launch and every DILU sweep of the BiCGStab engine
   bool timing = false;
-  struct KTime { long long launches = 0; double ms = 0.0, bytes = 0.0; } kt[3];
+  struct KTime { long long launches = 0; double ms = 0.0, bytes = 0.0; } kt[2];
   std::vector<hipEvent_t> tev;  // event pool
   int64_t nnz_stencil = -1;     // stored AAii couplings (cached)
 };
@@ -364,28 +365,28 @@ static void dilu_bwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
   }
 }
 
-// Red-black tile ordering: forward red then black (with the red faces),
-// backward black then red (with the black faces).
-template <class T, class V>
-static void dilu_rb_fwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
-  for (int L = 0; L < tp.nlev; ++L) {
-    a.tiles = tp.tiles + tp.lvl_off[L];
-    a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
-    if (a.ntiles == 0) continue;
-    if (L == 0) k_dilu_rb<T, V, 8, 8, 8, true, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
-    else k_dilu_rb<T, V, 8, 8, 8, true, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
-  }
+// Red-black tile ordering: red forward, black forward+backward, red
+// backward, each one persistent launch.
+static int rb_grid(mt_ctx* ctx, int nwork) {
+  // 8 blocks per CU (the LDS of one block is 18-28 KB), a multiple of 8 XCDs
+  int g = std::min(nwork, ctx->num_cu * 8);
+  g = std::max(8, (g + 7) / 8 * 8);
+  return g;
 }
 
 template <class T, class V>
-static void dilu_rb_bwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
-  a.in = a.out;
-  for (int L = tp.nlev - 1; L >= 0; --L) {
+static void dilu_rb_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
+  const int nl = tp.nlev;  // 2: red, black (a lattice of one tile has no black level)
+  for (int step = 0; step < 3; ++step) {
+    const int L = (step == 1) ? 1 : 0;
+    if (L >= nl) continue;
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
     if (a.ntiles == 0) continue;
-    if (L == tp.nlev - 1) k_dilu_rb<T, V, 8, 8, 8, false, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
-    else k_dilu_rb<T, V, 8, 8, 8, false, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    const int g = rb_grid(ctx, a.ntiles * a.nact);
+    if (step == 0) k_dilu_rb2<T, V, RB_RED_FWD><<<g, 64, 0, ctx->stream>>>(a, tp.shape);
+    else if (step == 1) k_dilu_rb2<T, V, RB_BLACK_FB><<<g, 64, 0, ctx->stream>>>(a, tp.shape);
+    else k_dilu_rb2<T, V, RB_RED_BWD><<<g, 64, 0, ctx->stream>>>(a, tp.shape);
   }
 }
 
@@ -447,13 +448,12 @@ struct Engine {
     hipEventRecord(ctx->tev[tnext], ctx->stream);
     return tnext++;
   }
-  void sweep(bool fwd, const DiluArgs<T, V>& d) {
+  void apply_pc(const DiluArgs<T, V>& d) {
     if (a.pc == PCK_DILU_RB) {
-      if (fwd) dilu_rb_fwd<T, V>(ctx, ctx->tiles_rb, d);
-      else dilu_rb_bwd<T, V>(ctx, ctx->tiles_rb, d);
+      dilu_rb_apply<T, V>(ctx, ctx->tiles_rb, d);
     } else {
-      if (fwd) dilu_fwd<T, V>(ctx, ctx->tiles, d);
-      else dilu_bwd<T, V>(ctx, ctx->tiles, d);
+      dilu_fwd<T, V>(ctx, ctx->tiles, d);
+      dilu_bwd<T, V>(ctx, ctx->tiles, d);
     }
   }
   // out = M^-1 (in - alpha*in2) (in2 may be null)
@@ -465,17 +465,12 @@ struct Engine {
                                        a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN);
     d.in2 = in2;
     if (!ctx->timing) {
-      sweep(true, d);
-      sweep(false, d);
+      apply_pc(d);
       return;
     }
     const int e0 = trec();
-    sweep(true, d);
-    const int e1 = trec();
-    sweep(false, d);
-    const int e2 = trec();
-    tpairs.push_back({MT_KT_DILU_FWD, e0, e1});
-    tpairs.push_back({MT_KT_DILU_BWD, e1, e2});
+    apply_pc(d);
+    tpairs.push_back({MT_KT_PRECOND, e0, trec()});
   }
   template <int WHICH>
   void spmv(int nblk) {
@@ -510,7 +505,7 @@ struct Engine {
 
   // event pool for one chunk: 2 SpMV + 2 DILU applications per iteration
   int ensure_events(int chunk) {
-    const size_t need = size_t(chunk) * 10 + 4;
+    const size_t need = size_t(chunk) * 8 + 4;  // 4 pairs per iteration
     while (ctx->tev.size() < need) {
       hipEvent_t e;
       MT_HIP_CHECK(ctx, hipEventCreate(&e));
@@ -556,32 +551,38 @@ struct Engine {
   }
   // After a chunk and the host read of the scalars: add the event times of the
   // chunk and the algorithmic bytes of the iterations it ran.  Per iteration
-  // a system takes part in 2 SpMV launches and 2 DILU applications.
+  // a system takes part in 2 SpMV launches and 2 preconditioner applications.
   int account(const std::vector<SysScal>& h) {
     if (!ctx->timing) return MT_OK;
     long long units = 0;
     for (size_t q = 0; q < h.size() && q < last_iter.size(); ++q) {
-      units += 2LL * std::max(0, h[q].iter - last_iter[q]);
+      units += std::max(0, h[q].iter - last_iter[q]);
       last_iter[q] = h[q].iter;
     }
     const Grid& g = ctx->hop.g;
     const double sv = sizeof(V), sc2 = 2.0 * sizeof(T), n = double(g.nInt);
-    int cnt[3] = {0, 0, 0};
-    for (const TPair& tp : tpairs) {
+    const TilePlan& tp = a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles;
+    const int pc_launches = a.pc == PCK_DILU_RB ? 3 : 2 * tp.nlev;
+    int cnt[2] = {0, 0};
+    for (const TPair& tq : tpairs) {
       float ms = 0.f;
-      MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->tev[tp.e0], ctx->tev[tp.e1]));
-      mt_ctx::KTime& k = ctx->kt[tp.fam];
+      MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->tev[tq.e0], ctx->tev[tq.e1]));
+      mt_ctx::KTime& k = ctx->kt[tq.fam];
       k.ms += ms;
-      k.launches += tp.fam == MT_KT_SPMV ? 1 : (a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles).nlev;
-      ++cnt[tp.fam];
+      k.launches += tq.fam == MT_KT_SPMV ? 1 : pc_launches;
+      ++cnt[tq.fam];
     }
-    // shared per launch (SpMV: stored couplings + frequency-free diagonal) or
-    // per sweep (DILU: the three coupling slots), plus per system-unit
-    // vectors: SpMV x, y, and the dot partner; DILU input, output, pivots
-    ctx->kt[MT_KT_SPMV].bytes += cnt[MT_KT_SPMV] * (double(stencil_nnz(ctx)) * sv + n * sv) +
-                                 (cnt[MT_KT_SPMV] ? units * 3.0 * n * sc2 : 0.0);
-    for (int f = MT_KT_DILU_FWD; f <= MT_KT_DILU_BWD; ++f)
-      ctx->kt[f].bytes += cnt[f] * 3.0 * n * sv + (cnt[f] ? units * 3.0 * n * sc2 : 0.0);
+    // SpMV, per launch: stored couplings + frequency-free diagonal; per system:
+    // x read, y written, the dot partner read.  Preconditioner application,
+    // per application: the six lattice couplings; per system: the right-hand
+    // side(s) read, the result written, the pivots read.
+    if (cnt[MT_KT_SPMV])
+      ctx->kt[MT_KT_SPMV].bytes += cnt[MT_KT_SPMV] * (double(stencil_nnz(ctx)) * sv + n * sv) +
+                                   2.0 * units * 3.0 * n * sc2;
+    // (the second application of an iteration reads r and v when fused)
+    if (cnt[MT_KT_PRECOND])
+      ctx->kt[MT_KT_PRECOND].bytes += cnt[MT_KT_PRECOND] * 6.0 * n * sv +
+                                      units * (3.0 + (a.fused ? 4.0 : 3.0)) * n * sc2;
     return MT_OK;
   }
   ~Engine() {
@@ -957,6 +958,11 @@ int mt_ctx_create(int device, mt_ctx** out) {
     return MT_E_DEVICE;
   }
   for (auto& e : ctx->ev) hipEventCreate(&e);
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      ncu > 0)
+    ctx->num_cu = ncu;
+  (void)hipGetLastError();
   *out = ctx;
   return MT_OK;
 }
@@ -1245,8 +1251,7 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
   } else if (precond == MT_PC_DILU_RB) {
     const DiluArgs<double, double> d = dilu_args<double, double>(
         ctx->tiles_rb, ctx->dop.coef64, g.nE, g.nE, d_act, 1, d_sc, dx, dy, dinv, ST_RUN);
-    dilu_rb_fwd<double, double>(ctx, ctx->tiles_rb, d);
-    dilu_rb_bwd<double, double>(ctx, ctx->tiles_rb, d);
+    dilu_rb_apply<double, double>(ctx, ctx->tiles_rb, d);
   } else if (precond == MT_PC_JACOBI) {
     // y = dinv .* x via the p-update kernel path is overkill; do it on the host view
     std::vector<Z> hd(g.nE), hx(g.nE), hy(g.nE);
@@ -1561,7 +1566,7 @@ int mt_timing_enable(mt_ctx* ctx, int on) {
 
 int mt_kernel_timing(const mt_ctx* ctx, int family, long long* launches, double* ms,
                      double* bytes) {
-  if (!ctx || family < 0 || family > 2) return MT_E_ARG;
+  if (!ctx || family < 0 || family > 1) return MT_E_ARG;
   const mt_ctx::KTime& k = ctx->kt[family];
   if (launches) *launches = k.launches;
   if (ms) *ms = k.ms;

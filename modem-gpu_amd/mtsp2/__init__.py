@@ -23,7 +23,7 @@ MT_OK, MT_E_ARG, MT_E_DEVICE, MT_E_OOM, MT_E_NAN_RHS = 0, 1, 2, 3, 4
 MT_E_BREAKDOWN, MT_E_MAXIT, MT_E_STATE, MT_E_STENCIL = 5, 6, 7, 8
 PC_DILU, PC_JACOBI, PC_NONE, PC_DILU_RB = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
-KT_SPMV, KT_DILU_FWD, KT_DILU_BWD = 0, 1, 2
+KT_SPMV, KT_PRECOND = 0, 1
 
 
 class MTError(RuntimeError):
@@ -359,8 +359,8 @@ class Context:
         self._chk(lib().mt_timing_enable(self._h, int(bool(on))))
 
     def kernel_timing(self, family):
-        """(launches, ms, algorithmic bytes) of KT_SPMV / KT_DILU_FWD / KT_DILU_BWD
-        since timing_enable()."""
+        """(launches, ms, algorithmic bytes) of KT_SPMV / KT_PRECOND since
+        timing_enable()."""
         n, ms, by = C.c_longlong(), _D(), _D()
         self._chk(lib().mt_kernel_timing(self._h, int(family), C.byref(n), C.byref(ms),
                                          C.byref(by)))

@@ -165,12 +165,11 @@ def test_kernel_timing_graph_and_eager(gpu_ctx):
         try:
             X, st = gpu_ctx.solve_batch(om, B, X0, tol=1e-8, precision=mtsp2.PREC_MIXED,
                                         use_graph=graph, chunk=4)
-            res[graph] = [gpu_ctx.kernel_timing(f) for f in
-                          (mtsp2.KT_SPMV, mtsp2.KT_DILU_FWD, mtsp2.KT_DILU_BWD)]
+            res[graph] = [gpu_ctx.kernel_timing(f) for f in (mtsp2.KT_SPMV, mtsp2.KT_PRECOND)]
         finally:
             gpu_ctx.timing_enable(False)
         assert all(s["status"] == 0 for s in st)
-    for f in range(3):
+    for f in range(2):
         (n1, ms1, b1), (n0, ms0, b0) = res[1][f], res[0][f]
         assert n1 == n0 > 0 and b1 == b0 > 0
         assert ms1 > 0 and ms0 > 0
