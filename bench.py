@@ -77,12 +77,19 @@ def main():
     ap.add_argument("--config", default="cfg1")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-7)
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="least residual reduction asked of each FP32 inner solve")
+    ap.add_argument("--restart", type=int, default=120,
+                    help="BiCGStab restart interval of the inner solves (the reference's 120)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--precond", type=int, default=3,
                     help="preconditioner of the mixed solver's FP32 inner solves "
                          "(0 block-DILU in the reference ordering, 3 red-black tile block-DILU); "
                          "the plain FP64 baseline always uses the reference ordering")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the isolated SpMV launches (FP32 x 16 systems, FP64 x 1) and a "
+                         "1 GiB device copy, for hardware-counter runs")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
     args = ap.parse_args()
@@ -107,13 +114,29 @@ def main():
     om, B, X0 = P.forward_systems(ctx, grid, fhz)
     nsys, n = B.shape
     zb = nsys * n * 16
+    if args.kernels_only:
+        for prec, ns in ((mtsp2.PREC_MIXED, nsys), (mtsp2.PREC_FP64, 1)):
+            kms, kby = ctx.bench_spmv(ns, prec, reps=5)
+            log(f"SpMV {'FP32' if prec == mtsp2.PREC_MIXED else 'FP64'} x {ns}: {kms * 1e3:.1f} us, "
+                f"{kby / 1e6:.1f} MB algorithmic, {kby / (kms * 1e-3) / 1e9:.0f} GB/s")
+        # calibration kernel for the counters: 1 GiB read + 1 GiB written
+        gib = 1 << 30
+        src, dst = ctx.dev_alloc(gib), ctx.dev_alloc(gib)
+        for _ in range(3):
+            ctx.dev_copy(dst, src, gib)
+        log("device copy of 1 GiB x 3 done")
+        ctx.dev_free(src)
+        ctx.dev_free(dst)
+        ctx.close()
+        return
     d_b, d_x0, d_x = ctx.dev_alloc(zb), ctx.dev_alloc(zb), ctx.dev_alloc(zb)
     ctx.upload(d_b, B)
     ctx.upload(d_x0, X0)
     tuned = ctx.autotune(nsys)
     log(f"autotune: {tuned} row-tile groups per system")
     opts = dict(tol=args.tol, maxit=4000, precision=mtsp2.PREC_MIXED, precond=args.precond,
-                inner_tol=1e-4, chunk=10, use_graph=0 if args.no_graph else 1)
+                inner_tol=args.inner_tol, restart=args.restart, chunk=10,
+                use_graph=0 if args.no_graph else 1)
 
     def step():
         ctx.dev_copy(d_x, d_x0, zb)
@@ -208,7 +231,8 @@ def main():
         "data": "synthetic (seeded blocky model, plane-wave boundary data)",
         "config": {"workload": f"{args.config}: 100x100x60+10 air, 2052270 interior edges, "
                                "8 f x 2 pol forward systems, tol 1e-7 FP64 true residual",
-                   "nsys": nsys, "seed": args.seed,
+                   "nsys": nsys, "seed": args.seed, "inner_tol": args.inner_tol,
+                   "restart": args.restart,
                    "precond": {0: "block-DILU (reference ordering)", 1: "Jacobi", 2: "none",
                                3: "block-DILU, red-black 8x8x8 tile ordering"}[args.precond]},
         "converged": ok, "max_relres": relres,

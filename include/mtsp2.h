@@ -218,6 +218,14 @@ int mt_host_gradient(int nx, int ny, int nzAir, int nzEarth, const double *dx, c
  * several row-tile group counts per system and keeps the fastest; *variant
  * receives the chosen group count. */
 int mt_autotune(mt_ctx *ctx, int nsys, int *variant);
+/* Chooses the SpMV of the solvers: 0 row-parallel (every row gathers its 14
+ * neighbours from memory); 2 k-marching (a block keeps three k layers of x of
+ * an (i, j) tile in LDS); 1 (default) k-marching in FP32 solves and
+ * row-parallel in FP64 ones, the faster of the two for each.
+ * layers_per_segment (> 0) sets the k extent of one k-marching block, 0
+ * keeps it.  Has no counterpart in the reference (cuSPARSE's SpMV is fixed
+ * there). */
+int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
 /* Fixes the row-tile group count per system (0: automatic). */
 int mt_set_tuning(mt_ctx *ctx, int groups_per_system);
 /* Times the solver's fused SpMV+dot kernel (v = A x, <rt,v>) for nsys systems

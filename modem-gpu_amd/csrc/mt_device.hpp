@@ -61,6 +61,7 @@ template <class V>
 struct StencilView {
   const V* coef;
   const V* vms1;
+  const V* rowc;   // [nE][16]: a row's 15 couplings and vms1, contiguous (may be null)
   int64_t nE;
   int64_t nXi, nYi, nInt;
   int nx, ny, nz;
@@ -252,10 +253,12 @@ struct SysScal {
 
 // Block -> (system, row chunk) mapping, XCD-aware.  nblk = nact * G with G a
 // multiple of 8.  Block b is dispatched to XCD b % 8.  Each XCD owns a
-// contiguous eighth of the rows (chunks xcd*G/8 .. (xcd+1)*G/8-1), so the
-// neighbouring lattice planes a stencil row reads sit in the same XCD's L2,
-// and consecutive blocks on one XCD cover all systems of one chunk, so the
-// shared stencil coefficients are fetched once per XCD.
+// contiguous eighth of the rows (chunks xcd*G/8 .. (xcd+1)*G/8-1) and
+// consecutive blocks on one XCD cover all systems of one chunk, so the
+// stencil couplings, shared by the systems, are fetched once per XCD.  (The
+// alternative, an XCD sweeping whole systems, keeps a system's k-1/k+1 planes
+// in its L2 but re-reads the couplings per system: measured slower, with
+// twice the fetched bytes.)
 __device__ __forceinline__ void map_sys_block(int nact, int G, int& si, int& g) {
   const int b = blockIdx.x;
   const int xcd = b & 7, w = b >> 3;

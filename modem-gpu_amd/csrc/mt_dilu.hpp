@@ -25,6 +25,17 @@
 
 namespace mt {
 
+// The tile kernels run one wave64 per block.  A wave's LDS instructions are
+// executed in order, so a read issued after a write sees it whichever lane
+// wrote it: between the hyperplanes of a sweep only the compiler must be kept
+// from reordering, no s_waitcnt for the writes is needed (a block barrier
+// would wait for them to complete on every hyperplane).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 constexpr int kMaxTilePts = 1024;
 struct TileShape {
   int tx, ty, tz;
@@ -166,7 +177,7 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
     if (p < NP) spts[p] = pk[u];
   }
   if (lane <= NLEV) slvl[lane] = lv;
-  __syncthreads();
+  wave_sync();
   // ---- hyperplane sweep
   for (int hh = 0; hh < NLEV; ++hh) {
     const int h = FWD ? hh : NLEV - 1 - hh;
@@ -202,7 +213,7 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
         yb[q] = cmul(dv[p], y);
       }
     }
-    __syncthreads();
+    wave_sync();
   }
   // ---- write the tile back
   for (int p = lane; p < NP; p += 64) {
@@ -219,149 +230,18 @@ __global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileSh
 // admissible): the tiles of each lattice are coloured red/black by the parity
 // of ti+tj+tk, all red tiles come first, then all black tiles, and inside a
 // tile the points keep the lexicographic order.  Red tiles only couple to
-// black tiles, so each colour is one launch: a sweep takes 2 launches instead
-// of one per tile level.  With this ordering
+// black tiles, so each colour is one launch per sweep instead of one per
+// tile level.  With this ordering
 //   lower(e) = in-tile lower neighbours, plus (black e) its face neighbours in
 //              the adjacent red tiles;
 //   upper(e) = in-tile upper neighbours, plus (red e) its face neighbours in
 //              the adjacent black tiles,
 // and the pivots, the forward and the backward sweep are those of DILU with
-// these sets.  HALO=true is the colour whose cross-face neighbours are final
-// (black in the forward sweep and in the pivots, red in the backward sweep):
-// their contributions are folded into the tile's right-hand side (or
-// diagonal) when it is loaded, and the tile is then swept like an isolated one.
-template <class T, class V, int TX, int TY, int TZ, bool FWD, bool HALO>
-__global__ __launch_bounds__(64) void k_dilu_rb(DiluArgs<T, V> a, const TileShape* shp) {
-  constexpr int BX = TX + 1, BY = TY + 1, BZ = TZ + 1, NB = BX * BY * BZ, NP = TX * TY * TZ;
-  constexpr int NU = (NP + 63) / 64;
-  constexpr int NLEV = TX + TY + TZ - 2;
-  __shared__ C2<T> yb[NB];   // FWD: w = dinv*y; BWD: y -> z; the pad layer stays zero
-  __shared__ C2<T> rb[FWD ? NP : 1];
-  __shared__ C2<T> dv[NP];
-  __shared__ V cf[3][NP];
-  __shared__ unsigned int spts[NP];
-  __shared__ int slvl[NLEV + 1];
-  const int lane = threadIdx.x;
-  const int tix = blockIdx.x / a.nact, si = blockIdx.x - tix * a.nact;
-  const int sys = a.act[si];
-  if (a.sc[sys].status != a.status_req) return;
-  const int4 tl = a.tiles[tix];
-  const LatticeInfo L = a.lat[tl.x];
-  const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
-  const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
-  const int64_t o = (int64_t)sys * a.vstride;
-  const C2<T>* __restrict__ pin = a.in + o;
-  const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
-  const C2<T> al = ccast<T>(a.sc[sys].alpha);
-  const C2<T>* __restrict__ pout = a.out + o;
-  const C2<T>* __restrict__ dinv = a.dinv + (int64_t)a.sc[sys].fidx * a.cstride;
-  const V* __restrict__ cfs = a.coef;
-  const int64_t cst = a.cstride;
-  const int64_t sj = L.a, sk = int64_t(L.a) * L.b;  // lattice strides
-  // ---- loads: tile values, pivots, in-tile couplings, cross-face terms
-  C2<T> rv[NU], dd[NU];
-  V c3[NU][3];
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const int p = lane + 64 * u;
-    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
-    const bool inTile = p < NP && li < ex && lj < ey && lk < ez;
-    rv[u] = {T(0), T(0)};
-    dd[u] = {T(0), T(0)};
-    c3[u][0] = c3[u][1] = c3[u][2] = V(0);
-    if (inTile) {
-      const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
-      const int64_t e = lat_e(L, gi, gj, gk);
-      rv[u] = (FWD ? pin : pout)[e];
-      if (FWD && pin2) rv[u] = csub(rv[u], cmul(al, pin2[e]));
-      dd[u] = dinv[e];
-      // in-tile couplings: lower (slots 1,3,5) forward, upper (2,4,6) backward
-      const bool has0 = FWD ? li > 0 : li + 1 < ex;
-      const bool has1 = FWD ? lj > 0 : lj + 1 < ey;
-      const bool has2 = FWD ? lk > 0 : lk + 1 < ez;
-      const int s0 = FWD ? 1 : 2;
-      if (has0) c3[u][0] = cfs[(int64_t)s0 * cst + e];
-      if (has1) c3[u][1] = cfs[(int64_t)(s0 + 2) * cst + e];
-      if (has2) c3[u][2] = cfs[(int64_t)(s0 + 4) * cst + e];
-      if (HALO) {
-        // neighbours across the tile faces (the other colour, final)
-        const bool lo0 = li == 0 && gi > 0, hi0 = li + 1 == ex && gi + 1 < L.A;
-        const bool lo1 = lj == 0 && gj > 0, hi1 = lj + 1 == ey && gj + 1 < L.B;
-        const bool lo2 = lk == 0 && gk > 0, hi2 = lk + 1 == ez && gk + 1 < L.C;
-        const bool fl[6] = {lo0, hi0, lo1, hi1, lo2, hi2};
-        const int64_t st[6] = {-1, 1, -sj, sj, -sk, sk};
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-          if (!fl[f]) continue;
-          const T c = T(cfs[(int64_t)(1 + f) * cst + e]);
-          const C2<T> v = FWD ? cmul(dinv[e + st[f]], pout[e + st[f]]) : pout[e + st[f]];
-          rv[u].x -= c * v.x;
-          rv[u].y -= c * v.y;
-        }
-      }
-    }
-  }
-  for (int q = lane; q < NB; q += 64) yb[q] = {T(0), T(0)};
-  for (int p = lane; p < NP; p += 64) spts[p] = shp->pts[p];
-  if (lane <= NLEV) slvl[lane] = shp->lvl[lane];
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const int p = lane + 64 * u;
-    if (p >= NP) break;
-    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
-#pragma unroll
-    for (int s = 0; s < 3; ++s) cf[s][p] = c3[u][s];
-    dv[p] = dd[u];
-    if (FWD) rb[p] = rv[u];
-    else yb[li + BX * (lj + BY * lk)] = rv[u];
-  }
-  __syncthreads();
-  // ---- in-tile sweep
-  for (int hh = 0; hh < NLEV; ++hh) {
-    const int h = FWD ? hh : NLEV - 1 - hh;
-    for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
-      const unsigned int pt = spts[pp];
-      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = pt >> 16;
-      if (li >= ex || lj >= ey || lk >= ez) continue;
-      const int p = li + TX * (lj + TY * lk);
-      if (FWD) {
-        const int q = (li + 1) + BX * ((lj + 1) + BY * (lk + 1));
-        C2<T> y = rb[p];
-        const int qn[3] = {q - 1, q - BX, q - BX * BY};
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const T c = T(cf[s][p]);
-          const C2<T> w = yb[qn[s]];
-          y.x -= c * w.x;
-          y.y -= c * w.y;
-        }
-        rb[p] = y;
-        yb[q] = cmul(dv[p], y);
-      } else {
-        const int q = li + BX * (lj + BY * lk);
-        C2<T> y = yb[q];
-        const int qn[3] = {q + 1, q + BX, q + BX * BY};
-#pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const T c = T(cf[s][p]);
-          const C2<T> z = yb[qn[s]];
-          y.x -= c * z.x;
-          y.y -= c * z.y;
-        }
-        yb[q] = cmul(dv[p], y);
-      }
-    }
-    __syncthreads();
-  }
-  C2<T>* __restrict__ dst = a.out + o;
-  for (int p = lane; p < NP; p += 64) {
-    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
-    if (li >= ex || lj >= ey || lk >= ez) continue;
-    dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = FWD ? rb[p] : yb[li + BX * (lj + BY * lk)];
-  }
-}
-
+// these sets.  The colour whose cross-face neighbours are already final
+// (black in the forward sweep and in the pivots, red in the backward sweep)
+// takes their contributions into the tile's right-hand side (or diagonal)
+// before it sweeps the tile like an isolated one.
+//
 // ------------------------------------------- red-black sweeps, persistent
 // The red-black application as three launches over 8x8x8 tiles:
 //   RB_RED_FWD   red tiles, forward:  y = L^-1 r on each tile alone;
@@ -415,7 +295,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
     pk[h] = (b + lane < e) ? shp->pts[b + lane] : 0xffffffffu;
   }
   for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
-  __syncthreads();
+  wave_sync();
   const V* __restrict__ cfs = a.coef;
   const int64_t cst = a.cstride;
   // ---- prefetch registers (one item)
@@ -532,7 +412,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         CH[2][p] = (MODE == RB_RED_BWD || up2) ? ch[u][2] : V(0);
       }
     }
-    __syncthreads();
+    wave_sync();
     // fold the face terms the sweeps do not reach: black forward, the high
     // faces (red neighbours above); red backward, the low faces (black below)
     if (HALO) {
@@ -571,7 +451,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
           W[q] = y;
         }
       }
-      __syncthreads();
+      wave_sync();
     }
     // ---- the next item's loads go out now and land during the sweeps
     int wn = w + gx;
@@ -598,7 +478,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
           RB[p] = y;
           W[q] = cmul(DV[p], y);
         }
-        __syncthreads();
+        wave_sync();
       }
     }
     if (BWDP) {
@@ -620,7 +500,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
           }
           W[q] = cmul(DV[p], y);
         }
-        __syncthreads();
+        wave_sync();
       }
     }
     // ---- write the tile back (red forward: y; otherwise z)
@@ -631,7 +511,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
       const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
       dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = MODE == RB_RED_FWD ? RB[p] : W[q];
     }
-    __syncthreads();
+    wave_sync();
     w = wn;
   }
 }
@@ -712,7 +592,7 @@ __global__ __launch_bounds__(64) void k_dilu_pivots(PivotArgs a, const TileShape
   }
   for (int p = lane; p < NP; p += 64) spts[p] = shp->pts[p];
   if (lane <= NLEV) slvl[lane] = shp->lvl[lane];
-  __syncthreads();
+  wave_sync();
   for (int h = 0; h < NLEV; ++h) {
     for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
       const unsigned int pt = spts[pp];
@@ -733,7 +613,7 @@ __global__ __launch_bounds__(64) void k_dilu_pivots(PivotArgs a, const TileShape
       }
       pv[q] = zdiv({1.0, 0.0}, piv);
     }
-    __syncthreads();
+    wave_sync();
   }
   for (int p = lane; p < NP; p += 64) {
     const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
@@ -812,7 +692,7 @@ __global__ __launch_bounds__(64) void k_dilu_pivots_rb(PivotArgs a, const TileSh
   }
   for (int p = lane; p < NP; p += 64) spts[p] = shp->pts[p];
   if (lane <= NLEV) slvl[lane] = shp->lvl[lane];
-  __syncthreads();
+  wave_sync();
   const int stp[3] = {1, TX, TX * TY};
   for (int h = 0; h < NLEV; ++h) {
     for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
@@ -832,7 +712,7 @@ __global__ __launch_bounds__(64) void k_dilu_pivots_rb(PivotArgs a, const TileSh
       }
       pv[p] = zdiv({1.0, 0.0}, piv);
     }
-    __syncthreads();
+    wave_sync();
   }
   for (int p = lane; p < NP; p += 64) {
     const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);

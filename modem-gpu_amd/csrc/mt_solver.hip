@@ -1,5 +1,6 @@
 // libmtsp2: context, operator upload, BiCGStab drivers and the C ABI
-// (include/mtsp2.h).  Kernels live in mt_kernels.hpp / mt_dilu.hpp.
+// (include/mtsp2.h).  Kernels live in mt_kernels.hpp, mt_spmv.hpp,
+// mt_dilu.hpp and mt_divcor.hpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +18,7 @@
 #include "mt_host_ops.hpp"
 #include "mt_internal.h"
 #include "mt_kernels.hpp"
+#include "mt_spmv.hpp"
 
 using namespace mt;
 
@@ -29,6 +31,11 @@ struct DeviceOperator {
   float* vms32 = nullptr;
   double* coefT64 = nullptr;  // transposed stencil (lazy)
   float* coefT32 = nullptr;
+  // row-major copies [nE][16] (15 couplings + vms1) for the k-marching SpMV
+  double* row64 = nullptr;
+  float* row32 = nullptr;
+  double* rowT64 = nullptr;
+  float* rowT32 = nullptr;
   int* imap = nullptr;       // interior -> all-edge
   int off[3][15];
 };
@@ -67,6 +74,9 @@ struct mt_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int G_override = 0;
   int num_cu = 256;             // compute units of the device
+  int spmv_variant = 1;         // 0 row-parallel SpMV; 1 k-marching for FP32
+                                // solves, row-parallel for FP64; 2 k-marching
+  int km_ks = 16;               // k-marching SpMV: layers per k segment
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
   // SpMV launch and every DILU sweep of the BiCGStab engine
   bool timing = false;
@@ -113,6 +123,7 @@ static int ws(mt_ctx* ctx, const std::string& name, size_t count, P** out) {
 static void free_dop(DeviceOperator& d) {
   hipFree(d.coef64); hipFree(d.coef32); hipFree(d.vms64); hipFree(d.vms32);
   hipFree(d.coefT64); hipFree(d.coefT32); hipFree(d.imap);
+  hipFree(d.row64); hipFree(d.row32); hipFree(d.rowT64); hipFree(d.rowT32);
   d = DeviceOperator();
 }
 
@@ -139,9 +150,11 @@ static StencilView<V> view(const mt_ctx* ctx, bool transpose) {
   const Grid& g = ctx->hop.g;
   if constexpr (sizeof(V) == 8) {
     A.coef = reinterpret_cast<const V*>(transpose ? ctx->dop.coefT64 : ctx->dop.coef64);
+    A.rowc = reinterpret_cast<const V*>(transpose ? ctx->dop.rowT64 : ctx->dop.row64);
     A.vms1 = reinterpret_cast<const V*>(ctx->dop.vms64);
   } else {
     A.coef = reinterpret_cast<const V*>(transpose ? ctx->dop.coefT32 : ctx->dop.coef32);
+    A.rowc = reinterpret_cast<const V*>(transpose ? ctx->dop.rowT32 : ctx->dop.row32);
     A.vms1 = reinterpret_cast<const V*>(ctx->dop.vms32);
   }
   A.nE = g.nE;
@@ -240,7 +253,22 @@ static int upload_operator(mt_ctx* ctx) {
   for (int64_t q = 0; q < g.nE; ++q) tmp[q] = float(ctx->hop.vms1[q]);
   MT_HIP_CHECK(ctx, hipMemcpy(d.vms32, tmp.data(), g.nE * sizeof(float), hipMemcpyHostToDevice));
   MT_HIP_CHECK(ctx, hipMemcpy(d.imap, ctx->hop.imap.data(), g.nInt * sizeof(int), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMalloc(&d.row64, size_t(16) * g.nE * sizeof(double)));
+  MT_HIP_CHECK(ctx, hipMalloc(&d.row32, size_t(16) * g.nE * sizeof(float)));
+  k_rows_from_slots<double><<<2048, 256, 0, ctx->stream>>>(d.coef64, d.vms64, g.nE, d.row64);
+  k_rows_from_slots<float><<<2048, 256, 0, ctx->stream>>>(d.coef32, d.vms32, g.nE, d.row32);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   build_offsets(g, d.off);
+  // the k-marching SpMV carries its own copy of the neighbour table
+  for (int c = 0; c < 3; ++c) {
+    Nb nb[kSlots];
+    stencil_neighbours(c, nb);
+    for (int sl = 0; sl < kSlots; ++sl)
+      if (km_nb(c, sl, 0) != nb[sl].comp || km_nb(c, sl, 1) != nb[sl].di ||
+          km_nb(c, sl, 2) != nb[sl].dj || km_nb(c, sl, 3) != nb[sl].dk)
+        return mt_fail(ctx, MT_E_STATE, "k-marching SpMV neighbour table mismatch");
+  }
   d.ready = true;
   LatticeInfo lat[3];
   edge_lattices(g, lat);
@@ -261,6 +289,14 @@ static int ensure_transpose(mt_ctx* ctx) {
   MT_HIP_CHECK(ctx, hipMalloc(&ctx->dop.coefT32, nc * sizeof(float)));
   MT_HIP_CHECK(ctx, hipMemcpy(ctx->dop.coefT64, cT.data(), nc * sizeof(double), hipMemcpyHostToDevice));
   MT_HIP_CHECK(ctx, hipMemcpy(ctx->dop.coefT32, t32.data(), nc * sizeof(float), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMalloc(&ctx->dop.rowT64, size_t(16) * g.nE * sizeof(double)));
+  MT_HIP_CHECK(ctx, hipMalloc(&ctx->dop.rowT32, size_t(16) * g.nE * sizeof(float)));
+  k_rows_from_slots<double><<<2048, 256, 0, ctx->stream>>>(ctx->dop.coefT64, ctx->dop.vms64, g.nE,
+                                                           ctx->dop.rowT64);
+  k_rows_from_slots<float><<<2048, 256, 0, ctx->stream>>>(ctx->dop.coefT32, ctx->dop.vms32, g.nE,
+                                                          ctx->dop.rowT32);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MT_OK;
 }
 
@@ -427,6 +463,19 @@ static int pick_G(const mt_ctx* ctx, int nact) {
   return G;
 }
 
+// Work items of the k-marching SpMV for the grid of ctx.
+static KmGeom km_geom(const mt_ctx* ctx) {
+  const Grid& g = ctx->hop.g;
+  KmGeom k;
+  k.nti = (g.nx + 1 + KM_TI - 1) / KM_TI;
+  k.ntj = (g.ny + 1 + KM_TJ - 1) / KM_TJ;
+  k.ks = std::max(1, ctx->km_ks);
+  k.nseg = (g.nz + 1 + k.ks - 1) / k.ks;
+  k.Greal = k.nti * k.ntj * k.nseg;
+  k.Gpad = (k.Greal + 7) / 8 * 8;
+  return k;
+}
+
 template <class T, class V>
 struct Engine {
   mt_ctx* ctx;
@@ -437,6 +486,8 @@ struct Engine {
   bool recheck = true;  // FP64 solves re-check the true residual; inner solves do not
   hipGraphExec_t gexec = nullptr;
   int g_nact = -1, g_chunk = -1;
+  bool use_km = true;  // k-marching SpMV (set from the context at solve start)
+  KmGeom km{};
   // timing: event pairs (family, first, second) recorded by one chunk; with a
   // graph they are recorded by the graph's event nodes on every replay
   struct TPair { int fam, e0, e1; };
@@ -472,11 +523,20 @@ struct Engine {
     apply_pc(d);
     tpairs.push_back({MT_KT_PRECOND, e0, trec()});
   }
+  // the SpMV and its dot partials; returns the partial count per system
   template <int WHICH>
-  void spmv(int nblk) {
+  int spmv(int nblk) {
     const int e0 = ctx->timing ? trec() : -1;
-    k_spmv_dot<T, V, WHICH><<<nblk, BS, 0, ctx->stream>>>(a);
+    int G;
+    if (use_km) {
+      k_spmv_km<T, V, WHICH><<<a.nact * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
+      G = km.Gpad;
+    } else {
+      k_spmv_dot<T, V, WHICH><<<nblk, BS, 0, ctx->stream>>>(a);
+      G = a.G;
+    }
     if (ctx->timing) tpairs.push_back({MT_KT_SPMV, e0, trec()});
+    return G;
   }
 
   void launch_iteration() {
@@ -484,16 +544,16 @@ struct Engine {
     hipStream_t s = ctx->stream;
     k_pupdate<T, V><<<nblk, BS, 0, s>>>(a);
     precond(a.p, a.ph);
-    spmv<1>(nblk);
-    k_finalize<FIN_ALPHA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    const int g1 = spmv<1>(nblk);
+    k_finalize<FIN_ALPHA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, g1, a.partials, a.rerr, a.rerr_len, a.tol);
     if (a.fused) {
       precond(a.r, a.sh, a.v);  // sh = M^-1 (r - alpha v)
     } else {
       k_supdate<T, V><<<nblk, BS, 0, s>>>(a);
       precond(a.s, a.sh);
     }
-    spmv<2>(nblk);
-    k_finalize<FIN_OMEGA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
+    const int g2 = spmv<2>(nblk);
+    k_finalize<FIN_OMEGA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, g2, a.partials, a.rerr, a.rerr_len, a.tol);
     if (a.fused) k_rupdate_fused<T, V><<<nblk, BS, 0, s>>>(a);
     else k_rupdate<T, V><<<nblk, BS, 0, s>>>(a);
     k_finalize<FIN_RUPDATE><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, a.G, a.partials, a.rerr, a.rerr_len, a.tol);
@@ -669,7 +729,9 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   C2<double>* dinv = nullptr;
   if ((rc = build_dinv<double>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   a.dinv = dinv;
-  const int Gmax = pick_G(ctx, 1);
+  E.use_km = ctx->spmv_variant == 2;
+  E.km = km_geom(ctx);
+  const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
   SysScal* d_sc;
   int* d_act;
   if ((rc = ws(ctx, "f64_sc", nsys, &d_sc)) || (rc = ws(ctx, "act", nsys, &d_act)) ||
@@ -772,7 +834,9 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   C2<float>* dinv = nullptr;
   if ((rc = build_dinv<float>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   a.dinv = dinv;
-  const int Gmax = pick_G(ctx, 1);
+  E.use_km = ctx->spmv_variant >= 1;
+  E.km = km_geom(ctx);
+  const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
   SysScal *d_so, *d_si;
   int* d_act;
   if ((rc = ws(ctx, "mx_so", nsys, &d_so)) || (rc = ws(ctx, "mx_si", nsys, &d_si)) ||
@@ -1490,8 +1554,11 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   SysScal* d_sc;
   int* d_act;
   Z* partials;
+  const KmGeom km = km_geom(ctx);
+  const bool use_km = ctx->spmv_variant == 2 || (ctx->spmv_variant == 1 && precision == MT_PREC_MIXED);
   if ((rc = ws(ctx, "bn_sc", nsys, &d_sc)) || (rc = ws(ctx, "bn_act", nsys, &d_act)) ||
-      (rc = ws(ctx, "partials", size_t(nsys) * pick_G(ctx, 1) * 2 + 64, &partials)))
+      (rc = ws(ctx, "partials", size_t(nsys) * std::max(pick_G(ctx, 1), km.Gpad) * 2 + 64,
+               &partials)))
     return rc;
   MT_HIP_CHECK(ctx, hipMemcpyAsync(d_sc, h.data(), nsys * sizeof(SysScal), hipMemcpyHostToDevice, ctx->stream));
   MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, act.data(), nsys * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -1516,9 +1583,13 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
         (r2 = ws(ctx, sizeof(T) == 8 ? "bn_rt64" : "bn_rt32", nv, &vrt)))
       return r2;
     a.ph = vin; a.v = vout; a.rt = vrt;
-    k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);  // warm-up
+    auto launch = [&]() {
+      if (use_km) k_spmv_km<T, T, 1><<<nsys * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
+      else k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);
+    };
+    launch();  // warm-up
     MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    for (int q = 0; q < reps; ++q) k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);
+    for (int q = 0; q < reps; ++q) launch();
     MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     MT_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[1]));
     MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
@@ -1530,6 +1601,14 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   if (rc) return rc;
   if (ms_per_launch) *ms_per_launch = double(ms) / reps;
   if (bytes_per_launch) *bytes_per_launch = bytes;
+  return MT_OK;
+}
+
+int mt_set_spmv_variant(mt_ctx* ctx, int variant, int layers_per_segment) {
+  if (!ctx || variant < 0 || variant > 2 || layers_per_segment < 0)
+    return mt_fail(ctx, MT_E_ARG, "mt_set_spmv_variant: variant 0, 1 or 2, layers >= 0");
+  ctx->spmv_variant = variant;
+  if (layers_per_segment > 0) ctx->km_ks = layers_per_segment;
   return MT_OK;
 }
 
@@ -1545,13 +1624,21 @@ int mt_autotune(mt_ctx* ctx, int nsys, int* variant) {
   const int cands[] = {64, 128, 256, 512, 1024, 2048};
   double best = 1e30;
   int bestG = 0;
+  // G is the row-chunk count of the row-parallel kernels (the row SpMV and
+  // the vector updates): time it on the row SpMV whatever variant is chosen
+  const int keep = ctx->spmv_variant;
+  ctx->spmv_variant = 0;
   for (int G : cands) {
     if (G * nsys > 65536) continue;
     ctx->G_override = G;
     double ms = 0.0;
-    if ((rc = mt_bench_spmv(ctx, nsys, MT_PREC_MIXED, 5, &ms, nullptr))) return rc;
+    if ((rc = mt_bench_spmv(ctx, nsys, MT_PREC_MIXED, 5, &ms, nullptr))) {
+      ctx->spmv_variant = keep;
+      return rc;
+    }
     if (ms < best) { best = ms; bestG = G; }
   }
+  ctx->spmv_variant = keep;
   ctx->G_override = bestG;
   if (variant) *variant = bestG;
   return MT_OK;

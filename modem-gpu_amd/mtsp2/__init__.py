@@ -85,6 +85,7 @@ _SIGS = {
     "mt_set_tuning": (_I, [_P, _I]),
     "mt_bench_spmv": (_I, [_P, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_timing_enable": (_I, [_P, _I]),
+    "mt_set_spmv_variant": (_I, [_P, _I, _I]),
     "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
 }
 
@@ -351,6 +352,11 @@ class Context:
         v = _I()
         self._chk(lib().mt_autotune(self._h, int(nsys), C.byref(v)))
         return v.value
+
+    def set_spmv_variant(self, variant, layers_per_segment=0):
+        """0 row-parallel SpMV, 2 k-marching, 1 (default) k-marching in FP32
+        solves and row-parallel in FP64 ones."""
+        self._chk(lib().mt_set_spmv_variant(self._h, int(variant), int(layers_per_segment)))
 
     def set_tuning(self, groups):
         self._chk(lib().mt_set_tuning(self._h, int(groups)))
