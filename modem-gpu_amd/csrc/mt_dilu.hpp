@@ -267,10 +267,14 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
   constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
   constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
   __shared__ C2<T> W[NB];                     // box: w = dinv*y (forward), y -> z (backward)
-  __shared__ C2<T> RB[FWDP ? NP : 1];         // rhs -> y on the tile
-  __shared__ C2<T> DV[NP];
-  __shared__ V CL[FWDP ? 3 : 1][FWDP ? NP : 1];  // lower couplings used by the forward sweep
-  __shared__ V CH[BWDP ? 3 : 1][BWDP ? NP : 1];  // upper couplings used by the backward sweep
+  // per tile point, packed so that a sweep step reads each with one LDS
+  // instruction: rhs -> y with the pivot, and the couplings by four
+  struct alignas(16) PtA { C2<T> rb, dv; };
+  struct alignas(16) Cpl { V c[4]; };
+  __shared__ PtA PD[FWDP ? NP : 1];           // forward sweeps: rhs -> y, pivot
+  __shared__ C2<T> DV[FWDP ? 1 : NP];         // red backward: pivot
+  __shared__ Cpl CL[FWDP ? NP : 1];           // lower couplings used by the forward sweep
+  __shared__ Cpl CH[BWDP ? NP : 1];           // upper couplings used by the backward sweep
   __shared__ int s_sys[64], s_ok[64], s_fidx[64];
   __shared__ C2<T> s_al[64];
   const int lane = threadIdx.x;
@@ -394,22 +398,28 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
       const bool up0 = li + 1 < ex, up1 = lj + 1 < ey, up2 = lk + 1 < ez;      // upper in tile
       C2<T> r = rv[u];
       if (FWDP && a.in2) r = csub(r, cmul(al, v2[u]));
-      DV[p] = dd[u];
       if (FWDP) {
-        RB[p] = r;
+        PD[p] = PtA{r, dd[u]};
         // red: face neighbours are upper, so only in-tile lower couplings;
         // black: every lower-side coupling (in tile or across the face)
-        CL[0][p] = (MODE == RB_BLACK_FB || in0) ? cl[u][0] : V(0);
-        CL[1][p] = (MODE == RB_BLACK_FB || in1) ? cl[u][1] : V(0);
-        CL[2][p] = (MODE == RB_BLACK_FB || in2) ? cl[u][2] : V(0);
+        Cpl c;
+        c.c[0] = (MODE == RB_BLACK_FB || in0) ? cl[u][0] : V(0);
+        c.c[1] = (MODE == RB_BLACK_FB || in1) ? cl[u][1] : V(0);
+        c.c[2] = (MODE == RB_BLACK_FB || in2) ? cl[u][2] : V(0);
+        c.c[3] = V(0);
+        CL[p] = c;
       } else {
+        DV[p] = dd[u];
         W[q] = r;
       }
       if (BWDP) {
         // black: upper neighbours in the tile only; red: in tile or across
-        CH[0][p] = (MODE == RB_RED_BWD || up0) ? ch[u][0] : V(0);
-        CH[1][p] = (MODE == RB_RED_BWD || up1) ? ch[u][1] : V(0);
-        CH[2][p] = (MODE == RB_RED_BWD || up2) ? ch[u][2] : V(0);
+        Cpl c;
+        c.c[0] = (MODE == RB_RED_BWD || up0) ? ch[u][0] : V(0);
+        c.c[1] = (MODE == RB_RED_BWD || up1) ? ch[u][1] : V(0);
+        c.c[2] = (MODE == RB_RED_BWD || up2) ? ch[u][2] : V(0);
+        c.c[3] = V(0);
+        CH[p] = c;
       }
     }
     wave_sync();
@@ -423,7 +433,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         if (!(li < ex && lj < ey && lk < ez)) continue;
         const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
         if (MODE == RB_BLACK_FB) {
-          C2<T> r = RB[p];
+          C2<T> r = PD[p].rb;
           const bool f0 = li == TX - 1, f1 = lj == TX - 1, f2 = lk == TX - 1;
           const int st[3] = {1, BX, BX * BX};
           const bool fl[3] = {f0, f1, f2};
@@ -435,7 +445,7 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
             r.x -= c * v.x;
             r.y -= c * v.y;
           }
-          RB[p] = r;
+          PD[p].rb = r;
         } else {
           C2<T> y = W[q];
           const int st[3] = {1, BX, BX * BX};
@@ -466,17 +476,19 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
           const int p = li + TX * (lj + TX * lk);
           const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
-          C2<T> y = RB[p];
+          const PtA pt = PD[p];
+          const Cpl cl4 = CL[p];
+          C2<T> y = pt.rb;
           const int qn[3] = {q - 1, q - BX, q - BX * BX};
 #pragma unroll
           for (int s = 0; s < 3; ++s) {
-            const T c = T(CL[s][p]);
+            const T c = T(cl4.c[s]);
             const C2<T> v = W[qn[s]];
             y.x -= c * v.x;
             y.y -= c * v.y;
           }
-          RB[p] = y;
-          W[q] = cmul(DV[p], y);
+          PD[p].rb = y;
+          W[q] = cmul(pt.dv, y);
         }
         wave_sync();
       }
@@ -489,16 +501,25 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
           const int p = li + TX * (lj + TX * lk);
           const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
-          C2<T> y = MODE == RB_BLACK_FB ? RB[p] : W[q];
+          C2<T> y, dvp;
+          if constexpr (FWDP) {
+            const PtA pt = PD[p];
+            y = pt.rb;
+            dvp = pt.dv;
+          } else {
+            y = W[q];
+            dvp = DV[p];
+          }
+          const Cpl ch4 = CH[p];
           const int qn[3] = {q + 1, q + BX, q + BX * BX};
 #pragma unroll
           for (int s = 0; s < 3; ++s) {
-            const T c = T(CH[s][p]);
+            const T c = T(ch4.c[s]);
             const C2<T> z = W[qn[s]];
             y.x -= c * z.x;
             y.y -= c * z.y;
           }
-          W[q] = cmul(DV[p], y);
+          W[q] = cmul(dvp, y);
         }
         wave_sync();
       }
@@ -509,7 +530,10 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
       const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
       if (!(li < ex && lj < ey && lk < ez)) continue;
       const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
-      dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = MODE == RB_RED_FWD ? RB[p] : W[q];
+      C2<T> zv;
+      if constexpr (MODE == RB_RED_FWD) zv = PD[p].rb;
+      else zv = W[q];
+      dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = zv;
     }
     wave_sync();
     w = wn;
