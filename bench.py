@@ -119,6 +119,11 @@ def main():
             kms, kby = ctx.bench_spmv(ns, prec, reps=5)
             log(f"SpMV {'FP32' if prec == mtsp2.PREC_MIXED else 'FP64'} x {ns}: {kms * 1e3:.1f} us, "
                 f"{kby / 1e6:.1f} MB algorithmic, {kby / (kms * 1e-3) / 1e9:.0f} GB/s")
+        for pc in (mtsp2.PC_DILU_RB, mtsp2.PC_DILU):
+            kms, kby = ctx.bench_precond(nsys, mtsp2.PREC_MIXED, pc, reps=5)
+            log(f"DILU {'red-black' if pc == mtsp2.PC_DILU_RB else 'reference order'} FP32 x {nsys}: "
+                f"{kms * 1e3:.1f} us, {kby / 1e6:.1f} MB algorithmic, "
+                f"{kby / (kms * 1e-3) / 1e9:.0f} GB/s")
         # calibration kernel for the counters: 1 GiB read + 1 GiB written
         gib = 1 << 30
         src, dst = ctx.dev_alloc(gib), ctx.dev_alloc(gib)
@@ -214,6 +219,12 @@ def main():
     roofline["spmv_isolated_fp32_16sys"] = {"us_per_launch": spmv_ms * 1e3,
                                             "bytes_per_launch": spmv_bytes,
                                             "GBps": spmv_bytes / (spmv_ms * 1e-3) / 1e9}
+    pc_ms, pc_bytes = ctx.bench_precond(nsys, mtsp2.PREC_MIXED, mtsp2.PC_DILU_RB, reps=10)
+    roofline["precond_isolated_fp32_16sys"] = {"us_per_apply": pc_ms * 1e3,
+                                               "bytes_per_apply": pc_bytes,
+                                               "GBps": pc_bytes / (pc_ms * 1e-3) / 1e9}
+    log(f"red-black DILU isolated (FP32, {nsys} systems): {pc_ms * 1e3:.1f} us/application, "
+        f"{pc_bytes / 1e6:.1f} MB, {pc_bytes / (pc_ms * 1e-3) / 1e9:.0f} GB/s")
     spmv64_ms, spmv64_bytes = ctx.bench_spmv(1, mtsp2.PREC_FP64, reps=20)
     log(f"SpMV (FP64, 1 system): {spmv64_ms*1e3:.1f} us/launch, "
         f"{spmv64_bytes/spmv64_ms/1e6:.0f} GB/s")

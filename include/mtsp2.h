@@ -218,6 +218,13 @@ int mt_host_gradient(int nx, int ny, int nzAir, int nzEarth, const double *dx, c
  * several row-tile group counts per system and keeps the fastest; *variant
  * receives the chosen group count. */
 int mt_autotune(mt_ctx *ctx, int nsys, int *variant);
+/* Times one block-DILU application (MT_PC_DILU or MT_PC_DILU_RB) on nsys
+ * systems, two per frequency, at MT_PREC_FP64 or MT_PREC_MIXED (FP32), with
+ * HIP events on the context stream: average ms per application and its
+ * algorithmic bytes (6 coupling slots + per system right-hand side, result
+ * and pivots).  Has no counterpart in the reference. */
+int mt_bench_precond(mt_ctx *ctx, int nsys, int precision, int precond, int reps,
+                     double *ms_per_apply, double *bytes_per_apply);
 /* Chooses the SpMV of the solvers: 0 row-parallel (every row gathers its 14
  * neighbours from memory); 2 k-marching (a block keeps three k layers of x of
  * an (i, j) tile in LDS); 1 (default) k-marching in FP32 solves and

@@ -209,6 +209,10 @@ static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInf
           if (li + lj + lk == h) sh.pts[w++] = unsigned(li) | (unsigned(lj) << 8) | (unsigned(lk) << 16);
   }
   sh.lvl[sh.nlev] = w;
+  // the red-black sweeps give each lane one point per hyperplane
+  for (int h = 0; h < sh.nlev; ++h)
+    if (sh.lvl[h + 1] - sh.lvl[h] > 64)
+      return mt_fail(ctx, MT_E_STATE, "tile hyperplane wider than a wave");
   MT_HIP_CHECK(ctx, hipMalloc(&tp.tiles, sizeof(int4) * std::max<size_t>(1, flat.size())));
   MT_HIP_CHECK(ctx, hipMalloc(&tp.shape, sizeof(TileShape)));
   MT_HIP_CHECK(ctx, hipMemcpy(tp.tiles, flat.data(), sizeof(int4) * flat.size(), hipMemcpyHostToDevice));
@@ -1313,7 +1317,7 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
   if (precond == MT_PC_DILU) {
     dilu_apply<double, double>(ctx, false, d_act, 1, d_sc, dx, dy, dinv, ST_RUN);
   } else if (precond == MT_PC_DILU_RB) {
-    const DiluArgs<double, double> d = dilu_args<double, double>(
+    DiluArgs<double, double> d = dilu_args<double, double>(
         ctx->tiles_rb, ctx->dop.coef64, g.nE, g.nE, d_act, 1, d_sc, dx, dy, dinv, ST_RUN);
     dilu_rb_apply<double, double>(ctx, ctx->tiles_rb, d);
   } else if (precond == MT_PC_JACOBI) {
@@ -1601,6 +1605,72 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   if (rc) return rc;
   if (ms_per_launch) *ms_per_launch = double(ms) / reps;
   if (bytes_per_launch) *bytes_per_launch = bytes;
+  return MT_OK;
+}
+
+int mt_bench_precond(mt_ctx* ctx, int nsys, int precision, int precond, int reps,
+                     double* ms_per_apply, double* bytes_per_apply) {
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  if (nsys < 1 || nsys > 64 || reps < 1 || (precond != MT_PC_DILU && precond != MT_PC_DILU_RB))
+    return mt_fail(ctx, MT_E_ARG, "mt_bench_precond: bad arguments");
+  hipSetDevice(ctx->device);
+  const Grid& g = ctx->hop.g;
+  const size_t nv = size_t(g.vlen(nsys));
+  // two systems (polarizations) per frequency, as in a forward batch
+  const int nf = (nsys + 1) / 2;
+  std::vector<double> freqs(nf);
+  for (int f = 0; f < nf; ++f) freqs[f] = 2.0 * kPi * std::pow(10.0, -3.0 + 5.0 * f / std::max(1, nf - 1));
+  std::vector<SysScal> h(nsys);
+  std::vector<int> act(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    std::memset(&h[s], 0, sizeof(SysScal));
+    h[s].status = ST_RUN;
+    h[s].fidx = s / 2;
+    h[s].freq_omega = freqs[s / 2];
+    h[s].alpha = {1.0, 0.0};
+    act[s] = s;
+  }
+  SysScal* d_sc;
+  int* d_act;
+  if ((rc = ws(ctx, "bp_sc", nsys, &d_sc)) || (rc = ws(ctx, "bp_act", nsys, &d_act))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(d_sc, h.data(), nsys * sizeof(SysScal), hipMemcpyHostToDevice, ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, act.data(), nsys * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  float ms = 0.f;
+  auto run = [&](auto tagT) -> int {
+    using T = decltype(tagT);
+    C2<T>* dinv;
+    int r2 = build_dinv<T>(ctx, precond, false, freqs, &dinv);
+    if (r2) return r2;
+    C2<T>* vin;
+    C2<T>* vout;
+    if ((r2 = ws(ctx, sizeof(T) == 8 ? "bp_in64" : "bp_in32", nv, &vin)) ||
+        (r2 = ws(ctx, sizeof(T) == 8 ? "bp_out64" : "bp_out32", nv, &vout)))
+      return r2;
+    const TilePlan& tp = precond == MT_PC_DILU_RB ? ctx->tiles_rb : ctx->tiles;
+    DiluArgs<T, T> d = dilu_args<T, T>(tp, view<T>(ctx, false).coef, g.nE, g.nE, d_act,
+                                       nsys, d_sc, vin, vout, dinv, ST_RUN);
+    auto apply = [&]() {
+      if (precond == MT_PC_DILU_RB) {
+        dilu_rb_apply<T, T>(ctx, tp, d);
+      } else {
+        dilu_fwd<T, T>(ctx, tp, d);
+        dilu_bwd<T, T>(ctx, tp, d);
+      }
+    };
+    apply();  // warm-up
+    MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int q = 0; q < reps; ++q) apply();
+    MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    MT_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[1]));
+    MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    if (bytes_per_apply)
+      *bytes_per_apply = 6.0 * double(g.nInt) * sizeof(T) + double(nsys) * g.nInt * 3.0 * 2.0 * sizeof(T);
+    return MT_OK;
+  };
+  rc = (precision == MT_PREC_MIXED) ? run(float()) : run(double());
+  if (rc) return rc;
+  if (ms_per_apply) *ms_per_apply = double(ms) / reps;
   return MT_OK;
 }
 

@@ -86,6 +86,7 @@ _SIGS = {
     "mt_bench_spmv": (_I, [_P, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_timing_enable": (_I, [_P, _I]),
     "mt_set_spmv_variant": (_I, [_P, _I, _I]),
+    "mt_bench_precond": (_I, [_P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
 }
 
@@ -371,6 +372,12 @@ class Context:
         self._chk(lib().mt_kernel_timing(self._h, int(family), C.byref(n), C.byref(ms),
                                          C.byref(by)))
         return n.value, ms.value, by.value
+
+    def bench_precond(self, nsys, precision=PREC_MIXED, precond=PC_DILU_RB, reps=10):
+        ms, by = _D(), _D()
+        self._chk(lib().mt_bench_precond(self._h, int(nsys), int(precision), int(precond),
+                                         int(reps), C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def bench_spmv(self, nsys, precision=PREC_FP64, reps=20):
         ms, by = _D(), _D()
