@@ -16,6 +16,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_BIN = os.path.join(HERE, "_ref", "modem_ref")
+DF_BIN = os.path.join(HERE, "_ref", "ref_datafunc")
 
 
 def available() -> bool:
@@ -141,3 +142,48 @@ def _cleanup(wd, own):
         for fn in os.listdir(wd):
             os.remove(os.path.join(wd, fn))
         os.rmdir(wd)
+
+
+def datafunc_available() -> bool:
+    return os.path.isfile(DF_BIN) and os.access(DF_BIN, os.X_OK)
+
+
+def run_datafunc(grid, origin, omega, dtype, x, e1, e2, d, r=None, az=None, exist=None,
+                 timeout=120):
+    """Runs the reference's dataResp (every site) and LmultT (all sites) for
+    one transmitter and one data type (driver oracle/ref/ref_datafunc.f90).
+    x, r: (nsite, 3); az: (nsite, 7) degrees; d, exist: (nsite, ncomp).
+    Returns (resp (nsite, ncomp), comb1, comb2)."""
+    dx = np.asarray(grid["dx"], np.float64)
+    dy = np.asarray(grid["dy"], np.float64)
+    dz = np.asarray(grid["dz"], np.float64)
+    nx, ny, nz = len(dx), len(dy), len(dz)
+    nza = int(grid["nzAir"])
+    x = np.asarray(x, np.float64).reshape(-1, 3)
+    ns = x.shape[0]
+    d = np.asarray(d, np.float64).reshape(ns, -1)
+    nc = d.shape[1]
+    r = np.zeros_like(x) if r is None else np.asarray(r, np.float64).reshape(ns, 3)
+    if az is None:
+        az = np.tile(np.array([0.0, 90.0, 0.0, 0.0, 90.0, 0.0, 90.0]), (ns, 1))
+    az = np.asarray(az, np.float64).reshape(ns, 7)
+    ex = np.ones((ns, nc), np.int32) if exist is None else np.asarray(exist, np.int32)
+    wd = tempfile.mkdtemp(prefix="mtdf_")
+    inp = os.path.join(wd, "input.bin")
+    with open(inp, "wb") as f:
+        np.array([nx, ny, nza, nz - nza], np.int32).tofile(f)
+        dx.tofile(f); dy.tofile(f); dz.tofile(f)
+        np.asarray(origin, np.float64).tofile(f)
+        np.array([omega], np.float64).tofile(f)
+        np.array([dtype, ns, nc], np.int32).tofile(f)
+        x.tofile(f); r.tofile(f); az.tofile(f)     # row-major (nsite, k) = Fortran (k, nsite)
+        d.tofile(f); ex.tofile(f)
+        np.asarray(e1, np.complex128).tofile(f)
+        np.asarray(e2, np.complex128).tofile(f)
+    subprocess.run([DF_BIN, inp, wd], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    resp = np.fromfile(os.path.join(wd, "resp.bin"), np.float64).reshape(ns, nc)
+    comb = np.fromfile(os.path.join(wd, "comb.bin"), np.complex128)
+    ne = len(comb) // 2
+    _cleanup(wd, True)
+    return resp, comb[:ne], comb[ne:]

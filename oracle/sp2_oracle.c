@@ -1420,3 +1420,614 @@ void sp2o_pmultT(const sp2o_problem *p, double omega, int npol, const zdouble *e
   free(acc);
   free(re);
 }
+
+/* ======================================================================
+ * Data functionals: impedance responses (dataResp), their linearisations
+ * (Lrows) and the adjoint source LmultT.
+ * ====================================================================== */
+
+/* math_constants.f90:32 D2R = PI/180 (truncated PI); :43 LARGE_REAL = 1.0e13
+ * is written as a default-kind (single precision) literal, so its value is the
+ * float nearest 1e13, 9999998976. */
+#define O_D2R (O_PI / 180.0)
+static const double O_LARGE_REAL = (double)1.0e13f;
+
+/* utilities.f90:100-114 clean(x) = dnint(x*LARGE_REAL)/LARGE_REAL */
+static double o_clean(double x) { return round(x * O_LARGE_REAL) / O_LARGE_REAL; }
+
+/* utilities.f90:196-225 minNode: 1-based ix with xNode(ix) <= x < xNode(ix+1);
+ * 0 below the first node, n beyond the last. */
+static int o_min_node(double x, const double *node, int n) {
+  for (int i = 1; i <= n; i++)
+    if (o_clean(node[i - 1]) > o_clean(x)) return i - 1;
+  return n;
+}
+
+/* node and centre coordinates, GridDef.f90:301-399 setup_grid (arrays are
+ * 0-based here: xE[i-1] is the reference's xEdge(i)) */
+typedef struct {
+  double *xE, *yE, *zE, *xC, *yC, *zC;
+} o_coords;
+
+static void o_coords_make(const o_grid *g, const double org[3], o_coords *c) {
+  int i;
+  double cum;
+  c->xE = XALLOC(double, g->nx + 1); c->yE = XALLOC(double, g->ny + 1);
+  c->zE = XALLOC(double, g->nz + 1); c->xC = XALLOC(double, g->nx);
+  c->yC = XALLOC(double, g->ny); c->zC = XALLOC(double, g->nz);
+  c->xE[0] = org[0]; c->yE[0] = org[1]; c->zE[0] = org[2];
+  cum = 0.0;
+  for (i = 0; i < g->nx; i++) { cum = cum + g->dx[i]; c->xE[i + 1] = cum + org[0]; }
+  cum = 0.0;
+  for (i = 0; i < g->ny; i++) { cum = cum + g->dy[i]; c->yE[i + 1] = cum + org[1]; }
+  cum = 0.0;
+  for (i = 0; i < g->nz; i++) { cum = cum + g->dz[i]; c->zE[i + 1] = cum; }
+  double airThick = c->zE[g->nzAir];
+  cum = 0.0;
+  for (i = 0; i < g->nx; i++) { cum = cum + g->delX[i]; c->xC[i] = cum + org[0]; }
+  cum = 0.0;
+  for (i = 0; i < g->ny; i++) { cum = cum + g->delY[i]; c->yC[i] = cum + org[1]; }
+  cum = 0.0;
+  for (i = 0; i < g->nz; i++) { cum = cum + g->delZ[i]; c->zC[i] = cum; }
+  /* the top edge keeps its initial oz before the shift, as in the reference */
+  for (i = 0; i < g->nz; i++) {
+    c->zC[i] = c->zC[i] - airThick + org[2];
+    c->zE[i] = c->zE[i] - airThick + org[2];
+  }
+  c->zE[g->nz] = c->zE[g->nz] - airThick + org[2];
+}
+static void o_coords_free(o_coords *c) {
+  free(c->xE); free(c->yE); free(c->zE); free(c->xC); free(c->yC); free(c->zC);
+}
+
+/* sparsevecc (FIELDS/FiniteDiff3D/sg_sparse_vector.f90:78-102): coefficients
+ * c on grid components (xyz, i, j, k), 1-based indices as in the reference. */
+#define SV_MAX 1024
+typedef struct {
+  int n, face;
+  int i[SV_MAX], j[SV_MAX], k[SV_MAX], xyz[SV_MAX];
+  zdouble c[SV_MAX];
+} o_svec;
+
+static o_svec *sv_new(void) { return XALLOC(o_svec, 1); }
+
+/* sg_sparse_vector.f90:430-501 linComb_sparsevecc: out = c1*a (in a's order),
+ * then each entry of b is added to the FIRST matching entry of a, or appended. */
+static void sv_lincomb(const o_svec *a, zdouble c1, const o_svec *b, zdouble c2, o_svec *out) {
+  o_svec *t = sv_new();
+  int old[SV_MAX];
+  for (int m = 0; m < b->n; m++) {
+    old[m] = -1;
+    for (int n = 0; n < a->n; n++)
+      if (a->xyz[n] == b->xyz[m] && a->i[n] == b->i[m] && a->j[n] == b->j[m] && a->k[n] == b->k[m]) {
+        old[m] = n;
+        break;
+      }
+  }
+  t->face = a->face;
+  for (int n = 0; n < a->n; n++) {
+    t->i[n] = a->i[n]; t->j[n] = a->j[n]; t->k[n] = a->k[n]; t->xyz[n] = a->xyz[n];
+    t->c[n] = c1 * a->c[n];
+  }
+  int nm = a->n;
+  for (int m = 0; m < b->n; m++) {
+    if (old[m] < 0) {
+      if (nm >= SV_MAX) abort();
+      t->i[nm] = b->i[m]; t->j[nm] = b->j[m]; t->k[nm] = b->k[m]; t->xyz[nm] = b->xyz[m];
+      t->c[nm] = c2 * b->c[m];
+      nm++;
+    } else {
+      t->c[old[m]] = t->c[old[m]] + c2 * b->c[m];
+    }
+  }
+  t->n = nm;
+  memcpy(out, t, sizeof(o_svec));
+  free(t);
+}
+
+/* EMfieldInterp.f90:141-324 EinterpSetUp (tri-linear; no conductivity weights) */
+static void o_einterp(const o_grid *g, const o_coords *co, const double x[3], int xyz, o_svec *L) {
+  int nxMax = g->nx + 1, nyMax = g->ny + 1, nzMax = g->nz + 1;
+  int i0 = o_min_node(x[0], co->xE, g->nx + 1);
+  int j0 = o_min_node(x[1], co->yE, g->ny + 1);
+  int k0 = o_min_node(x[2], co->zE, g->nz + 1);
+  double w[3][2];
+  if (xyz == 1) { i0 = o_min_node(x[0], co->xC, g->nx); nxMax = nxMax - 1; }
+  else if (xyz == 2) { j0 = o_min_node(x[1], co->yC, g->ny); nyMax = nyMax - 1; }
+  else { k0 = o_min_node(x[2], co->zC, g->nz); nzMax = nzMax - 1; }
+  if (i0 > 0 && i0 < nxMax)
+    w[0][1] = xyz == 1 ? (x[0] - co->xC[i0 - 1]) / g->delX[i0] : (x[0] - co->xE[i0 - 1]) / g->dx[i0 - 1];
+  else w[0][1] = i0 <= 0 ? 1.0 : 0.0;
+  if (j0 > 0 && j0 < nyMax)
+    w[1][1] = xyz == 2 ? (x[1] - co->yC[j0 - 1]) / g->delY[j0] : (x[1] - co->yE[j0 - 1]) / g->dy[j0 - 1];
+  else w[1][1] = j0 <= 0 ? 1.0 : 0.0;
+  if (k0 > 0 && k0 < nzMax)
+    w[2][1] = xyz == 3 ? (x[2] - co->zC[k0 - 1]) / g->delZ[k0] : (x[2] - co->zE[k0 - 1]) / g->dz[k0 - 1];
+  else w[2][1] = k0 <= 0 ? 1.0 : 0.0;
+  for (int d = 0; d < 3; d++) w[d][0] = 1.0 - w[d][1];
+  L->n = 0;
+  L->face = 0;
+  for (int n = 0; n < 2; n++)
+    for (int m = 0; m < 2; m++)
+      for (int p = 0; p < 2; p++) {
+        double wadd = w[0][n] * w[1][m] * w[2][p];
+        if (wadd > 0) {
+          int q = L->n++;
+          L->i[q] = i0 + n; L->j[q] = j0 + m; L->k[q] = k0 + p; L->xyz[q] = xyz;
+          L->c[q] = CMPLX(wadd, 0.0);
+        }
+      }
+}
+
+/* EMfieldInterp.f90:329-465 BinterpSetUp (face-centred H, tri-linear) */
+static void o_binterp(const o_grid *g, const o_coords *co, const double x[3], int xyz, o_svec *L) {
+  int nxMax = g->nx, nyMax = g->ny, nzMax = g->nz, i0, j0, k0;
+  double w[3][2];
+  if (xyz == 1) {
+    i0 = o_min_node(x[0], co->xE, g->nx + 1); j0 = o_min_node(x[1], co->yC, g->ny);
+    k0 = o_min_node(x[2], co->zC, g->nz); nxMax = nxMax + 1;
+  } else if (xyz == 2) {
+    i0 = o_min_node(x[0], co->xC, g->nx); j0 = o_min_node(x[1], co->yE, g->ny + 1);
+    k0 = o_min_node(x[2], co->zC, g->nz); nyMax = nyMax + 1;
+  } else {
+    i0 = o_min_node(x[0], co->xC, g->nx); j0 = o_min_node(x[1], co->yC, g->ny);
+    k0 = o_min_node(x[2], co->zE, g->nz + 1); nzMax = nzMax + 1;
+  }
+  if (i0 > 0 && i0 < nxMax)
+    w[0][1] = xyz == 1 ? (x[0] - co->xE[i0 - 1]) / g->dx[i0 - 1] : (x[0] - co->xC[i0 - 1]) / g->delX[i0];
+  else w[0][1] = i0 <= 0 ? 1.0 : 0.0;
+  if (j0 > 0 && j0 < nyMax)
+    w[1][1] = xyz == 2 ? (x[1] - co->yE[j0 - 1]) / g->dy[j0 - 1] : (x[1] - co->yC[j0 - 1]) / g->delY[j0];
+  else w[1][1] = j0 <= 0 ? 1.0 : 0.0;
+  if (k0 > 0 && k0 < nzMax)
+    w[2][1] = xyz == 3 ? (x[2] - co->zE[k0 - 1]) / g->dz[k0 - 1] : (x[2] - co->zC[k0 - 1]) / g->delZ[k0];
+  else w[2][1] = k0 <= 0 ? 1.0 : 0.0;
+  for (int d = 0; d < 3; d++) w[d][0] = 1.0 - w[d][1];
+  L->n = 0;
+  L->face = 1;
+  for (int n = 0; n < 2; n++)
+    for (int m = 0; m < 2; m++)
+      for (int p = 0; p < 2; p++) {
+        double wadd = w[0][n] * w[1][m] * w[2][p];
+        if (wadd > 0) {
+          int q = L->n++;
+          L->i[q] = i0 + n; L->j[q] = j0 + m; L->k[q] = k0 + p; L->xyz[q] = xyz;
+          L->c[q] = CMPLX(wadd, 0.0);
+        }
+      }
+}
+
+/* EMfieldInterp.f90:470-647 BfromESetUp: H at x from the curl of E on the four
+ * edges around each face of the B interpolation, h = e*sum(l)/S, times
+ * i_omega_inv = ISIGN*i/omega.  l_E and S_F are GridCalc.f90:230-254 EdgeLength
+ * and :373-418 FaceArea (dx, dy, dz and their products). */
+static void o_bfrome(const o_grid *g, const o_coords *co, double omega, const double x[3], int xyz,
+                     o_svec *L) {
+  zdouble iw = O_ISIGN * CMPLX(0.0, 1.0 / omega);
+  o_svec *LCH = sv_new(), *one = sv_new(), *tmp = sv_new();
+  o_binterp(g, co, x, xyz, LCH);
+  one->face = 0;
+  for (int ii = 0; ii < LCH->n; ii++) {
+    int EI[4], EJ[4], EK[4], AX[4];
+    zdouble C[4];
+    const int i = LCH->i[ii], j = LCH->j[ii], k = LCH->k[ii];
+    /* l_E%x(i,.,.) = dx(i), %y = dy(j), %z = dz(k); S_F%x = dy*dz, %y = dx*dz, %z = dx*dy */
+    if (LCH->xyz[ii] == 1) {
+      int ax[4] = {2, 2, 3, 3}, jj[4] = {j, j, j, j + 1}, kk[4] = {k, k + 1, k, k};
+      double S = g->dy[j - 1] * g->dz[k - 1];
+      for (int q = 0; q < 4; q++) { AX[q] = ax[q]; EI[q] = i; EJ[q] = jj[q]; EK[q] = kk[q]; }
+      C[0] = CMPLX(1 * g->dy[EJ[0] - 1] / S, 0.0);
+      C[1] = CMPLX(-1 * g->dy[EJ[1] - 1] / S, 0.0);
+      C[2] = CMPLX(-1 * g->dz[EK[2] - 1] / S, 0.0);
+      C[3] = CMPLX(1 * g->dz[EK[3] - 1] / S, 0.0);
+    } else if (LCH->xyz[ii] == 2) {
+      int ax[4] = {3, 3, 1, 1}, ii4[4] = {i, i + 1, i, i}, kk[4] = {k, k, k, k + 1};
+      double S = g->dx[i - 1] * g->dz[k - 1];
+      for (int q = 0; q < 4; q++) { AX[q] = ax[q]; EI[q] = ii4[q]; EJ[q] = j; EK[q] = kk[q]; }
+      C[0] = CMPLX(1 * g->dz[EK[0] - 1] / S, 0.0);
+      C[1] = CMPLX(-1 * g->dz[EK[1] - 1] / S, 0.0);
+      C[2] = CMPLX(-1 * g->dx[EI[2] - 1] / S, 0.0);
+      C[3] = CMPLX(1 * g->dx[EI[3] - 1] / S, 0.0);
+    } else {
+      int ax[4] = {1, 1, 2, 2}, ii4[4] = {i, i, i, i + 1}, jj[4] = {j, j + 1, j, j};
+      double S = g->dx[i - 1] * g->dy[j - 1];
+      for (int q = 0; q < 4; q++) { AX[q] = ax[q]; EI[q] = ii4[q]; EJ[q] = jj[q]; EK[q] = k; }
+      C[0] = CMPLX(1 * g->dx[EI[0] - 1] / S, 0.0);
+      C[1] = CMPLX(-1 * g->dx[EI[1] - 1] / S, 0.0);
+      C[2] = CMPLX(-1 * g->dy[EJ[2] - 1] / S, 0.0);
+      C[3] = CMPLX(1 * g->dy[EJ[3] - 1] / S, 0.0);
+    }
+    if (ii == 0) {
+      L->n = 4;
+      L->face = 0;
+      for (int q = 0; q < 4; q++) {
+        L->i[q] = EI[q]; L->j[q] = EJ[q]; L->k[q] = EK[q]; L->xyz[q] = AX[q];
+        L->c[q] = C[q] * LCH->c[0] * iw;
+      }
+    } else {
+      memcpy(tmp, L, sizeof(o_svec));
+      one->n = 4;
+      for (int q = 0; q < 4; q++) {
+        one->i[q] = EI[q]; one->j[q] = EJ[q]; one->k[q] = EK[q]; one->xyz[q] = AX[q]; one->c[q] = C[q];
+      }
+      sv_lincomb(tmp, CMPLX(1.0, 0.0), one, LCH->c[ii] * iw, L);
+    }
+  }
+  free(LCH); free(one); free(tmp);
+}
+
+/* all-edge 0-based index of (xyz, i, j, k) (1-based), [x ; y ; z] ordering */
+static int o_edge_index(const sp2o_problem *p, int xyz, int i, int j, int k) {
+  int off = xyz == 1 ? 0 : (xyz == 2 ? p->nXe : p->nXe + p->nYe);
+  return off + vidx(&p->g, XEDGE + xyz - 1, i - 1, j - 1, k - 1);
+}
+
+/* sg_sparse_vector.f90:669-728 dotProd_noConj_scvector_f */
+static zdouble o_sdot(const sp2o_problem *p, const o_svec *L, const zdouble *e) {
+  zdouble c = 0.0;
+  for (int q = 0; q < L->n; q++) c = c + L->c[q] * e[o_edge_index(p, L->xyz[q], L->i[q], L->j[q], L->k[q])];
+  return c;
+}
+
+/* sg_sparse_vector.f90:803-860 add_scvector: v = cs*L + v entry by entry */
+static void o_sadd(const sp2o_problem *p, zdouble cs, const o_svec *L, zdouble *v) {
+  for (int q = 0; q < L->n; q++) {
+    int e = o_edge_index(p, L->xyz[q], L->i[q], L->j[q], L->k[q]);
+    v[e] = cs * L->c[q] + v[e];
+  }
+}
+
+/* DataFunc.f90:931-957 rotate_Model2Data */
+static void o_rotate(zdouble *ch1, zdouble *ch2, double a1, double a2, double a) {
+  zdouble t1 = *ch1, t2 = *ch2;
+  double tmpv = 1.0 / (sin(a2 * O_D2R - a1 * O_D2R));
+  zdouble r1 = sin(a2 * O_D2R - a * O_D2R) * t1 - cos(a2 * O_D2R - a * O_D2R) * t2;
+  *ch1 = zdivr(r1, tmpv);
+  zdouble r2 = -sin(a1 * O_D2R - a * O_D2R) * t1 + cos(a1 * O_D2R - a * O_D2R) * t2;
+  *ch2 = zdivr(r2, tmpv);
+}
+
+/* inverse of the 2x2 matrix M (column j = mode) by Kramer's rule, in place,
+ * DataFunc.f90:165-172 */
+static void o_kramer(zdouble M[3][2]) {
+  zdouble det = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+  zdouble t = M[0][0];
+  M[0][0] = zdiv(M[1][1], det);
+  M[1][1] = zdiv(t, det);
+  M[0][1] = zdiv(-M[0][1], det);
+  M[1][0] = zdiv(-M[1][0], det);
+}
+
+/* dataTypes.f90:87-222: number of real components and complex flag per type */
+static int o_dt_ncomp(int dt, int *cplx) {
+  static const int nc[12] = {0, 8, 4, 4, 8, 4, 4, 2, 2, 2, 2, 2};
+  static const int cx[12] = {0, 1, 1, 1, 1, 0, 0, 1, 1, 1, 1, 1};
+  if (dt < 1 || dt > 11) return 0;
+  if (cplx) *cplx = cx[dt];
+  return nc[dt];
+}
+
+/* one receiver: location x, reference location r, azimuths (degrees):
+ * Hx, Hy, Hz, Ex, Ey, Hx_ref, Hy_ref (fields_orientation.f90:23-58) */
+typedef struct {
+  double x[3], r[3], az[7];
+} o_site;
+
+/* DataFunc.f90:43-462 dataResp.  Z[6] is the module state the reference keeps
+ * for Lrows (:37); Binv (2x2, [row][mode]) is its optional output. */
+static void o_data_resp(const sp2o_problem *p, const o_coords *co, int dt, double omega,
+                        const o_site *s, const zdouble *e1, const zdouble *e2, double *resp,
+                        zdouble Z[6], zdouble Binv[2][2]) {
+  const o_grid *g = &p->g;
+  const zdouble *pol[2] = {e1, e2};
+  zdouble EE[3][2], BB[3][2], RR[3][2], tz[4];
+  double Hx = s->az[0], Hy = s->az[1], Ex = s->az[3], Ey = s->az[4], Hxr = s->az[5], Hyr = s->az[6];
+  o_svec *Lex = sv_new(), *Ley = sv_new(), *Lbx = sv_new(), *Lby = sv_new(), *Lbz = sv_new(),
+         *Lrx = sv_new(), *Lry = sv_new();
+  int cplx = 0, ncomp = o_dt_ncomp(dt, &cplx), nfunc = cplx ? ncomp / 2 : ncomp;
+  memset(EE, 0, sizeof(EE)); memset(BB, 0, sizeof(BB)); memset(RR, 0, sizeof(RR));
+  switch (dt) {
+    case 7: o_einterp(g, co, s->x, 1, Lex); for (int q = 0; q < 6; q++) Z[q] = o_sdot(p, Lex, e1); break;
+    case 8: o_einterp(g, co, s->x, 2, Ley); for (int q = 0; q < 6; q++) Z[q] = o_sdot(p, Ley, e1); break;
+    case 9: o_bfrome(g, co, omega, s->x, 1, Lbx); for (int q = 0; q < 6; q++) Z[q] = o_sdot(p, Lbx, e1); break;
+    case 10: o_bfrome(g, co, omega, s->x, 2, Lby); for (int q = 0; q < 6; q++) Z[q] = o_sdot(p, Lby, e1); break;
+    case 11: o_bfrome(g, co, omega, s->x, 3, Lbz); for (int q = 0; q < 6; q++) Z[q] = o_sdot(p, Lbz, e1); break;
+    case 1: case 2: case 5: case 6:
+      o_einterp(g, co, s->x, 1, Lex);
+      o_einterp(g, co, s->x, 2, Ley);
+      o_bfrome(g, co, omega, s->x, 1, Lbx);
+      o_bfrome(g, co, omega, s->x, 2, Lby);
+      for (int m = 0; m < 2; m++) {
+        EE[0][m] = o_sdot(p, Lex, pol[m]);
+        EE[1][m] = o_sdot(p, Ley, pol[m]);
+        o_rotate(&EE[0][m], &EE[1][m], Ex, Ey, 0.0);
+        BB[0][m] = o_sdot(p, Lbx, pol[m]);
+        BB[1][m] = o_sdot(p, Lby, pol[m]);
+        o_rotate(&BB[0][m], &BB[1][m], Hx, Hy, 0.0);
+      }
+      o_kramer(BB);
+      if (dt == 1) {
+        for (int j = 0; j < 2; j++)
+          for (int i = 0; i < 2; i++) Z[2 * i + j] = EE[i][0] * BB[0][j] + EE[i][1] * BB[1][j];
+      } else if (dt == 2) {
+        Z[0] = EE[0][0] * BB[0][1] + EE[0][1] * BB[1][1];
+        Z[1] = EE[1][0] * BB[0][0] + EE[1][1] * BB[1][0];
+      } else if (dt == 5) {
+        tz[0] = EE[0][0] * BB[0][1] + EE[0][1] * BB[1][1];
+        tz[1] = EE[1][0] * BB[0][0] + EE[1][1] * BB[1][0];
+        Z[0] = log10(pow(cabs(tz[0]), 2) * O_MU0 / omega);
+        Z[1] = atan2(O_ISIGN * cimag(tz[0]), creal(tz[0]));
+        Z[2] = log10(pow(cabs(tz[1]), 2) * O_MU0 / omega);
+        Z[3] = atan2(O_ISIGN * cimag(tz[1]), creal(tz[1]));
+      } else {
+        for (int j = 0; j < 2; j++)
+          for (int i = 0; i < 2; i++) tz[2 * i + j] = EE[i][0] * BB[0][j] + EE[i][1] * BB[1][j];
+        double detX = creal(tz[0]) * creal(tz[3]) - creal(tz[1]) * creal(tz[2]);
+        Z[0] = O_ISIGN * (creal(tz[3]) * cimag(tz[0]) - creal(tz[1]) * cimag(tz[2])) / detX;
+        Z[1] = O_ISIGN * (creal(tz[3]) * cimag(tz[1]) - creal(tz[1]) * cimag(tz[3])) / detX;
+        Z[2] = O_ISIGN * (creal(tz[0]) * cimag(tz[2]) - creal(tz[2]) * cimag(tz[0])) / detX;
+        Z[3] = O_ISIGN * (creal(tz[0]) * cimag(tz[3]) - creal(tz[2]) * cimag(tz[1])) / detX;
+      }
+      break;
+    case 3:
+      o_bfrome(g, co, omega, s->x, 1, Lbx);
+      o_bfrome(g, co, omega, s->x, 2, Lby);
+      o_bfrome(g, co, omega, s->x, 3, Lbz);
+      for (int m = 0; m < 2; m++) {
+        BB[0][m] = o_sdot(p, Lbx, pol[m]);
+        BB[1][m] = o_sdot(p, Lby, pol[m]);
+        BB[2][m] = o_sdot(p, Lbz, pol[m]);
+        o_rotate(&BB[0][m], &BB[1][m], Hx, Hy, 0.0);
+      }
+      o_kramer(BB);
+      Z[0] = BB[2][0] * BB[0][0] + BB[2][1] * BB[1][0];
+      Z[1] = BB[2][0] * BB[0][1] + BB[2][1] * BB[1][1];
+      break;
+    case 4:
+      o_bfrome(g, co, omega, s->x, 1, Lbx);
+      o_bfrome(g, co, omega, s->x, 2, Lby);
+      o_bfrome(g, co, omega, s->r, 1, Lrx);
+      o_bfrome(g, co, omega, s->r, 2, Lry);
+      for (int m = 0; m < 2; m++) {
+        BB[0][m] = o_sdot(p, Lbx, pol[m]);
+        BB[1][m] = o_sdot(p, Lby, pol[m]);
+        o_rotate(&BB[0][m], &BB[1][m], Hx, Hy, 0.0);
+        RR[0][m] = o_sdot(p, Lrx, pol[m]);
+        RR[1][m] = o_sdot(p, Lry, pol[m]);
+        o_rotate(&RR[0][m], &RR[1][m], Hxr, Hyr, 0.0);
+      }
+      o_kramer(RR);
+      for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) Z[2 * i + j] = BB[i][0] * RR[0][j] + BB[i][1] * RR[1][j];
+      Z[0] = Z[0] - 1.0;
+      Z[3] = Z[3] - 1.0;
+      break;
+    default: break;
+  }
+  if (resp) {
+    int ic = 0;
+    for (int f = 0; f < nfunc; f++) {
+      resp[ic++] = creal(Z[f]);
+      if (cplx) resp[ic++] = cimag(Z[f]);
+    }
+  }
+  if (Binv)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) Binv[a][b] = dt == 4 ? RR[a][b] : BB[a][b];
+  free(Lex); free(Ley); free(Lbx); free(Lby); free(Lbz); free(Lrx); free(Lry);
+}
+
+/* DataFunc.f90:465-869 Lrows: sparse rows of the linearised functionals of
+ * one site, L[n][k] for functional n applied to mode k.  Returns nfunc. */
+static int o_lrows(const sp2o_problem *p, const o_coords *co, int dt, double omega, const o_site *s,
+                   const zdouble *e1, const zdouble *e2, o_svec *L[4][2]) {
+  const o_grid *g = &p->g;
+  zdouble Z[6] = {0}, Binv[2][2];
+  int IJ[3][4] = {{0}}, ncomp = 0, hz = 0;
+  switch (dt) {
+    case 1: case 6:
+      ncomp = 4;
+      for (int j = 1; j <= 2; j++)
+        for (int i = 1; i <= 2; i++) {
+          IJ[0][2 * (i - 1) + j - 1] = i; IJ[1][2 * (i - 1) + j - 1] = j; IJ[2][2 * (i - 1) + j - 1] = i;
+        }
+      o_data_resp(p, co, 1, omega, s, e1, e2, NULL, Z, Binv);
+      break;
+    case 2: case 5:
+      ncomp = 2;
+      IJ[0][0] = 1; IJ[1][0] = 2; IJ[0][1] = 2; IJ[1][1] = 1; IJ[2][0] = 1; IJ[2][1] = 2;
+      o_data_resp(p, co, 1, omega, s, e1, e2, NULL, Z, Binv);
+      break;
+    case 3:
+      ncomp = 2; hz = 1;
+      IJ[0][0] = 1; IJ[0][1] = 1; IJ[1][0] = 1; IJ[1][1] = 2; IJ[2][0] = 3; IJ[2][1] = 3;
+      o_data_resp(p, co, 3, omega, s, e1, e2, NULL, Z, Binv);
+      break;
+    case 4:
+      ncomp = 4;
+      for (int j = 1; j <= 2; j++)
+        for (int i = 1; i <= 2; i++) {
+          IJ[0][2 * (i - 1) + j - 1] = i; IJ[1][2 * (i - 1) + j - 1] = j; IJ[2][2 * (i - 1) + j - 1] = i + 3;
+        }
+      o_data_resp(p, co, 4, omega, s, e1, e2, NULL, Z, Binv);
+      break;
+    case 11: hz = 1; ncomp = 1; break;
+    default: ncomp = 1; break;
+  }
+  o_svec *Lex = sv_new(), *Ley = sv_new(), *Lbx = sv_new(), *Lby = sv_new(), *Lbz = sv_new(),
+         *Lrx = sv_new(), *Lry = sv_new(), *Lexr = sv_new(), *Leyr = sv_new(), *Lbxr = sv_new(),
+         *Lbyr = sv_new(), *Lrxr = sv_new(), *Lryr = sv_new(), *L1 = sv_new(), *L2 = sv_new(),
+         *L3 = sv_new(), *P[4];
+  for (int q = 0; q < 4; q++) P[q] = sv_new();
+  o_einterp(g, co, s->x, 1, Lex);
+  o_einterp(g, co, s->x, 2, Ley);
+  o_bfrome(g, co, omega, s->x, 1, Lbx);
+  o_bfrome(g, co, omega, s->x, 2, Lby);
+  if (hz) o_bfrome(g, co, omega, s->x, 3, Lbz);
+  o_bfrome(g, co, omega, s->r, 1, Lrx);
+  o_bfrome(g, co, omega, s->r, 2, Lry);
+  double Hx = s->az[0] * O_D2R, Hy = s->az[1] * O_D2R, Ex = s->az[3] * O_D2R, Ey = s->az[4] * O_D2R;
+  double Hxr = s->az[5] * O_D2R, Hyr = s->az[6] * O_D2R;
+  sv_lincomb(Lex, sin(Ey) / sin(Ey - Ex), Ley, -cos(Ey) / sin(Ey - Ex), Lexr);
+  sv_lincomb(Lex, -sin(Ex) / sin(Ey - Ex), Ley, cos(Ex) / sin(Ey - Ex), Leyr);
+  sv_lincomb(Lbx, sin(Hy) / sin(Hy - Hx), Lby, -cos(Hy) / sin(Hy - Hx), Lbxr);
+  sv_lincomb(Lbx, -sin(Hx) / sin(Hy - Hx), Lby, cos(Hx) / sin(Hy - Hx), Lbyr);
+  sv_lincomb(Lrx, sin(Hyr) / sin(Hyr - Hxr), Lry, -cos(Hyr) / sin(Hyr - Hxr), Lrxr);
+  sv_lincomb(Lrx, -sin(Hxr) / sin(Hyr - Hxr), Lry, cos(Hxr) / sin(Hyr - Hxr), Lryr);
+  const zdouble one = CMPLX(1.0, 0.0), zero = CMPLX(0.0, 0.0);
+  if (dt >= 7) {
+    const o_svec *src = dt == 7 ? Lex : dt == 8 ? Ley : dt == 9 ? Lbx : dt == 10 ? Lby : Lbz;
+    memcpy(L[0][0], src, sizeof(o_svec));
+    L[0][1]->n = 0;   /* created empty, never assigned */
+  } else {
+    for (int n = 0; n < ncomp; n++) {
+      int i = IJ[0][n], j = IJ[1][n], pc = IJ[2][n];
+      zdouble c1 = Z[2 * (i - 1)], c2 = Z[2 * (i - 1) + 1];
+      if (dt == 4) sv_lincomb(Lrxr, c1, Lryr, c2, L1);
+      else sv_lincomb(Lbxr, c1, Lbyr, c2, L1);
+      for (int k = 0; k < 2; k++) {
+        zdouble b = Binv[k][j - 1];
+        const o_svec *row = pc == 1 ? Lexr : pc == 2 ? Leyr : pc == 3 ? Lbz : pc == 4 ? Lbxr : Lbyr;
+        sv_lincomb(row, b, L1, -b, L[n][k]);
+      }
+    }
+  }
+  if (dt == 5) {
+    for (int k = 0; k < 2; k++) {
+      double a3 = cabs(Z[2]), a2 = cabs(Z[1]);
+      /* complex/real quotients go through __divdc3 as in the reference */
+      zdouble c = zdivr(CMPLX(0.0, 1.0) * conj(Z[2]), a3 * a3);
+      sv_lincomb(L[1][k], c, L[1][k], zero, L[3][k]);
+      c = zdivr(zdivr(2.0 * conj(Z[2]), a3 * a3), log(10.0));
+      sv_lincomb(L[1][k], c, L[1][k], zero, L[2][k]);
+      c = zdivr(CMPLX(0.0, 1.0) * conj(Z[1]), a2 * a2);
+      sv_lincomb(L[0][k], c, L[0][k], zero, L[1][k]);
+      c = zdivr(zdivr(2.0 * conj(Z[1]), a2 * a2), log(10.0));
+      sv_lincomb(L[0][k], c, L[0][k], zero, L1);
+      memcpy(L[0][k], L1, sizeof(o_svec));
+    }
+  }
+  if (dt == 6) {
+    double r1 = creal(Z[0]), r2 = creal(Z[1]), r3 = creal(Z[2]), r4 = creal(Z[3]);
+    double m1 = cimag(Z[0]), m2 = cimag(Z[1]), m3 = cimag(Z[2]), m4 = cimag(Z[3]);
+    for (int k = 0; k < 2; k++) {
+      double detX = r1 * r4 - r2 * r3, PT[2][2];
+      PT[0][0] = O_ISIGN * (r4 * m1 - r2 * m3) / detX;
+      PT[0][1] = O_ISIGN * (r4 * m2 - r2 * m4) / detX;
+      PT[1][0] = O_ISIGN * (r1 * m3 - r3 * m1) / detX;
+      PT[1][1] = O_ISIGN * (r1 * m4 - r3 * m2) / detX;
+      /* PTXX */
+      sv_lincomb(L[0][k], CMPLX(-1.0 * PT[0][0] * r4 / detX, 0.0), L[1][k],
+                 CMPLX((PT[0][0] * r3 - O_ISIGN * m3) / detX, 0.0), L1);
+      sv_lincomb(L[2][k], CMPLX(PT[0][0] * r2 / detX, 0.0), L[3][k],
+                 CMPLX((-1.0 * PT[0][0] * r1 + O_ISIGN * m1) / detX, 0.0), L2);
+      sv_lincomb(L1, one, L2, one, L3);
+      sv_lincomb(L[0][k], CMPLX(0.0, r4 / detX), L[2][k], CMPLX(0.0, -1.0 * r2 / detX), L1);
+      sv_lincomb(L3, one, L1, one, P[0]);
+      /* PTXY */
+      sv_lincomb(L[0][k], CMPLX(-1.0 * PT[0][1] * r4 / detX, 0.0), L[1][k],
+                 CMPLX((PT[0][1] * r3 - O_ISIGN * m4) / detX, 0.0), L1);
+      sv_lincomb(L[2][k], CMPLX(PT[0][1] * r2 / detX, 0.0), L[3][k],
+                 CMPLX((-1.0 * PT[0][1] * r1 + O_ISIGN * m2) / detX, 0.0), L2);
+      sv_lincomb(L1, one, L2, one, L3);
+      sv_lincomb(L[1][k], CMPLX(0.0, r4 / detX), L[3][k], CMPLX(0.0, -1.0 * r2 / detX), L1);
+      sv_lincomb(L3, one, L1, one, P[1]);
+      /* PTYX */
+      sv_lincomb(L[0][k], CMPLX((-1.0 * PT[1][0] * r4 + O_ISIGN * m3) / detX, 0.0), L[1][k],
+                 CMPLX(PT[1][0] * r3 / detX, 0.0), L1);
+      sv_lincomb(L[2][k], CMPLX((PT[1][0] * r2 - O_ISIGN * m1) / detX, 0.0), L[3][k],
+                 CMPLX(-1.0 * PT[1][0] * r1 / detX, 0.0), L2);
+      sv_lincomb(L1, one, L2, one, L3);
+      sv_lincomb(L[0][k], CMPLX(0.0, -1.0 * r3 / detX), L[2][k], CMPLX(0.0, r1 / detX), L1);
+      sv_lincomb(L3, one, L1, one, P[2]);
+      /* PTYY */
+      sv_lincomb(L[0][k], CMPLX((-1.0 * PT[1][1] * r4 + O_ISIGN * m4) / detX, 0.0), L[1][k],
+                 CMPLX(PT[1][1] * r3 / detX, 0.0), L1);
+      sv_lincomb(L[2][k], CMPLX((PT[1][1] * r2 - O_ISIGN * m2) / detX, 0.0), L[3][k],
+                 CMPLX(-1.0 * PT[1][1] * r1 / detX, 0.0), L2);
+      sv_lincomb(L1, one, L2, one, L3);
+      sv_lincomb(L[1][k], CMPLX(0.0, -1.0 * r3 / detX), L[3][k], CMPLX(0.0, r1 / detX), L1);
+      sv_lincomb(L3, one, L1, one, P[3]);
+      for (int q = 0; q < 4; q++) sv_lincomb(P[q], one, P[q], zero, L[q][k]);
+    }
+  }
+  free(Lex); free(Ley); free(Lbx); free(Lby); free(Lbz); free(Lrx); free(Lry);
+  free(Lexr); free(Leyr); free(Lbxr); free(Lbyr); free(Lrxr); free(Lryr);
+  free(L1); free(L2); free(L3);
+  for (int q = 0; q < 4; q++) free(P[q]);
+  int cplx = 0, nc = o_dt_ncomp(dt, &cplx);
+  return cplx ? nc / 2 : nc;
+}
+
+static void o_sites(int nsite, const double *x, const double *r, const double *az, o_site *S) {
+  static const double az0[7] = {0.0, 90.0, 0.0, 0.0, 90.0, 0.0, 90.0};
+  for (int s = 0; s < nsite; s++) {
+    for (int d = 0; d < 3; d++) {
+      S[s].x[d] = x[3 * s + d];
+      S[s].r[d] = r ? r[3 * s + d] : 0.0;
+    }
+    for (int d = 0; d < 7; d++) S[s].az[d] = az ? az[7 * s + d] : az0[d];
+  }
+}
+
+int sp2o_data_resp(const sp2o_problem *p, int dt, double omega, const double *origin, int nsite,
+                   const double *x, const double *r, const double *az, const zdouble *e1,
+                   const zdouble *e2, double *resp) {
+  int cplx = 0, nc = o_dt_ncomp(dt, &cplx);
+  if (!nc) return -1;
+  o_coords co;
+  o_coords_make(&p->g, origin, &co);
+  o_site *S = XALLOC(o_site, nsite);
+  o_sites(nsite, x, r, az, S);
+  for (int s = 0; s < nsite; s++) {
+    zdouble Z[6] = {0}, B[2][2];
+    o_data_resp(p, &co, dt, omega, &S[s], e1, e2, resp + (size_t)s * nc, Z, B);
+  }
+  free(S);
+  o_coords_free(&co);
+  return nc;
+}
+
+/* SENS/DataSens.f90:130-225 LmultT for one transmitter and one data type:
+ * comb_k = sum over sites and functionals of conj-data * L(n)%L(k), added
+ * entry by entry (add_sparseVrhsV, SolnSpace.f90:1026-1067).  Complex data
+ * enter as cmplx(re, -im), real data as cmplx(v, 0); a functional counts
+ * when exist() of its first component is set. */
+int sp2o_lmultT(const sp2o_problem *p, int dt, double omega, const double *origin, int nsite,
+                const double *x, const double *r, const double *az, const double *d,
+                const unsigned char *exist, const zdouble *e1, const zdouble *e2, zdouble *comb1,
+                zdouble *comb2) {
+  int cplx = 0, nc = o_dt_ncomp(dt, &cplx);
+  if (!nc) return -1;
+  o_coords co;
+  o_coords_make(&p->g, origin, &co);
+  o_site *S = XALLOC(o_site, nsite);
+  o_sites(nsite, x, r, az, S);
+  o_svec *L[4][2];
+  for (int n = 0; n < 4; n++)
+    for (int k = 0; k < 2; k++) L[n][k] = sv_new();
+  for (int e = 0; e < p->nE; e++) comb1[e] = comb2[e] = 0.0;
+  zdouble *comb[2] = {comb1, comb2};
+  for (int s = 0; s < nsite; s++) {
+    int nfunc = o_lrows(p, &co, dt, omega, &S[s], e1, e2, L);
+    int ic = 0;
+    for (int f = 0; f < nfunc; f++) {
+      int ex = exist ? exist[(size_t)s * nc + ic] : 1;
+      zdouble z;
+      if (cplx) {
+        z = CMPLX(d[(size_t)s * nc + ic], -d[(size_t)s * nc + ic + 1]);
+        ic += 2;
+      } else {
+        z = CMPLX(d[(size_t)s * nc + ic], 0.0);
+        ic += 1;
+      }
+      if (ex)
+        for (int k = 0; k < 2; k++) o_sadd(p, z, L[f][k], comb[k]);
+    }
+  }
+  for (int n = 0; n < 4; n++)
+    for (int k = 0; k < 2; k++) free(L[n][k]);
+  free(S);
+  o_coords_free(&co);
+  return nc;
+}

@@ -93,4 +93,19 @@ void sp2o_pmultT(const sp2o_problem *p, double omega, int npol, const zdouble *e
 /* cell-conductivity -> edge/node maps alone (LOGE), for a fresh model vector */
 void sp2o_model2edge(const sp2o_problem *p, const double *logcond, double *sigEdge);
 
+/* Data functionals (DataFunc.f90:43-869, EMfieldInterp.f90:141-647) and the
+ * adjoint source LmultT (SENS/DataSens.f90:130-225) for one transmitter.
+ * dt: data type numbering of dataTypes.f90:65-77 (1..11).  Sites: x, r
+ * (3*nsite, location and reference location in the grid frame), az (7*nsite,
+ * degrees Hx,Hy,Hz,Ex,Ey,Hx_ref,Hy_ref; NULL = 0/90 defaults); origin = the
+ * grid's (ox, oy, oz).  e1/e2: all-edge fields of the two polarisations.
+ * resp / d / exist: nsite x ncomp(dt) (site-major).  Return ncomp, or -1. */
+int sp2o_data_resp(const sp2o_problem *p, int dt, double omega, const double *origin, int nsite,
+                   const double *x, const double *r, const double *az, const zdouble *e1,
+                   const zdouble *e2, double *resp);
+int sp2o_lmultT(const sp2o_problem *p, int dt, double omega, const double *origin, int nsite,
+                const double *x, const double *r, const double *az, const double *d,
+                const unsigned char *exist, const zdouble *e1, const zdouble *e2, zdouble *comb1,
+                zdouble *comb2);
+
 #endif

@@ -60,6 +60,9 @@ def lib():
             "sp2o_edge2model": (None, [_P, _PD, _PD]),
             "sp2o_pmultT": (None, [_P, _D, _I, _PZ, _PZ, _PD]),
             "sp2o_model2edge": (None, [_P, _PD, _PD]),
+            "sp2o_data_resp": (_I, [_P, _I, _D, _PD, _I, _PD, _P, _P, _PZ, _PZ, _PD]),
+            "sp2o_lmultT": (_I, [_P, _I, _D, _PD, _I, _PD, _P, _P, _PD, _P, _PZ, _PZ,
+                                 _PZ, _PZ]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -214,3 +217,46 @@ class Problem:
         lib().sp2o_model2edge(self._h, np.ascontiguousarray(
             np.asarray(logcond, np.float64).ravel(order="F")), se)
         return se
+
+    # ---- data functionals: dataResp / LmultT for one transmitter -------------
+    @staticmethod
+    def _sites(x, r, az):
+        x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, 3))
+        rr = None if r is None else np.ascontiguousarray(np.asarray(r, np.float64).reshape(-1, 3))
+        aa = None if az is None else np.ascontiguousarray(np.asarray(az, np.float64).reshape(-1, 7))
+        return x, rr, aa
+
+    @staticmethod
+    def _ptr(a):
+        return None if a is None else a.ctypes.data_as(_P)
+
+    def data_resp(self, dtype, omega, origin, x, e1, e2, r=None, az=None):
+        """Responses (nsite, ncomp) of data type dtype (dataTypes.f90 numbering)."""
+        x, rr, aa = self._sites(x, r, az)
+        ns = x.shape[0]
+        resp = np.zeros(ns * 8)
+        nc = lib().sp2o_data_resp(self._h, int(dtype), float(omega),
+                                  np.ascontiguousarray(origin, np.float64), ns, x,
+                                  self._ptr(rr), self._ptr(aa),
+                                  np.ascontiguousarray(e1, np.complex128),
+                                  np.ascontiguousarray(e2, np.complex128), resp)
+        if nc < 0:
+            raise ValueError(f"unsupported data type {dtype}")
+        return resp[:ns * nc].reshape(ns, nc)
+
+    def lmultT(self, dtype, omega, origin, x, d, e1, e2, r=None, az=None, exist=None):
+        """Adjoint sources (comb1, comb2) on all edges for data d (nsite, ncomp)."""
+        x, rr, aa = self._sites(x, r, az)
+        ns = x.shape[0]
+        d = np.ascontiguousarray(d, np.float64)
+        ex = None if exist is None else np.ascontiguousarray(exist, np.uint8)
+        c1 = np.zeros(self.nE, np.complex128)
+        c2 = np.zeros(self.nE, np.complex128)
+        nc = lib().sp2o_lmultT(self._h, int(dtype), float(omega),
+                               np.ascontiguousarray(origin, np.float64), ns, x, self._ptr(rr),
+                               self._ptr(aa), d.ravel(), self._ptr(ex),
+                               np.ascontiguousarray(e1, np.complex128),
+                               np.ascontiguousarray(e2, np.complex128), c1, c2)
+        if nc < 0:
+            raise ValueError(f"unsupported data type {dtype}")
+        return c1, c2

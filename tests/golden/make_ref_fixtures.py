@@ -1,6 +1,8 @@
 """Writes tests/golden/ref_small.npz: what the reference's own CPU SP2 path
 (the flang build of the reference sources, oracle/ref/Makefile) computes on a
-small seeded grid.  Inputs and outputs only -- data, no reference code.
+small seeded grid, and tests/golden/ref_datafunc.npz: its dataResp and LmultT
+for every data type on the same grid.  Inputs and outputs only -- data, no
+reference code.
 
 Run where the reference is mounted:  python tests/golden/make_ref_fixtures.py
 """
@@ -52,5 +54,49 @@ def main():
     print("wrote", os.path.join(HERE, "ref_small.npz"))
 
 
+DF_NCOMP = {1: 8, 2: 4, 3: 4, 4: 8, 5: 4, 6: 4, 7: 2, 8: 2, 9: 2, 10: 2, 11: 2}
+
+
+def datafunc_case():
+    """Sites on the case() grid: one exactly on an x/y node (minNode ties go
+    through clean()), one at the Earth surface, one with non-orthogonal
+    channel azimuths; reference sites for the interstation type."""
+    rng, dx, dy, dz, nza, lc = case()
+    rng = np.random.default_rng(77)
+    origin = np.array([-dx.sum() / 2, -dy.sum() / 2, 0.0])
+    xe = origin[0] + np.concatenate([[0.0], np.cumsum(dx)])
+    ye = origin[1] + np.concatenate([[0.0], np.cumsum(dy)])
+    x = np.array([[xe[3], ye[3], 20.0],
+                  [0.5 * (xe[2] + xe[3]), 0.3 * ye[2] + 0.7 * ye[3], 0.0],
+                  [0.2 * xe[4] + 0.8 * xe[3], 0.6 * ye[4] + 0.4 * ye[3], 75.0],
+                  [0.9 * xe[2] + 0.1 * xe[3], 0.5 * (ye[2] + ye[3]), 130.0]])
+    r = x[[2, 3, 0, 1]].copy()
+    az = np.tile([0.0, 90.0, 0.0, 0.0, 90.0, 0.0, 90.0], (4, 1))
+    az[2] = [12.0, 97.0, 0.0, -8.0, 85.0, 5.0, 100.0]
+    az[3] = [30.0, 120.0, 0.0, 30.0, 120.0, 30.0, 120.0]
+    ne = 7 * 7 * 9 + 8 * 6 * 9 + 8 * 7 * 8
+    e1 = rng.standard_normal(ne) + 1j * rng.standard_normal(ne)
+    e2 = rng.standard_normal(ne) + 1j * rng.standard_normal(ne)
+    return dict(dx=dx, dy=dy, dz=dz, nzAir=nza, origin=origin, omega=2 * np.pi * 0.05,
+                x=x, r=r, az=az, e1=e1, e2=e2), rng
+
+
+def datafunc_main():
+    c, rng = datafunc_case()
+    grid = dict(dx=c["dx"], dy=c["dy"], dz=c["dz"], nzAir=c["nzAir"])
+    out = dict(c)
+    for dt, nc in DF_NCOMP.items():
+        d = rng.standard_normal((4, nc))
+        ex = np.ones((4, nc), np.int32)
+        ex[1, 0] = 0
+        resp, c1, c2 = refdriver.run_datafunc(grid, c["origin"], c["omega"], dt, c["x"], c["e1"],
+                                              c["e2"], d, r=c["r"], az=c["az"], exist=ex)
+        out.update({f"d_{dt}": d, f"exist_{dt}": ex, f"resp_{dt}": resp, f"comb1_{dt}": c1,
+                    f"comb2_{dt}": c2})
+    np.savez_compressed(os.path.join(HERE, "ref_datafunc.npz"), **out)
+    print("wrote", os.path.join(HERE, "ref_datafunc.npz"))
+
+
 if __name__ == "__main__":
     main()
+    datafunc_main()
