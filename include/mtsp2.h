@@ -214,6 +214,49 @@ int mt_host_gradient(int nx, int ny, int nzAir, int nzEarth, const double *dx, c
                      int mode);
 
 /* ---- tuning / introspection ------------------------------------------------ */
+/* ---- data functionals (DataFunc.f90, EMfieldInterp.f90, SENS/DataSens.f90) */
+/* Data types, numbered as dataTypes.f90:65-77; the number of real components
+ * per site (ncomp) is 8, 4, 4, 8, 4, 4 for types 1-6 and 2 for types 7-11. */
+#define MT_DT_FULL_IMPEDANCE 1
+#define MT_DT_OFF_DIAGONAL_IMPEDANCE 2
+#define MT_DT_FULL_VERTICAL_COMPONENTS 3
+#define MT_DT_FULL_INTERSTATION_TF 4
+#define MT_DT_OFF_DIAGONAL_RHO_PHASE 5
+#define MT_DT_PHASE_TENSOR 6
+#define MT_DT_EX_FIELD 7
+#define MT_DT_EY_FIELD 8
+#define MT_DT_BX_FIELD 9
+#define MT_DT_BY_FIELD 10
+#define MT_DT_BZ_FIELD 11
+/* Receivers (the reference's rxDict, 3D_MT/DICT/receivers.f90, plus the
+ * per-site orientations of dataBlock_t, SENS/DataSpace.f90:104): the grid
+ * (cells, spacings, origin (ox, oy, oz) as in GridDef.f90:301-399), site
+ * locations x (3*nsite), reference-site locations r (3*nsite, NULL = x) and
+ * channel azimuths (7*nsite degrees Hx,Hy,Hz,Ex,Ey,Hx_ref,Hy_ref, NULL = the
+ * defaults of fields_orientation.f90:38-58).  Builds the E and H
+ * interpolation rows of EinterpSetUp / BfromESetUp (EMfieldInterp.f90:141-647)
+ * on the host and keeps them on the device.  MT_E_ARG if a site's stencil
+ * leaves the grid. */
+int mt_receivers_set(mt_ctx *ctx, int nx, int ny, int nzAir, int nzEarth, const double *dx,
+                     const double *dy, const double *dz, const double *origin, int nsite,
+                     const double *x, const double *r, const double *azimuth);
+/* Responses of data type dtype at every receiver for nfreq transmitters
+ * (dataResp, DataFunc.f90:43-462): e_dev = device fields, all edges,
+ * [freq][polarisation 1,2][n_edge] complex; resp = host
+ * [freq][site][ncomp] (real and imaginary parts interleaved for complex
+ * types, as dataResp's Resp). */
+int mt_data_resp(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const void *e_dev,
+                 double *resp);
+/* Adjoint source of data d (LmultT, SENS/DataSens.f90:130-225, one dataBlock
+ * per transmitter): comb_dev (device, [freq][mode 1,2][n_edge] complex,
+ * overwritten) = sum over sites and functionals of z * L, with L the
+ * linearised functionals of Lrows (DataFunc.f90:465-869) at the fields e_dev
+ * and z = cmplx(d_re, -d_im) for complex types, cmplx(d, 0) for real ones.
+ * d, exist: host [freq][site][ncomp]; exist (NULL = all present) is read at
+ * each functional's first component. */
+int mt_lmultT(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const void *e_dev,
+              const double *d, const unsigned char *exist, void *comb_dev);
+
 /* Runs the SpMV autotuner for nsys systems: times the solver's SpMV kernel for
  * several row-tile group counts per system and keeps the fastest; *variant
  * receives the chosen group count. */

@@ -13,11 +13,13 @@
 #include <string>
 #include <vector>
 
+#include "mt_datafunc.hpp"
 #include "mt_dilu.hpp"
 #include "mt_divcor.hpp"
 #include "mt_host_ops.hpp"
 #include "mt_internal.h"
 #include "mt_kernels.hpp"
+#include "mt_sites.hpp"
 #include "mt_spmv.hpp"
 
 using namespace mt;
@@ -61,6 +63,21 @@ struct DivcorData {
   TilePlan tiles;
 };
 
+// receivers of the data functionals (mt_receivers_set)
+struct SiteData {
+  bool ready = false;
+  int nsite = 0, nu = 0;
+  int64_t nE = 0;
+  int *ptr = nullptr, *idx = nullptr, *uedge = nullptr, *gptr = nullptr, *grow = nullptr;
+  double *val = nullptr, *rot = nullptr, *gval = nullptr;
+};
+
+static void free_sites(SiteData& s) {
+  hipFree(s.ptr); hipFree(s.idx); hipFree(s.uedge); hipFree(s.gptr); hipFree(s.grow);
+  hipFree(s.val); hipFree(s.rot); hipFree(s.gval);
+  s = SiteData();
+}
+
 struct mt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -70,6 +87,7 @@ struct mt_ctx {
   TilePlan tiles;
   TilePlan tiles_rb;  // red-black tile ordering of the same lattices
   DivcorData dc;
+  SiteData sites;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int G_override = 0;
@@ -1047,6 +1065,7 @@ int mt_ctx_destroy(mt_ctx* ctx) {
   hipFree(ctx->tiles_rb.shape);
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
+  free_sites(ctx->sites);
   for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
   for (auto& e : ctx->tev) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
@@ -1732,3 +1751,123 @@ int mt_kernel_timing(const mt_ctx* ctx, int family, long long* launches, double*
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Data functionals: receivers, responses, adjoint sources
+// ---------------------------------------------------------------------------
+template <class T>
+static int dev_upload(mt_ctx* ctx, const std::vector<T>& v, T** out) {
+  MT_HIP_CHECK(ctx, hipMalloc(out, std::max<size_t>(v.size(), 1) * sizeof(T)));
+  if (!v.empty())
+    MT_HIP_CHECK(ctx, hipMemcpy(*out, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return MT_OK;
+}
+
+// real components per data type, dataTypes.f90:87-222
+static int dtype_ncomp(int dt) {
+  static const int nc[12] = {0, 8, 4, 4, 8, 4, 4, 2, 2, 2, 2, 2};
+  return (dt >= 1 && dt <= 11) ? nc[dt] : 0;
+}
+
+int mt_receivers_set(mt_ctx* ctx, int nx, int ny, int nzAir, int nzEarth, const double* dx,
+                     const double* dy, const double* dz, const double* origin, int nsite,
+                     const double* x, const double* r, const double* azimuth) {
+  if (!ctx || !dx || !dy || !dz || !x || nsite < 1 || nx < 2 || ny < 2 || nzAir < 0 || nzEarth < 1)
+    return mt_fail(ctx, MT_E_ARG, "mt_receivers_set: bad arguments");
+  Grid g;
+  g.set(nx, ny, nzAir + nzEarth);
+  const double org0[3] = {0.0, 0.0, 0.0};
+  SiteRows rows;
+  std::string err;
+  // without reference locations every site is its own reference
+  int rc = build_site_rows(g, dx, dy, dz, nzAir, origin ? origin : org0, nsite, x, r ? r : x,
+                           azimuth, rows, err);
+  if (rc) return mt_fail(ctx, rc, "mt_receivers_set: " + err);
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  free_sites(ctx->sites);
+  SiteData& S = ctx->sites;
+  if ((rc = dev_upload(ctx, rows.ptr, &S.ptr)) || (rc = dev_upload(ctx, rows.idx, &S.idx)) ||
+      (rc = dev_upload(ctx, rows.val, &S.val)) || (rc = dev_upload(ctx, rows.rot, &S.rot)) ||
+      (rc = dev_upload(ctx, rows.uedge, &S.uedge)) || (rc = dev_upload(ctx, rows.gptr, &S.gptr)) ||
+      (rc = dev_upload(ctx, rows.grow, &S.grow)) || (rc = dev_upload(ctx, rows.gval, &S.gval))) {
+    free_sites(ctx->sites);
+    return rc;
+  }
+  S.nsite = nsite;
+  S.nu = int(rows.uedge.size());
+  S.nE = g.nE;
+  S.ready = true;
+  return MT_OK;
+}
+
+static int site_launch(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const void* e_dev,
+                       double* resp_host, const double* d, const unsigned char* exist,
+                       void* comb_dev) {
+  const SiteData& S = ctx->sites;
+  const int ncomp = dtype_ncomp(dtype);
+  if (!ncomp) return mt_fail(ctx, MT_E_ARG, "data type must be 1..11 (dataTypes.f90 numbering)");
+  if (!S.ready) return mt_fail(ctx, MT_E_STATE, "no receivers: call mt_receivers_set first");
+  if (nfreq < 1 || !omegas || !e_dev) return mt_fail(ctx, MT_E_ARG, "bad frequency batch");
+  for (int f = 0; f < nfreq; ++f)
+    if (!(omegas[f] > 0.0)) return mt_fail(ctx, MT_E_ARG, "omegas must be positive");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const bool adj = comb_dev != nullptr;
+  const size_t nsd = size_t(nfreq) * S.nsite;
+  void *w_om, *w_out, *w_d = nullptr, *w_ex = nullptr;
+  int rc;
+  if ((rc = ws_get(ctx, "df_omega", nfreq * sizeof(double), &w_om))) return rc;
+  if ((rc = ws_get(ctx, "df_out", adj ? nsd * 14 * sizeof(Cd) : nsd * ncomp * sizeof(double), &w_out)))
+    return rc;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(w_om, omegas, nfreq * sizeof(double), hipMemcpyHostToDevice,
+                                   ctx->stream));
+  if (adj) {
+    if ((rc = ws_get(ctx, "df_d", nsd * ncomp * sizeof(double), &w_d))) return rc;
+    MT_HIP_CHECK(ctx, hipMemcpyAsync(w_d, d, nsd * ncomp * sizeof(double), hipMemcpyHostToDevice,
+                                     ctx->stream));
+    if (exist) {
+      if ((rc = ws_get(ctx, "df_exist", nsd * ncomp, &w_ex))) return rc;
+      MT_HIP_CHECK(ctx, hipMemcpyAsync(w_ex, exist, nsd * ncomp, hipMemcpyHostToDevice, ctx->stream));
+    }
+  }
+  SiteArgs a;
+  a.nsite = S.nsite; a.nfreq = nfreq; a.dtype = dtype; a.ncomp = ncomp; a.want_coef = adj;
+  a.omegas = (const double*)w_om;
+  a.e = (const Cd*)e_dev;
+  a.nE = S.nE;
+  a.ptr = S.ptr; a.idx = S.idx; a.val = S.val; a.rot = S.rot;
+  a.resp = adj ? nullptr : (double*)w_out;
+  a.d = (const double*)w_d;
+  a.exist = (const unsigned char*)w_ex;
+  a.coef = adj ? (Cd*)w_out : nullptr;
+  const int nt = nfreq * S.nsite;
+  hipLaunchKernelGGL(k_site_eval, dim3((nt + 63) / 64), dim3(64), 0, ctx->stream, a);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  if (adj) {
+    MT_HIP_CHECK(ctx, hipMemsetAsync(comb_dev, 0, size_t(nfreq) * 2 * S.nE * sizeof(Cd), ctx->stream));
+    const int64_t ng = int64_t(nfreq) * S.nu;
+    if (ng > 0) {
+      hipLaunchKernelGGL(k_lmultT_gather, dim3(unsigned((ng + 255) / 256)), dim3(256), 0, ctx->stream,
+                         nfreq, S.nu, S.nsite, S.nE, (const double*)w_om, S.uedge, S.gptr, S.grow,
+                         S.gval, (const Cd*)w_out, (Cd*)comb_dev);
+      MT_HIP_CHECK(ctx, hipGetLastError());
+    }
+  } else {
+    MT_HIP_CHECK(ctx, hipMemcpyAsync(resp_host, w_out, nsd * ncomp * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+  }
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_data_resp(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const void* e_dev,
+                 double* resp) {
+  if (!ctx || !resp) return mt_fail(ctx, MT_E_ARG, "mt_data_resp: null argument");
+  return site_launch(ctx, dtype, nfreq, omegas, e_dev, resp, nullptr, nullptr, nullptr);
+}
+
+int mt_lmultT(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const void* e_dev,
+              const double* d, const unsigned char* exist, void* comb_dev) {
+  if (!ctx || !d || !comb_dev) return mt_fail(ctx, MT_E_ARG, "mt_lmultT: null argument");
+  return site_launch(ctx, dtype, nfreq, omegas, e_dev, nullptr, d, exist, comb_dev);
+}

@@ -24,6 +24,11 @@ MT_E_BREAKDOWN, MT_E_MAXIT, MT_E_STATE, MT_E_STENCIL = 5, 6, 7, 8
 PC_DILU, PC_JACOBI, PC_NONE, PC_DILU_RB = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 KT_SPMV, KT_PRECOND = 0, 1
+# data types, dataTypes.f90:65-77 numbering
+(DT_FULL_IMPEDANCE, DT_OFF_DIAGONAL_IMPEDANCE, DT_FULL_VERTICAL_COMPONENTS,
+ DT_FULL_INTERSTATION_TF, DT_OFF_DIAGONAL_RHO_PHASE, DT_PHASE_TENSOR, DT_EX_FIELD, DT_EY_FIELD,
+ DT_BX_FIELD, DT_BY_FIELD, DT_BZ_FIELD) = range(1, 12)
+DT_NCOMP = {1: 8, 2: 4, 3: 4, 4: 8, 5: 4, 6: 4, 7: 2, 8: 2, 9: 2, 10: 2, 11: 2}
 
 
 class MTError(RuntimeError):
@@ -88,6 +93,9 @@ _SIGS = {
     "mt_set_spmv_variant": (_I, [_P, _I, _I]),
     "mt_bench_precond": (_I, [_P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
+    "mt_receivers_set": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _P, _I, _PD, _P, _P]),
+    "mt_data_resp": (_I, [_P, _I, _I, _PD, _P, _PD]),
+    "mt_lmultT": (_I, [_P, _I, _I, _PD, _P, _PD, _P, _P]),
 }
 
 
@@ -378,6 +386,41 @@ class Context:
         self._chk(lib().mt_bench_precond(self._h, int(nsys), int(precision), int(precond),
                                          int(reps), C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    # -- data functionals ----------------------------------------------------
+    def receivers_set(self, dx, dy, dz, nzAir, x, r=None, az=None, origin=(0.0, 0.0, 0.0)):
+        """Receivers at x (nsite, 3) with reference sites r and channel
+        azimuths az (nsite, 7; degrees); dz lists all layers, air first."""
+        dx, dy, dz = (np.ascontiguousarray(v, np.float64) for v in (dx, dy, dz))
+        x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, 3))
+        self._keep_r = None if r is None else np.ascontiguousarray(
+            np.asarray(r, np.float64).reshape(-1, 3))
+        self._keep_az = None if az is None else np.ascontiguousarray(
+            np.asarray(az, np.float64).reshape(-1, 7))
+        org = np.ascontiguousarray(origin, np.float64)
+        self._nsite = x.shape[0]
+        self._chk(lib().mt_receivers_set(
+            self._h, len(dx), len(dy), int(nzAir), len(dz) - int(nzAir), dx, dy, dz,
+            org.ctypes.data, self._nsite, x,
+            None if self._keep_r is None else self._keep_r.ctypes.data,
+            None if self._keep_az is None else self._keep_az.ctypes.data))
+
+    def data_resp(self, dtype, omegas, e_ptr, ncomp):
+        """Responses [freq][site][ncomp] from device fields e_ptr
+        ([freq][2][n_edge] complex)."""
+        om = np.ascontiguousarray(np.atleast_1d(omegas), np.float64)
+        out = np.zeros(len(om) * self._nsite * ncomp)
+        self._chk(lib().mt_data_resp(self._h, int(dtype), len(om), om, C.c_void_p(e_ptr), out))
+        return out.reshape(len(om), self._nsite, ncomp)
+
+    def lmultT(self, dtype, omegas, e_ptr, d, comb_ptr, exist=None):
+        """Adjoint sources into device comb_ptr ([freq][2][n_edge] complex) for
+        data d [freq][site][ncomp]."""
+        om = np.ascontiguousarray(np.atleast_1d(omegas), np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        ex = None if exist is None else np.ascontiguousarray(exist, np.uint8)
+        self._chk(lib().mt_lmultT(self._h, int(dtype), len(om), om, C.c_void_p(e_ptr), d.ravel(),
+                                  None if ex is None else ex.ctypes.data, C.c_void_p(comb_ptr)))
 
     def bench_spmv(self, nsys, precision=PREC_FP64, reps=20):
         ms, by = _D(), _D()
