@@ -34,8 +34,9 @@ module mtsp2_fortran
 
    public :: mt_default_opts, mt_ctx_create, mt_ctx_destroy, mt_device_count, mt_set_grid, &
              mt_operator_set_csr, mt_bicg, mt_solve_batch, mt_apply, mt_precond_apply, &
-             mt_divcor, mt_pmultT, mt_last_error_string
-   public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release
+             mt_divcor, mt_pmultT, mt_last_error_string, mt_receivers_set, mt_data_resp, &
+             mt_lmultT, mt_dev_alloc, mt_dev_free, mt_dev_copy
+   public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, mtsp2_data_resp
 
    interface
       subroutine mt_default_opts(opts) bind(C, name='mt_default_opts')
@@ -129,6 +130,45 @@ module mtsp2_fortran
          integer(c_int), value :: npol, acc
          complex(c_double_complex), intent(in) :: e0(*), lam(*)
          real(c_double), intent(inout) :: dm(*)
+      end function
+      integer(c_int) function mt_receivers_set(ctx, nx, ny, nzAir, nzEarth, dx, dy, dz, origin, &
+            nsite, x, r, az) bind(C, name='mt_receivers_set')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nx, ny, nzAir, nzEarth, nsite
+         real(c_double), intent(in) :: dx(*), dy(*), dz(*), origin(3), x(*), r(*), az(*)
+      end function
+      integer(c_int) function mt_data_resp(ctx, dtype, nfreq, omegas, e_dev, resp) &
+            bind(C, name='mt_data_resp')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx, e_dev
+         integer(c_int), value :: dtype, nfreq
+         real(c_double), intent(in) :: omegas(*)
+         real(c_double), intent(out) :: resp(*)
+      end function
+      integer(c_int) function mt_lmultT(ctx, dtype, nfreq, omegas, e_dev, d, exist, comb_dev) &
+            bind(C, name='mt_lmultT')
+         import :: c_int, c_ptr, c_double, c_signed_char
+         type(c_ptr), value :: ctx, e_dev, comb_dev
+         integer(c_int), value :: dtype, nfreq
+         real(c_double), intent(in) :: omegas(*), d(*)
+         integer(c_signed_char), intent(in) :: exist(*)
+      end function
+      integer(c_int) function mt_dev_alloc(ctx, nbytes, p) bind(C, name='mt_dev_alloc')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: ctx
+         integer(c_size_t), value :: nbytes
+         type(c_ptr), intent(out) :: p
+      end function
+      integer(c_int) function mt_dev_free(ctx, p) bind(C, name='mt_dev_free')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, p
+      end function
+      integer(c_int) function mt_dev_copy(ctx, dst, src, nbytes, kind) bind(C, name='mt_dev_copy')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: ctx, dst, src
+         integer(c_size_t), value :: nbytes
+         integer(c_int), value :: kind
       end function
       integer(c_size_t) function c_strlen(s) bind(C, name='strlen')
          import :: c_size_t, c_ptr
@@ -261,6 +301,98 @@ contains
             op_key(:, d) = 0
          endif
       enddo
+   end subroutine
+
+
+   ! Library context for device_idx, created on first use.
+   subroutine dev_ctx(device_idx, ctx, ierr)
+      integer, intent(in)       :: device_idx
+      type(c_ptr), intent(out)  :: ctx
+      integer, intent(out)      :: ierr
+      ierr = MT_OK
+      if (device_idx < 0 .or. device_idx >= MAXDEV) then
+         ierr = MT_E_DEVICE
+         return
+      endif
+      if (.not. c_associated(ctxs(device_idx))) &
+         ierr = mt_ctx_create(int(device_idx, c_int), ctxs(device_idx))
+      ctx = ctxs(device_idx)
+   end subroutine
+
+   ! Fields of both modes onto the device: e_dev = [e1 ; e2].
+   subroutine fields_up(ctx, e1, e2, e_dev, ierr)
+      type(c_ptr), intent(in)                       :: ctx
+      complex(c_double_complex), intent(in), target :: e1(:), e2(:)
+      type(c_ptr), intent(out)                      :: e_dev
+      integer, intent(out)                          :: ierr
+      integer(c_size_t) :: nb
+      nb = int(size(e1), c_size_t) * 16_c_size_t
+      ierr = mt_dev_alloc(ctx, 2_c_size_t * nb, e_dev)
+      if (ierr /= MT_OK) return
+      ierr = mt_dev_copy(ctx, e_dev, c_loc(e1), nb, 0_c_int)
+      if (ierr == MT_OK) ierr = mt_dev_copy(ctx, transfer(transfer(e_dev, 0_c_intptr_t) + &
+                                            int(nb, c_intptr_t), e_dev), c_loc(e2), nb, 0_c_int)
+   end subroutine
+
+   ! Drop-in for dataResp (3D_MT/DataFunc.f90:43-462) over all sites of one
+   ! data block: grid (cells, spacings, origin), receivers x / r (3, nsite),
+   ! azimuths az (7, nsite, degrees), data type dtype, the fields of both
+   ! modes on all edges; resp (ncomp, nsite) as dataResp's Resp per site.
+   subroutine mtsp2_data_resp(device_idx, nx, ny, nzAir, nzEarth, dx, dy, dz, origin, x, r, az, &
+                              dtype, omega, e1, e2, resp, ierr)
+      integer, intent(in)                    :: device_idx, nx, ny, nzAir, nzEarth, dtype
+      real(c_double), intent(in)             :: dx(:), dy(:), dz(:), origin(3), omega
+      real(c_double), intent(in)             :: x(:,:), r(:,:), az(:,:)
+      complex(c_double_complex), intent(in)  :: e1(:), e2(:)
+      real(c_double), intent(out)            :: resp(:,:)
+      integer, intent(out)                   :: ierr
+      type(c_ptr) :: ctx, e_dev
+      integer :: rc
+      call dev_ctx(device_idx, ctx, ierr)
+      if (ierr /= MT_OK) return
+      ierr = mt_receivers_set(ctx, int(nx, c_int), int(ny, c_int), int(nzAir, c_int), &
+                              int(nzEarth, c_int), dx, dy, dz, origin, int(size(x, 2), c_int), &
+                              x, r, az)
+      if (ierr /= MT_OK) return
+      call fields_up(ctx, e1, e2, e_dev, ierr)
+      if (ierr == MT_OK) ierr = mt_data_resp(ctx, int(dtype, c_int), 1_c_int, [omega], e_dev, resp)
+      rc = mt_dev_free(ctx, e_dev)
+   end subroutine
+
+   ! Drop-in for LmultT (SENS/DataSens.f90:130-225) for one data block of one
+   ! transmitter: the adjoint sources comb1, comb2 (all edges, modes 1 and 2)
+   ! of data d (ncomp, nsite) with presence flags exist (ncomp, nsite), at the
+   ! background fields e1, e2 of the two modes.
+   subroutine mtsp2_lmultT(device_idx, nx, ny, nzAir, nzEarth, dx, dy, dz, origin, x, r, az, &
+                           dtype, omega, e1, e2, d, exist, comb1, comb2, ierr)
+      integer, intent(in)                            :: device_idx, nx, ny, nzAir, nzEarth, dtype
+      real(c_double), intent(in)                     :: dx(:), dy(:), dz(:), origin(3), omega
+      real(c_double), intent(in)                     :: x(:,:), r(:,:), az(:,:), d(:,:)
+      logical, intent(in)                            :: exist(:,:)
+      complex(c_double_complex), intent(in)          :: e1(:), e2(:)
+      complex(c_double_complex), intent(out), target :: comb1(:), comb2(:)
+      integer, intent(out)                           :: ierr
+      type(c_ptr) :: ctx, e_dev, c_dev
+      integer(c_signed_char), allocatable :: ex(:,:)
+      integer(c_size_t) :: nb
+      integer :: rc
+      call dev_ctx(device_idx, ctx, ierr)
+      if (ierr /= MT_OK) return
+      ierr = mt_receivers_set(ctx, int(nx, c_int), int(ny, c_int), int(nzAir, c_int), &
+                              int(nzEarth, c_int), dx, dy, dz, origin, int(size(x, 2), c_int), &
+                              x, r, az)
+      if (ierr /= MT_OK) return
+      allocate(ex(size(exist, 1), size(exist, 2)))
+      ex = merge(1_c_signed_char, 0_c_signed_char, exist)
+      nb = int(size(e1), c_size_t) * 16_c_size_t
+      call fields_up(ctx, e1, e2, e_dev, ierr)
+      if (ierr == MT_OK) ierr = mt_dev_alloc(ctx, 2_c_size_t * nb, c_dev)
+      if (ierr == MT_OK) ierr = mt_lmultT(ctx, int(dtype, c_int), 1_c_int, [omega], e_dev, d, ex, c_dev)
+      if (ierr == MT_OK) ierr = mt_dev_copy(ctx, c_loc(comb1), c_dev, nb, 1_c_int)
+      if (ierr == MT_OK) ierr = mt_dev_copy(ctx, c_loc(comb2), transfer(transfer(c_dev, &
+                                            0_c_intptr_t) + int(nb, c_intptr_t), c_dev), nb, 1_c_int)
+      rc = mt_dev_free(ctx, e_dev)
+      rc = mt_dev_free(ctx, c_dev)
    end subroutine
 
 end module mtsp2_fortran

@@ -17,6 +17,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_BIN = os.path.join(HERE, "_ref", "modem_ref")
 DF_BIN = os.path.join(HERE, "_ref", "ref_datafunc")
+DF_DROPIN_BIN = os.path.join(HERE, "_ref", "ref_dropin_datafunc")
 
 
 def available() -> bool:
@@ -149,11 +150,13 @@ def datafunc_available() -> bool:
 
 
 def run_datafunc(grid, origin, omega, dtype, x, e1, e2, d, r=None, az=None, exist=None,
-                 timeout=120):
+                 timeout=120, binary=DF_BIN):
     """Runs the reference's dataResp (every site) and LmultT (all sites) for
     one transmitter and one data type (driver oracle/ref/ref_datafunc.f90).
     x, r: (nsite, 3); az: (nsite, 7) degrees; d, exist: (nsite, ncomp).
-    Returns (resp (nsite, ncomp), comb1, comb2)."""
+    Returns (resp (nsite, ncomp), comb1, comb2).  With binary=DF_DROPIN_BIN
+    (oracle/ref/ref_dropin_datafunc.f90) it also returns the library's
+    (resp, comb1, comb2, status) from the same Fortran host."""
     dx = np.asarray(grid["dx"], np.float64)
     dy = np.asarray(grid["dy"], np.float64)
     dz = np.asarray(grid["dz"], np.float64)
@@ -180,10 +183,16 @@ def run_datafunc(grid, origin, omega, dtype, x, e1, e2, d, r=None, az=None, exis
         d.tofile(f); ex.tofile(f)
         np.asarray(e1, np.complex128).tofile(f)
         np.asarray(e2, np.complex128).tofile(f)
-    subprocess.run([DF_BIN, inp, wd], check=True, timeout=timeout,
+    subprocess.run([binary, inp, wd], check=True, timeout=timeout,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
     resp = np.fromfile(os.path.join(wd, "resp.bin"), np.float64).reshape(ns, nc)
     comb = np.fromfile(os.path.join(wd, "comb.bin"), np.complex128)
     ne = len(comb) // 2
+    out = (resp, comb[:ne], comb[ne:])
+    if binary == DF_DROPIN_BIN:
+        lresp = np.fromfile(os.path.join(wd, "resp_lib.bin"), np.float64).reshape(ns, nc)
+        lcomb = np.fromfile(os.path.join(wd, "comb_lib.bin"), np.complex128)
+        st = np.fromfile(os.path.join(wd, "ierr.bin"), np.int32)
+        out = out + (lresp, lcomb[:ne], lcomb[ne:], st)
     _cleanup(wd, True)
-    return resp, comb[:ne], comb[ne:]
+    return out

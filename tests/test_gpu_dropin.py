@@ -7,6 +7,8 @@
 2. oracle/_ref/ref_dropin (where it was built): the reference's own SP2 host
    code (ModelDataInit/UpdateCond/UpdateFreq) with `cuBiCG` replaced by the
    libmtsp2 call, next to the reference CPU BICG on the same system.
+3. oracle/_ref/ref_dropin_datafunc (where it was built): the reference's
+   dataResp/LmultT next to mtsp2_data_resp/mtsp2_lmultT, one Fortran host.
 """
 import os
 import subprocess
@@ -15,6 +17,7 @@ import tempfile
 import numpy as np
 import pytest
 
+import refdriver
 import sp2oracle
 from conftest import ROOT, rel, small_problem
 from mtsp2 import problem as P
@@ -101,3 +104,36 @@ def test_reference_host_with_library_solver():
             o.update_freq(w)
             assert np.linalg.norm(b - o.mult_aii(xg)) / np.linalg.norm(b) <= 1e-9
             assert rel(xg, xc) <= 1e-6
+
+
+@pytest.mark.skipif(not os.path.isfile(refdriver.DF_DROPIN_BIN),
+                    reason="reference data-functional drop-in not built here")
+def test_reference_host_with_library_lmultT():
+    """The reference's dataResp/LmultT and the library's, called from one
+    Fortran host (mtsp2_data_resp, mtsp2_lmultT) on the same inputs: every
+    data type to 1e-10 relative."""
+    print("case reference dropin LmultT", flush=True)
+    dx, dy, dz, nza, lc = small_problem(4)
+    o = sp2oracle.Problem(dx, dy, dz, nza, lc)
+    rng = np.random.default_rng(9)
+    ns = 5
+    x = np.column_stack([rng.uniform(-0.3, 0.3, ns) * dx.sum(),
+                         rng.uniform(-0.3, 0.3, ns) * dy.sum(), rng.uniform(0, 60, ns)])
+    r = x[::-1].copy()
+    az = np.tile([0.0, 90.0, 0.0, 0.0, 90.0, 0.0, 90.0], (ns, 1))
+    az[2] = [10.0, 95.0, 0.0, 5.0, 100.0, -3.0, 80.0]
+    origin = np.array([-dx.sum() / 2, -dy.sum() / 2, 0.0])
+    e1 = rng.standard_normal(o.nE) + 1j * rng.standard_normal(o.nE)
+    e2 = rng.standard_normal(o.nE) + 1j * rng.standard_normal(o.nE)
+    grid = dict(dx=dx, dy=dy, dz=dz, nzAir=nza)
+    for dt, nc in {1: 8, 2: 4, 3: 4, 4: 8, 5: 4, 6: 4, 7: 2, 11: 2}.items():
+        d = rng.standard_normal((ns, nc))
+        ex = np.ones((ns, nc), np.int32)
+        ex[1, 0] = 0
+        rr, c1, c2, lr, l1, l2, st = refdriver.run_datafunc(
+            grid, origin, 2 * np.pi * 0.2, dt, x, e1, e2, d, r=r, az=az, exist=ex,
+            binary=refdriver.DF_DROPIN_BIN)
+        assert np.all(st == 0), (dt, st)
+        assert np.max(np.abs(lr - rr)) <= 1e-10 * np.max(np.abs(rr)), dt
+        for a, b in ((l1, c1), (l2, c2)):
+            assert np.max(np.abs(a - b)) <= 1e-10 * max(np.max(np.abs(b)), 1e-300), dt
