@@ -217,7 +217,7 @@ __global__ __launch_bounds__(64) void k_site_eval(SiteArgs a) {
       }
     }
   }
-  // rotated base rows (Lrows, DataFunc.f90:621-643)
+  // rotated base rows (Lrows, DataFunc.f90:626-656)
   const double* Re = rot + 4;
   const double* Rh = rot + 12;
   const double* Rr = rot + 20;
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(64) void k_site_eval(SiteArgs a) {
         L[n] = v7_lin(b, row, {-b.x, -b.y}, L1);
       }
       if (dt == 5) {
-        // log10 rho and phase of Zxy, Zyx (Lrows, DataFunc.f90:716-745)
+        // log10 rho and phase of Zxy, Zyx (Lrows, DataFunc.f90:717-741)
         const double a2 = cabs2(Zl[1]), a3 = cabs2(Zl[2]), ln10 = log(10.0);
         const V7 Lxy = L[0], Lyx = L[1];
         L[3] = v7_lin(cscl(1.0 / a3, cmul({0.0, 1.0}, cconj(Zl[2]))), Lyx, zero, Lyx);
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(64) void k_site_eval(SiteArgs a) {
         L[1] = v7_lin(cscl(1.0 / a2, cmul({0.0, 1.0}, cconj(Zl[1]))), Lxy, zero, Lxy);
         L[0] = v7_lin(cscl(2.0 / a2 / ln10, cconj(Zl[1])), Lxy, zero, Lxy);
       } else if (dt == 6) {
-        // phase tensor (Lrows, DataFunc.f90:747-852)
+        // phase tensor (Lrows, DataFunc.f90:743-840)
         const double r1 = Zl[0].x, r2 = Zl[1].x, r3 = Zl[2].x, r4 = Zl[3].x;
         const double m1 = Zl[0].y, m2 = Zl[1].y, m3 = Zl[2].y, m4 = Zl[3].y;
         const double det = r1 * r4 - r2 * r3, sg = kIsignD;

@@ -25,7 +25,7 @@ struct SiteRows {
   // per site 3 channel frames (E, H at the site, H at the reference site),
   // each two 2x2 matrices: the rotation the responses use
   // (rotate_Model2Data, DataFunc.f90:931-957) and the one the linearised
-  // functionals use (Lrows, DataFunc.f90:621-643), row-major
+  // functionals use (Lrows, DataFunc.f90:626-656), row-major
   std::vector<double> rot;   // [nsite][3][2][4]
   // transpose of the rows for the deterministic adjoint gather: distinct
   // edges uedge[u], with items gptr[u] .. gptr[u+1]-1 = (row, coefficient)
