@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the isolated SpMV launches (FP32 x 16 systems, FP64 x 1) and a "
                          "1 GiB device copy, for hardware-counter runs")
+    ap.add_argument("--dilu-variant", type=int, default=4,
+                    help="red-black DILU launch shape: 1 single-system persistent, 4 systems per workgroup")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
     args = ap.parse_args()
@@ -108,6 +110,7 @@ def main():
     lc = P.config_model(args.config, grid, args.seed)
     fhz = P.config_freqs(args.config)
     ctx = mtsp2.Context(0)
+    ctx.set_dilu_variant(args.dilu_variant)
     t0 = time.time()
     ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
     log(f"assembled n_int={ctx.n_int} nnz={ctx.nnz} in {time.time()-t0:.1f}s")

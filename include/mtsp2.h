@@ -275,6 +275,11 @@ int mt_bench_precond(mt_ctx *ctx, int nsys, int precision, int precond, int reps
  * layers_per_segment (> 0) sets the k extent of one k-marching block, 0
  * keeps it.  Has no counterpart in the reference (cuSPARSE's SpMV is fixed
  * there). */
+/* Launch shape of the red-black DILU (MT_PC_DILU_RB) sweeps: 1 = one system
+ * per single-wave persistent workgroup with the next item prefetched into
+ * registers; 4 (default) = four systems of one tile per workgroup sharing the
+ * tile's couplings in LDS.  Same arithmetic. */
+int mt_set_dilu_variant(mt_ctx *ctx, int variant);
 int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
 /* Fixes the row-tile group count per system (0: automatic). */
 int mt_set_tuning(mt_ctx *ctx, int groups_per_system);

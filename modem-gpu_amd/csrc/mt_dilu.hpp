@@ -50,6 +50,9 @@ struct LatticeInfo {
   int oi, oj, ok;     // offset of the interior lattice in its full lattice
   int a, b;           // full-lattice dims (i, j) for addressing
   int64_t base;       // address of the full lattice's first point in the vector
+  // tiles of a tile plan over this lattice: ntx x nty x ntz, numbered
+  // tbase + ti + ntx*(tj + nty*tk) across all lattices of the plan
+  int ntx, nty, tbase;
 };
 
 template <class T, class V>
@@ -68,6 +71,14 @@ struct DiluArgs {
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed
+  // red-black sweeps: the i-faces (li = 0 and li = 7) of every tile, packed
+  // [half][system][tile][face][64], entry lj + 8*lk.  Half 0 holds dinv*y of
+  // the red tiles (written by the red forward sweep, read by the black one),
+  // half 1 holds z of the black tiles (written by the black sweep, read by
+  // the red backward one).  Packed, a neighbour face is one coalesced read
+  // instead of 64 reads of 8 bytes, one per cache line.
+  C2<T>* xf = nullptr;
+  int64_t xfsys = 0, xfhalf = 0;
 };
 
 __device__ __forceinline__ int64_t lat_e(const LatticeInfo& L, int i, int j, int k) {
@@ -357,9 +368,16 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         const bool ok = ti < ex && tj < ey && tk < ez && ni >= 0 && nj >= 0 && nk >= 0 &&
                         ni < L.A && nj < L.B && nk < L.C;
         if (ok) {
-          const int64_t l = lat_e(L, ni, nj, nk);
-          fv[f] = pout[l];
-          if (MODE == RB_BLACK_FB) fd[f] = dinv[l];
+          if (dir == 0) {
+            // the neighbour tile's facing i-face, packed (entry tj + 8*tk = lane)
+            const int tid = L.tbase + (tl.y + (hi ? 1 : -1)) + L.ntx * (tl.z + L.nty * tl.w);
+            fv[f] = a.xf[(MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys +
+                         (int64_t)tid * 128 + (hi ? 0 : 64) + lane];
+          } else {
+            const int64_t l = lat_e(L, ni, nj, nk);
+            fv[f] = pout[l];
+            if (MODE == RB_BLACK_FB) fd[f] = dinv[l];
+          }
         }
       }
     }
@@ -386,7 +404,8 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         if (dir == 0) q = (hi ? 9 : 0) + BX * ((fa + 1) + BX * (fb + 1));
         else if (dir == 1) q = (fa + 1) + BX * ((hi ? 9 : 0) + BX * (fb + 1));
         else q = (fa + 1) + BX * ((fb + 1) + BX * (hi ? 9 : 0));
-        W[q] = MODE == RB_BLACK_FB ? cmul(fd[f], fv[f]) : fv[f];
+        // black: red neighbours enter as dinv*y (the packed i-faces already are)
+        W[q] = (MODE == RB_BLACK_FB && dir != 0) ? cmul(fd[f], fv[f]) : fv[f];
       }
     }
 #pragma unroll
@@ -524,8 +543,12 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
         wave_sync();
       }
     }
-    // ---- write the tile back (red forward: y; otherwise z)
+    // ---- write the tile back (red forward: y; otherwise z), and the packed
+    // i-faces the other colour reads next (red forward: dinv*y; black: z)
     C2<T>* __restrict__ dst = a.out + o;
+    C2<T>* __restrict__ xfo =
+        a.xf + (MODE == RB_RED_FWD ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys +
+        (int64_t)(L.tbase + tl.y + L.ntx * (tl.z + L.nty * tl.w)) * 128;
     for (int p = lane; p < NP; p += 64) {
       const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
       if (!(li < ex && lj < ey && lk < ez)) continue;
@@ -534,9 +557,296 @@ __global__ __launch_bounds__(64) void k_dilu_rb2(DiluArgs<T, V> a, const TileSha
       if constexpr (MODE == RB_RED_FWD) zv = PD[p].rb;
       else zv = W[q];
       dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = zv;
+      if (MODE != RB_RED_BWD && (li == 0 || li == TX - 1)) {
+        C2<T> fvv = zv;
+        if constexpr (MODE == RB_RED_FWD) fvv = cmul(PD[p].dv, zv);
+        xfo[(li == 0 ? 0 : 64) + lj + 8 * lk] = fvv;
+      }
     }
     wave_sync();
     w = wn;
+  }
+}
+
+// ------------------------------------ red-black sweeps, systems per workgroup
+// The same three launches with NW systems of one tile per workgroup, one wave
+// per system.  The tile's couplings are loaded once, raw, into LDS shared by
+// the NW waves (the sweeps mask them by colour and position); each wave then
+// stages its own system's right-hand side, pivots and halo and sweeps the
+// tile alone, with wave-level synchronisation only.  Per-system LDS is the
+// box and the packed point data (16 KB in FP32), and nothing is held in
+// registers across items, so two workgroups share a CU and one's loads run
+// under the other's sweeps.  Work items are (tile, group of NW active
+// systems), groups fastest, cut into 8 contiguous per-XCD ranges, so the
+// groups of a tile run on one XCD and share its pivots and couplings in L2.
+template <class T, class V, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void k_dilu_rbm(DiluArgs<T, V> a, const TileShape* shp) {
+  constexpr int TX = 8, NP = 512, NU = 8, NLEV = 22, BX = 10, NB = 1000;
+  constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
+  constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
+  constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
+  struct alignas(16) PtA { C2<T> rb, dv; };
+  struct Cpl { V c[3]; };
+  __shared__ C2<T> Wall[NW][NB];                 // per system: the box
+  __shared__ PtA PDall[NW][FWDP ? NP : 1];       // per system: rhs -> y, pivot
+  __shared__ C2<T> DVall[NW][FWDP ? 1 : NP];     // per system (red backward): pivot
+  __shared__ Cpl CL[NP];                         // raw lower couplings (slots 1, 3, 5)
+  __shared__ Cpl CH[BWDP ? NP : 1];              // raw upper couplings (slots 2, 4, 6)
+  __shared__ int s_sys[64], s_ok[64], s_fidx[64];
+  __shared__ C2<T> s_al[64];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int ngrp = (a.nact + NW - 1) / NW;
+  const int nwork = a.ntiles * ngrp;
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3;
+  const int chunk = (nwork + 7) / 8;
+  const int w = xcd * chunk + lb;
+  if (lb >= chunk || w >= nwork) return;   // uniform over the workgroup
+  if (tid < a.nact) {
+    const int sy = a.act[tid];
+    const SysScal& sc = a.sc[sy];
+    s_sys[tid] = sy;
+    s_ok[tid] = sc.status == a.status_req;
+    s_fidx[tid] = sc.fidx;
+    s_al[tid] = ccast<T>(sc.alpha);
+  }
+  const int tix = w / ngrp, grp = w - tix * ngrp;
+  const int4 tl = a.tiles[tix];
+  const LatticeInfo L = a.lat[tl.x];
+  const int gi0 = tl.y * TX, gj0 = tl.z * TX, gk0 = tl.w * TX;
+  const int ex = min(TX, L.A - gi0), ey = min(TX, L.B - gj0), ez = min(TX, L.C - gk0);
+  const V* __restrict__ cfs = a.coef;
+  const int64_t cst = a.cstride;
+  // ---- the tile's couplings, once for the NW systems
+  for (int p = tid; p < NP; p += 64 * NW) {
+    const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+    Cpl c, h;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) c.c[s] = h.c[s] = V(0);
+    if (li < ex && lj < ey && lk < ez) {
+      const int64_t e = lat_e(L, gi0 + li, gj0 + lj, gk0 + lk);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        c.c[s] = cfs[(int64_t)(1 + 2 * s) * cst + e];
+        if (BWDP) h.c[s] = cfs[(int64_t)(2 + 2 * s) * cst + e];
+      }
+    }
+    CL[p] = c;
+    if (BWDP) CH[p] = h;
+  }
+  __syncthreads();   // s_ tables
+  const int si = grp * NW + wv;
+  const bool mine = si < a.nact && s_ok[si];
+  const int sys = mine ? s_sys[si] : 0;
+  C2<T>* __restrict__ W = Wall[wv];
+  PtA* __restrict__ PD = PDall[wv];
+  C2<T>* __restrict__ DV = DVall[wv];
+  const int64_t o = (int64_t)sys * a.vstride;
+  if (mine) {
+    for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
+    const C2<T>* __restrict__ pin = a.in + o;
+    const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
+    const C2<T>* __restrict__ pout = a.out + o;
+    const C2<T>* __restrict__ dinv = a.dinv + (int64_t)s_fidx[si] * cst;
+    const C2<T> al = s_al[si];
+    C2<T> rv[NU], v2[NU], dd[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      rv[u] = v2[u] = dd[u] = {T(0), T(0)};
+      if (li < ex && lj < ey && lk < ez) {
+        const int64_t e = lat_e(L, gi0 + li, gj0 + lj, gk0 + lk);
+        rv[u] = FWDP ? pin[e] : pout[e];
+        if (FWDP && pin2) v2[u] = pin2[e];
+        dd[u] = dinv[e];
+      }
+    }
+    C2<T> fv[6], fd[6];
+    if (HALO) {
+      const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int ti, tj, tk;   // tile point next to the face
+        if (dir == 0) { ti = hi ? 7 : 0; tj = fa; tk = fb; }
+        else if (dir == 1) { ti = fa; tj = hi ? 7 : 0; tk = fb; }
+        else { ti = fa; tj = fb; tk = hi ? 7 : 0; }
+        const int ni = gi0 + ti + (dir == 0 ? (hi ? 1 : -1) : 0);
+        const int nj = gj0 + tj + (dir == 1 ? (hi ? 1 : -1) : 0);
+        const int nk = gk0 + tk + (dir == 2 ? (hi ? 1 : -1) : 0);
+        fv[f] = fd[f] = {T(0), T(0)};
+        const bool ok = ti < ex && tj < ey && tk < ez && ni >= 0 && nj >= 0 && nk >= 0 &&
+                        ni < L.A && nj < L.B && nk < L.C;
+        if (ok) {
+          if (dir == 0) {
+            // the neighbour tile's facing i-face, packed (entry tj + 8*tk = lane)
+            const int tn = L.tbase + (tl.y + (hi ? 1 : -1)) + L.ntx * (tl.z + L.nty * tl.w);
+            fv[f] = a.xf[(MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys +
+                         (int64_t)tn * 128 + (hi ? 0 : 64) + lane];
+          } else {
+            const int64_t l = lat_e(L, ni, nj, nk);
+            fv[f] = pout[l];
+            if (MODE == RB_BLACK_FB) fd[f] = dinv[l];
+          }
+        }
+      }
+    }
+    // ---- stage into LDS
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+      C2<T> r = rv[u];
+      if (FWDP && pin2) r = csub(r, cmul(al, v2[u]));
+      if (FWDP) PD[p] = PtA{r, dd[u]};
+      else { DV[p] = dd[u]; W[q] = r; }
+    }
+    if (HALO) {
+      const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int q;
+        if (dir == 0) q = (hi ? 9 : 0) + BX * ((fa + 1) + BX * (fb + 1));
+        else if (dir == 1) q = (fa + 1) + BX * ((hi ? 9 : 0) + BX * (fb + 1));
+        else q = (fa + 1) + BX * ((fb + 1) + BX * (hi ? 9 : 0));
+        // black: red neighbours enter as dinv*y (the packed i-faces already are)
+        W[q] = (MODE == RB_BLACK_FB && dir != 0) ? cmul(fd[f], fv[f]) : fv[f];
+      }
+    }
+  }
+  __syncthreads();   // the shared couplings; the rest is per wave
+  if (!mine) return;
+  // fold the face terms the sweeps do not reach: black forward, the high
+  // faces (red neighbours above); red backward, the low faces (black below)
+  if (HALO) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      if (!(li < ex && lj < ey && lk < ez)) continue;
+      const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+      const int st[3] = {1, BX, BX * BX};
+      if (MODE == RB_BLACK_FB) {
+        const bool fl[3] = {li == TX - 1, lj == TX - 1, lk == TX - 1};
+        const Cpl h = CH[p];
+        C2<T> r = PD[p].rb;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          if (!fl[s]) continue;
+          const T c = T(h.c[s]);
+          const C2<T> v = W[q + st[s]];
+          r.x -= c * v.x;
+          r.y -= c * v.y;
+        }
+        PD[p].rb = r;
+      } else {
+        const bool fl[3] = {li == 0, lj == 0, lk == 0};
+        const Cpl c4 = CL[p];
+        C2<T> y = W[q];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          if (!fl[s]) continue;
+          const T c = T(c4.c[s]);
+          const C2<T> v = W[q - st[s]];
+          y.x -= c * v.x;
+          y.y -= c * v.y;
+        }
+        W[q] = y;
+      }
+    }
+    wave_sync();
+  }
+  // this lane's point on every hyperplane of the tile (at most 48 per plane)
+  unsigned int pk[NLEV];
+#pragma unroll
+  for (int h = 0; h < NLEV; ++h) {
+    const int b = shp->lvl[h], e = shp->lvl[h + 1];
+    pk[h] = (b + lane < e) ? shp->pts[b + lane] : 0xffffffffu;
+  }
+  if (FWDP) {
+#pragma unroll
+    for (int h = 0; h < NLEV; ++h) {
+      const unsigned int pt = pk[h];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+      if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
+        const int p = li + TX * (lj + TX * lk);
+        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        const PtA pa = PD[p];
+        const Cpl c4 = CL[p];
+        // red: face neighbours are upper, so only in-tile lower couplings;
+        // black: every lower-side coupling (in tile or across the face)
+        const bool in[3] = {li > 0, lj > 0, lk > 0};
+        C2<T> y = pa.rb;
+        const int qn[3] = {q - 1, q - BX, q - BX * BX};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T c = (MODE == RB_BLACK_FB || in[s]) ? T(c4.c[s]) : T(0);
+          const C2<T> v = W[qn[s]];
+          y.x -= c * v.x;
+          y.y -= c * v.y;
+        }
+        PD[p].rb = y;
+        W[q] = cmul(pa.dv, y);
+      }
+      wave_sync();
+    }
+  }
+  if (BWDP) {
+#pragma unroll
+    for (int h = NLEV - 1; h >= 0; --h) {
+      const unsigned int pt = pk[h];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+      if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
+        const int p = li + TX * (lj + TX * lk);
+        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        C2<T> y, dvp;
+        if constexpr (FWDP) {
+          const PtA pa = PD[p];
+          y = pa.rb;
+          dvp = pa.dv;
+        } else {
+          y = W[q];
+          dvp = DV[p];
+        }
+        const Cpl h4 = CH[p];
+        // black: upper neighbours in the tile only; red: in tile or across
+        const bool up[3] = {li + 1 < ex, lj + 1 < ey, lk + 1 < ez};
+        const int qn[3] = {q + 1, q + BX, q + BX * BX};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T c = (MODE == RB_RED_BWD || up[s]) ? T(h4.c[s]) : T(0);
+          const C2<T> z = W[qn[s]];
+          y.x -= c * z.x;
+          y.y -= c * z.y;
+        }
+        W[q] = cmul(dvp, y);
+      }
+      wave_sync();
+    }
+  }
+  // ---- write the tile back (red forward: y; otherwise z), and the packed
+  // i-faces the other colour reads next (red forward: dinv*y; black: z)
+  C2<T>* __restrict__ dst = a.out + o;
+  C2<T>* __restrict__ xfo =
+      a.xf + (MODE == RB_RED_FWD ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys +
+      (int64_t)(L.tbase + tl.y + L.ntx * (tl.z + L.nty * tl.w)) * 128;
+  for (int p = lane; p < NP; p += 64) {
+    const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+    if (!(li < ex && lj < ey && lk < ez)) continue;
+    const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+    C2<T> zv;
+    if constexpr (MODE == RB_RED_FWD) zv = PD[p].rb;
+    else zv = W[q];
+    dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = zv;
+    if (MODE != RB_RED_BWD && (li == 0 || li == TX - 1)) {
+      C2<T> fvv = zv;
+      if constexpr (MODE == RB_RED_FWD) fvv = cmul(PD[p].dv, zv);
+      xfo[(li == 0 ? 0 : 64) + lj + 8 * lk] = fvv;
+    }
   }
 }
 

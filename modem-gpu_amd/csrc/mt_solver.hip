@@ -46,6 +46,7 @@ struct TilePlan {
   int tx = 8, ty = 8, tz = 8;
   int nlev = 0;
   int nlat = 0;
+  int ntall = 0;              // tiles over all lattices (dense tile ids 0..ntall-1)
   LatticeInfo lat[3];
   std::vector<int> lvl_off;   // host: start of each tile level in tiles[]
   int4* tiles = nullptr;      // device
@@ -95,6 +96,8 @@ struct mt_ctx {
   int spmv_variant = 1;         // 0 row-parallel SpMV; 1 k-marching for FP32
                                 // solves, row-parallel for FP64; 2 k-marching
   int km_ks = 16;               // k-marching SpMV: layers per k segment
+  int rb_variant = 4;           // red-black DILU: 1 persistent single-system
+                                // workgroups; 4 four systems per workgroup
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
   // SpMV launch and every DILU sweep of the BiCGStab engine
   bool timing = false;
@@ -194,10 +197,15 @@ static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInf
   tp.shape = nullptr;
   tp.nlat = nlat;
   std::vector<std::vector<int4>> byLev;
+  int tbase = 0;
   for (int c = 0; c < nlat; ++c) {
     tp.lat[c] = lat[c];
     const int NTx = (lat[c].A + tp.tx - 1) / tp.tx, NTy = (lat[c].B + tp.ty - 1) / tp.ty,
               NTz = (lat[c].C + tp.tz - 1) / tp.tz;
+    tp.lat[c].ntx = NTx;
+    tp.lat[c].nty = NTy;
+    tp.lat[c].tbase = tbase;
+    tbase += NTx * NTy * NTz;
     for (int tk = 0; tk < NTz; ++tk)
       for (int tj = 0; tj < NTy; ++tj)
         for (int ti = 0; ti < NTx; ++ti) {
@@ -206,6 +214,7 @@ static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInf
           byLev[L].push_back(make_int4(c, ti, tj, tk));
         }
   }
+  tp.ntall = tbase;
   tp.nlev = (int)byLev.size();
   tp.lvl_off.assign(tp.nlev + 1, 0);
   std::vector<int4> flat;
@@ -434,6 +443,10 @@ static int rb_grid(mt_ctx* ctx, int nwork) {
 
 template <class T, class V>
 static void dilu_rb_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
+  if (!a.xf) {  // every caller provides the packed-face buffer (rb_faces)
+    ctx->err = "red-black DILU launched without its face buffer";
+    return;
+  }
   const int nl = tp.nlev;  // 2: red, black (a lattice of one tile has no black level)
   for (int step = 0; step < 3; ++step) {
     const int L = (step == 1) ? 1 : 0;
@@ -441,11 +454,30 @@ static void dilu_rb_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
     if (a.ntiles == 0) continue;
+    if (ctx->rb_variant == 4) {
+      constexpr int NW = 4;
+      const int64_t nwork = int64_t(a.ntiles) * ((a.nact + NW - 1) / NW);
+      const int g = int(8 * ((nwork + 7) / 8));
+      if (step == 0) k_dilu_rbm<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      else if (step == 1) k_dilu_rbm<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      else k_dilu_rbm<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      continue;
+    }
     const int g = rb_grid(ctx, a.ntiles * a.nact);
     if (step == 0) k_dilu_rb2<T, V, RB_RED_FWD><<<g, 64, 0, ctx->stream>>>(a, tp.shape);
     else if (step == 1) k_dilu_rb2<T, V, RB_BLACK_FB><<<g, 64, 0, ctx->stream>>>(a, tp.shape);
     else k_dilu_rb2<T, V, RB_RED_BWD><<<g, 64, 0, ctx->stream>>>(a, tp.shape);
   }
+}
+
+// Packed i-face buffer of the red-black sweeps for systems 0..nsys-1
+// (DiluArgs::xf); sets a.xf / a.xfsys / a.xfhalf.
+template <class T, class V>
+static int rb_faces(mt_ctx* ctx, int nsys, DiluArgs<T, V>& a) {
+  const TilePlan& tp = ctx->tiles_rb;
+  a.xfsys = int64_t(tp.ntall) * 128;
+  a.xfhalf = int64_t(nsys) * a.xfsys;
+  return ws(ctx, sizeof(T) == 8 ? "rb_faces64" : "rb_faces32", size_t(2 * a.xfhalf), &a.xf);
 }
 
 template <class T, class V>
@@ -510,6 +542,8 @@ struct Engine {
   int g_nact = -1, g_chunk = -1;
   bool use_km = true;  // k-marching SpMV (set from the context at solve start)
   KmGeom km{};
+  C2<T>* xf = nullptr;  // red-black DILU packed faces (rb_faces)
+  int64_t xfsys = 0, xfhalf = 0;
   // timing: event pairs (family, first, second) recorded by one chunk; with a
   // graph they are recorded by the graph's event nodes on every replay
   struct TPair { int fam, e0, e1; };
@@ -537,6 +571,9 @@ struct Engine {
     DiluArgs<T, V> d = dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE,
                                        a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN);
     d.in2 = in2;
+    d.xf = xf;
+    d.xfsys = xfsys;
+    d.xfhalf = xfhalf;
     if (!ctx->timing) {
       apply_pc(d);
       return;
@@ -751,6 +788,11 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   C2<double>* dinv = nullptr;
   if ((rc = build_dinv<double>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   a.dinv = dinv;
+  if (o.precond == PCK_DILU_RB) {
+    DiluArgs<double, double> fa;
+    if ((rc = rb_faces(ctx, nsys, fa))) return rc;
+    E.xf = fa.xf; E.xfsys = fa.xfsys; E.xfhalf = fa.xfhalf;
+  }
   E.use_km = ctx->spmv_variant == 2;
   E.km = km_geom(ctx);
   const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
@@ -856,6 +898,11 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   C2<float>* dinv = nullptr;
   if ((rc = build_dinv<float>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   a.dinv = dinv;
+  if (o.precond == PCK_DILU_RB) {
+    DiluArgs<float, float> fa;
+    if ((rc = rb_faces(ctx, nsys, fa))) return rc;
+    E.xf = fa.xf; E.xfsys = fa.xfsys; E.xfhalf = fa.xfhalf;
+  }
   E.use_km = ctx->spmv_variant >= 1;
   E.km = km_geom(ctx);
   const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
@@ -1338,6 +1385,7 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
   } else if (precond == MT_PC_DILU_RB) {
     DiluArgs<double, double> d = dilu_args<double, double>(
         ctx->tiles_rb, ctx->dop.coef64, g.nE, g.nE, d_act, 1, d_sc, dx, dy, dinv, ST_RUN);
+    if ((rc = rb_faces(ctx, 1, d))) return rc;
     dilu_rb_apply<double, double>(ctx, ctx->tiles_rb, d);
   } else if (precond == MT_PC_JACOBI) {
     // y = dinv .* x via the p-update kernel path is overkill; do it on the host view
@@ -1669,6 +1717,7 @@ int mt_bench_precond(mt_ctx* ctx, int nsys, int precision, int precond, int reps
     const TilePlan& tp = precond == MT_PC_DILU_RB ? ctx->tiles_rb : ctx->tiles;
     DiluArgs<T, T> d = dilu_args<T, T>(tp, view<T>(ctx, false).coef, g.nE, g.nE, d_act,
                                        nsys, d_sc, vin, vout, dinv, ST_RUN);
+    if (precond == MT_PC_DILU_RB && (r2 = rb_faces(ctx, nsys, d))) return r2;
     auto apply = [&]() {
       if (precond == MT_PC_DILU_RB) {
         dilu_rb_apply<T, T>(ctx, tp, d);
@@ -1690,6 +1739,13 @@ int mt_bench_precond(mt_ctx* ctx, int nsys, int precision, int precond, int reps
   rc = (precision == MT_PREC_MIXED) ? run(float()) : run(double());
   if (rc) return rc;
   if (ms_per_apply) *ms_per_apply = double(ms) / reps;
+  return MT_OK;
+}
+
+int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
+  if (!ctx || (variant != 1 && variant != 4))
+    return mt_fail(ctx, MT_E_ARG, "mt_set_dilu_variant: variant is 1 or 4");
+  ctx->rb_variant = variant;
   return MT_OK;
 }
 

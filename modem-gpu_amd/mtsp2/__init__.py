@@ -91,6 +91,7 @@ _SIGS = {
     "mt_bench_spmv": (_I, [_P, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_timing_enable": (_I, [_P, _I]),
     "mt_set_spmv_variant": (_I, [_P, _I, _I]),
+    "mt_set_dilu_variant": (_I, [_P, _I]),
     "mt_bench_precond": (_I, [_P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
     "mt_receivers_set": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _P, _I, _PD, _P, _P]),
@@ -366,6 +367,9 @@ class Context:
         """0 row-parallel SpMV, 2 k-marching, 1 (default) k-marching in FP32
         solves and row-parallel in FP64 ones."""
         self._chk(lib().mt_set_spmv_variant(self._h, int(variant), int(layers_per_segment)))
+
+    def set_dilu_variant(self, variant):
+        self._chk(lib().mt_set_dilu_variant(self._h, int(variant)))
 
     def set_tuning(self, groups):
         self._chk(lib().mt_set_tuning(self._h, int(groups)))
