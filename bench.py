@@ -204,9 +204,11 @@ def main():
 
     dom = max(kt, key=lambda k: kt[k][1])
     nl, kms, kby = kt[dom]
+    rb_kernel = ("k_dilu_rbm<float,float,RED_FWD|BLACK_FB|RED_BWD,4>" if args.dilu_variant == 4
+                 else "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD>")
     pc_kernels = {0: "k_dilu_tile<float,float,8,8,8,true|false> (block-DILU level sweeps)",
-                  3: "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD> (red-black tile block-DILU)"}
-    kernels = {"spmv": "k_spmv_dot<float,float,1|2> (v = A x with its dot products)",
+                  3: rb_kernel + " (red-black tile block-DILU, 3 launches per application)"}
+    kernels = {"spmv": "k_spmv_km<float,float,1|2> (k-marching v = A x with its dot products)",
                "precond": pc_kernels.get(args.precond, "preconditioner")}
     achieved = kby / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
