@@ -90,6 +90,10 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the isolated SpMV launches (FP32 x 16 systems, FP64 x 1) and a "
                          "1 GiB device copy, for hardware-counter runs")
+    ap.add_argument("--spmv-variant", type=int, default=None,
+                    help="SpMV kernel of the solves: 0 row, 1 k-marching (FP32), 3 multi-system row")
+    ap.add_argument("--spmv-systems", type=int, default=4,
+                    help="systems per block of the multi-system row SpMV (4 or 8)")
     ap.add_argument("--dilu-variant", type=int, default=4,
                     help="red-black DILU launch shape: 1 single-system persistent, 4 systems per workgroup")
     ap.add_argument("--no-graph", action="store_true",
@@ -142,6 +146,9 @@ def main():
     ctx.upload(d_x0, X0)
     tuned = ctx.autotune(nsys)
     log(f"autotune: {tuned} row-tile groups per system")
+    if args.spmv_variant is not None:
+        ctx.set_spmv_variant(args.spmv_variant,
+                             args.spmv_systems if args.spmv_variant == 3 else 0)
     opts = dict(tol=args.tol, maxit=4000, precision=mtsp2.PREC_MIXED, precond=args.precond,
                 inner_tol=args.inner_tol, restart=args.restart, chunk=10,
                 use_graph=0 if args.no_graph else 1)
@@ -208,7 +215,13 @@ def main():
                  else "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD>")
     pc_kernels = {0: "k_dilu_tile<float,float,8,8,8,true|false> (block-DILU level sweeps)",
                   3: rb_kernel + " (red-black tile block-DILU, 3 launches per application)"}
-    kernels = {"spmv": "k_spmv_km<float,float,1|2> (k-marching v = A x with its dot products)",
+    spv = 3 if args.spmv_variant is None else args.spmv_variant
+    spmv_kernels = {0: "k_spmv_dot<float,float,1|2> (row-parallel v = A x with its dot products)",
+                    1: "k_spmv_km<float,float,1|2> (k-marching v = A x with its dot products)",
+                    2: "k_spmv_km<float,float,1|2> (k-marching v = A x with its dot products)",
+                    3: f"k_spmv_ms<float,float,1|2,{args.spmv_systems}> (v = A x for "
+                       f"{args.spmv_systems} systems per block, with the dot products)"}
+    kernels = {"spmv": spmv_kernels[spv],
                "precond": pc_kernels.get(args.precond, "preconditioner")}
     achieved = kby / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",

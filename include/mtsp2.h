@@ -270,17 +270,20 @@ int mt_bench_precond(mt_ctx *ctx, int nsys, int precision, int precond, int reps
                      double *ms_per_apply, double *bytes_per_apply);
 /* Chooses the SpMV of the solvers: 0 row-parallel (every row gathers its 14
  * neighbours from memory); 2 k-marching (a block keeps three k layers of x of
- * an (i, j) tile in LDS); 1 (default) k-marching in FP32 solves and
- * row-parallel in FP64 ones, the faster of the two for each.
- * layers_per_segment (> 0) sets the k extent of one k-marching block, 0
+ * an (i, j) tile in LDS); 1 k-marching in FP32 solves and row-parallel in
+ * FP64 ones; 3 (default) rows of several systems per block in FP32 solves
+ * (each row's couplings loaded once for the block's systems) and
+ * row-parallel in FP64 ones.  layers_per_segment (> 0) sets the k extent of
+ * one k-marching block, or for variant 3 the systems per block (4 or 8); 0
  * keeps it.  Has no counterpart in the reference (cuSPARSE's SpMV is fixed
  * there). */
+int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
 /* Launch shape of the red-black DILU (MT_PC_DILU_RB) sweeps: 1 = one system
  * per single-wave persistent workgroup with the next item prefetched into
  * registers; 4 (default) = four systems of one tile per workgroup sharing the
- * tile's couplings in LDS.  Same arithmetic. */
+ * tile's couplings in LDS.  Same arithmetic.  No counterpart in the
+ * reference. */
 int mt_set_dilu_variant(mt_ctx *ctx, int variant);
-int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
 /* Fixes the row-tile group count per system (0: automatic). */
 int mt_set_tuning(mt_ctx *ctx, int groups_per_system);
 /* Times the solver's fused SpMV+dot kernel (v = A x, <rt,v>) for nsys systems

@@ -273,6 +273,30 @@ __device__ __forceinline__ void chunk_range(int64_t n, int G, int g, int64_t& lo
   hi = lo + per < n ? lo + per : n;
 }
 
+// Row range [lo, hi) of chunk g of the interior rows (EDGEi order: x rows,
+// then y, then z) when the G chunks are dealt to the three components in
+// turn: chunk g is part g/3 of component g%3.  An XCD owns G/8 consecutive
+// chunks, hence the same fraction -- the same k-slab -- of all three
+// components, whose rows read each other's x: the cross-component gathers
+// then hit that XCD's L2 instead of the other XCDs' share of x being fetched
+// again from HBM.
+__device__ __forceinline__ void chunk_range3(int64_t nXi, int64_t nYi, int64_t nInt, int G, int g,
+                                             int64_t& lo, int64_t& hi) {
+  if (G < 3) {
+    chunk_range(nInt, G, g, lo, hi);
+    return;
+  }
+  const int c = g % 3, j = g / 3;
+  const int cnt = (G - c + 2) / 3;   // chunks of component c
+  const int64_t base = c == 0 ? 0 : (c == 1 ? nXi : nXi + nYi);
+  const int64_t n = c == 0 ? nXi : (c == 1 ? nYi : nInt - nXi - nYi);
+  const int64_t per = ((n + cnt - 1) / cnt + 63) & ~int64_t(63);
+  lo = per * j < n ? per * j : n;
+  hi = lo + per < n ? lo + per : n;
+  lo += base;
+  hi += base;
+}
+
 // Cross-block reductions never communicate inside a kernel: each block
 // stores its partial sums, and a one-block finalize kernel launched next on
 // the stream reduces them in fixed order.  (An in-kernel last-block scheme

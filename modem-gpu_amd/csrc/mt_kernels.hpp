@@ -201,7 +201,7 @@ __global__ __launch_bounds__(BS) void k_init(IterArgs<T, V> a) {
   C2<T>* __restrict__ r = a.r + o;
   C2<T>* __restrict__ xb = a.xbest ? a.xbest + o : nullptr;
   int64_t lo, hi;
-  chunk_range(a.A.nInt, a.G, g, lo, hi);
+  chunk_range3(a.A.nXi, a.A.nYi, a.A.nInt, a.G, g, lo, hi);
   for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
     constexpr int C = decltype(cc)::value;
     const C2<T> bv = b[ri.e];
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(IterArgs<T, V> a) {
     const C2<T>* __restrict__ pv = a.v + o;
     const C2<T> al = ccast<T>(sc0.alpha);
     int64_t lo, hi;
-    chunk_range(a.A.nInt, a.G, g, lo, hi);
+    chunk_range3(a.A.nXi, a.A.nYi, a.A.nInt, a.G, g, lo, hi);
     for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
       constexpr int C = decltype(cc)::value;
       C2<T> w = pw[ri.e];
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(BS) void k_recheck(IterArgs<T, V> a) {
     const C2<T>* __restrict__ rt = a.rt + o;
     C2<T>* __restrict__ r = a.r + o;
     int64_t lo, hi;
-    chunk_range(a.A.nInt, a.G, g, lo, hi);
+    chunk_range3(a.A.nXi, a.A.nYi, a.A.nInt, a.G, g, lo, hi);
     for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
       constexpr int C = decltype(cc)::value;
       const C2<T> w = rt[ri.e];
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(BS) void k_outer_resid(OuterArgs a) {
     C2<float>* __restrict__ r32 = a.r32 + o;
     C2<float>* __restrict__ d32 = a.d32 + o;
     int64_t lo, hi;
-    chunk_range(a.A.nInt, a.G, g, lo, hi);
+    chunk_range3(a.A.nXi, a.A.nYi, a.A.nInt, a.G, g, lo, hi);
     for_rows(a.A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
       constexpr int C = decltype(cc)::value;
       const Z y = csub(b[ri.e], apply_row_c<C, double, double>(a.A, ri, x, dsc));

@@ -93,9 +93,11 @@ struct mt_ctx {
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   int G_override = 0;
   int num_cu = 256;             // compute units of the device
-  int spmv_variant = 1;         // 0 row-parallel SpMV; 1 k-marching for FP32
-                                // solves, row-parallel for FP64; 2 k-marching
+  int spmv_variant = 3;         // 0 row-parallel SpMV; 1 k-marching for FP32
+                                // solves, row-parallel for FP64; 2 k-marching;
+                                // 3 multi-system rows for FP32, row-parallel FP64
   int km_ks = 16;               // k-marching SpMV: layers per k segment
+  int ms_S = 4;                 // multi-system row SpMV (variant 3): systems per block
   int rb_variant = 4;           // red-black DILU: 1 persistent single-system
                                 // workgroups; 4 four systems per workgroup
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
@@ -542,6 +544,7 @@ struct Engine {
   int g_nact = -1, g_chunk = -1;
   bool use_km = true;  // k-marching SpMV (set from the context at solve start)
   KmGeom km{};
+  int ms = 0;          // >0: the multi-system row SpMV with this many systems per block
   C2<T>* xf = nullptr;  // red-black DILU packed faces (rb_faces)
   int64_t xfsys = 0, xfhalf = 0;
   // timing: event pairs (family, first, second) recorded by one chunk; with a
@@ -587,7 +590,12 @@ struct Engine {
   int spmv(int nblk) {
     const int e0 = ctx->timing ? trec() : -1;
     int G;
-    if (use_km) {
+    if (ms > 0) {
+      const int nb = ((a.nact + ms - 1) / ms) * a.G;
+      if (ms == 4) k_spmv_ms<T, V, WHICH, 4><<<nb, BS, 0, ctx->stream>>>(a);
+      else k_spmv_ms<T, V, WHICH, 8><<<nb, BS, 0, ctx->stream>>>(a);
+      G = a.G;
+    } else if (use_km) {
       k_spmv_km<T, V, WHICH><<<a.nact * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
       G = km.Gpad;
     } else {
@@ -903,7 +911,8 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     if ((rc = rb_faces(ctx, nsys, fa))) return rc;
     E.xf = fa.xf; E.xfsys = fa.xfsys; E.xfhalf = fa.xfhalf;
   }
-  E.use_km = ctx->spmv_variant >= 1;
+  E.use_km = ctx->spmv_variant == 1 || ctx->spmv_variant == 2;
+  E.ms = ctx->spmv_variant == 3 ? ctx->ms_S : 0;
   E.km = km_geom(ctx);
   const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
   SysScal *d_so, *d_si;
@@ -1627,6 +1636,7 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   Z* partials;
   const KmGeom km = km_geom(ctx);
   const bool use_km = ctx->spmv_variant == 2 || (ctx->spmv_variant == 1 && precision == MT_PREC_MIXED);
+  const int msS = (ctx->spmv_variant == 3 && precision == MT_PREC_MIXED) ? ctx->ms_S : 0;
   if ((rc = ws(ctx, "bn_sc", nsys, &d_sc)) || (rc = ws(ctx, "bn_act", nsys, &d_act)) ||
       (rc = ws(ctx, "partials", size_t(nsys) * std::max(pick_G(ctx, 1), km.Gpad) * 2 + 64,
                &partials)))
@@ -1655,7 +1665,9 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
       return r2;
     a.ph = vin; a.v = vout; a.rt = vrt;
     auto launch = [&]() {
-      if (use_km) k_spmv_km<T, T, 1><<<nsys * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
+      if (msS == 4) k_spmv_ms<T, T, 1, 4><<<((nsys + 3) / 4) * G, BS, 0, ctx->stream>>>(a);
+      else if (msS == 8) k_spmv_ms<T, T, 1, 8><<<((nsys + 7) / 8) * G, BS, 0, ctx->stream>>>(a);
+      else if (use_km) k_spmv_km<T, T, 1><<<nsys * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
       else k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);
     };
     launch();  // warm-up
@@ -1750,10 +1762,15 @@ int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
 }
 
 int mt_set_spmv_variant(mt_ctx* ctx, int variant, int layers_per_segment) {
-  if (!ctx || variant < 0 || variant > 2 || layers_per_segment < 0)
-    return mt_fail(ctx, MT_E_ARG, "mt_set_spmv_variant: variant 0, 1 or 2, layers >= 0");
+  if (!ctx || variant < 0 || variant > 3 || layers_per_segment < 0 ||
+      (variant == 3 && layers_per_segment != 0 && layers_per_segment != 4 && layers_per_segment != 8))
+    return mt_fail(ctx, MT_E_ARG, "mt_set_spmv_variant: variant 0..3, layers >= 0 (4 or 8 systems for 3)");
   ctx->spmv_variant = variant;
-  if (layers_per_segment > 0) ctx->km_ks = layers_per_segment;
+  if (variant == 3) {
+    if (layers_per_segment > 0) ctx->ms_S = layers_per_segment;
+  } else if (layers_per_segment > 0) {
+    ctx->km_ks = layers_per_segment;
+  }
   return MT_OK;
 }
 

@@ -264,4 +264,115 @@ __global__ __launch_bounds__(KM_BS) void k_spmv_km(IterArgs<T, V> a, KmGeom gm) 
   }
 }
 
+// ------------------------------------------------ several systems per row
+// y_s = A x_s for S systems per block over a chunk of rows: each row's 15
+// couplings and vms1 are loaded once (the row-major copy, 16 consecutive
+// values) and applied to the S systems' x.  With 16 systems the couplings are
+// 130 MB of the 920 MB an FP32 SpMV launch must move; loaded per system they
+// are read 16 times.  Blocks: (chunk, group of S active systems), groups
+// fastest inside an XCD; chunks dealt to the three components in turn
+// (chunk_range3) so an XCD's cross-component gathers stay in its L2.  Partial
+// dots per (system, chunk) as k_spmv_dot stores them.
+template <class T, class V, int WHICH, int S>
+__global__ __launch_bounds__(BS) void k_spmv_ms(IterArgs<T, V> a) {
+  __shared__ Z lds[S][2 * BS / 64];
+  const int ngrp = (a.nact + S - 1) / S;
+  const int b = blockIdx.x, xcd = b & 7, w = b >> 3;
+  const int grp = w % ngrp, g = xcd * (a.G >> 3) + w / ngrp;
+  const StencilView<V>& A = a.A;
+  const C2<T>* __restrict__ in[S];
+  C2<T>* __restrict__ out[S];
+  const C2<T>* __restrict__ pw[S];
+  const C2<T>* __restrict__ pv[S];
+  T dsc[S];
+  C2<T> al[S];
+  bool on[S];
+  const bool fz = WHICH == 2 && a.fused;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int si = grp * S + s;
+    on[s] = false;
+    in[s] = out[s] = nullptr;
+    pw[s] = pv[s] = nullptr;
+    dsc[s] = T(0);
+    al[s] = {T(0), T(0)};
+    if (si < a.nact) {
+      const int sys = a.act[si];
+      const SysScal& sc0 = a.sc[sys];
+      if (sc0.status == ST_RUN) {
+        on[s] = true;
+        const int64_t o = (int64_t)sys * a.nE;
+        dsc[s] = T(kIsignD * sc0.freq_omega);
+        in[s] = (WHICH == 1 ? a.ph : a.sh) + o;
+        out[s] = (WHICH == 1 ? a.v : a.t) + o;
+        pw[s] = (WHICH == 1 ? a.rt : (fz ? a.r : a.s)) + o;
+        pv[s] = a.v + o;
+        al[s] = ccast<T>(sc0.alpha);
+      }
+    }
+  }
+  Z acc[S][2];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc[s][0] = acc[s][1] = {0.0, 0.0};
+  int64_t lo, hi;
+  chunk_range3(A.nXi, A.nYi, A.nInt, a.G, g, lo, hi);
+  for_rows(A, lo, hi, threadIdx.x, BS, [&](const RowInfo& ri, auto cc) {
+    constexpr int C = decltype(cc)::value;
+    V cf[16];
+    {
+      using V4 = typename std::conditional<std::is_same<V, float>::value, float4, double2>::type;
+      constexpr int PER = sizeof(V4) / sizeof(V);
+      const V4* __restrict__ rp = reinterpret_cast<const V4*>(A.rowc + ri.e * 16);
+#pragma unroll
+      for (int q = 0; q < 16 / PER; ++q) {
+        const V4 v4 = rp[q];
+        if constexpr (PER == 4) {
+          cf[4 * q] = v4.x; cf[4 * q + 1] = v4.y; cf[4 * q + 2] = v4.z; cf[4 * q + 3] = v4.w;
+        } else {
+          cf[2 * q] = v4.x; cf[2 * q + 1] = v4.y;
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      if (!on[s]) continue;
+      const C2<T>* __restrict__ x = in[s];
+      const C2<T> xe = x[ri.e];
+      C2<T> y = cscale(T(cf[0]), xe);
+#pragma unroll
+      for (int k = 1; k < 15; ++k) {
+        const int64_t base = k < 7 ? ri.e : (k < 11 ? ri.b1 : ri.b2);
+        const C2<T> xv = x[base + A.off[C][k]];
+        const T c = T(cf[k]);
+        y.x += c * xv.x;
+        y.y += c * xv.y;
+      }
+      const T d = dsc[s] * T(cf[15]);
+      y.x += -d * xe.y;
+      y.y += d * xe.x;
+      out[s][ri.e] = y;
+      C2<T> wv = pw[s][ri.e];
+      if (fz) wv = csub(wv, cmul(al[s], pv[s][ri.e]));
+      if (WHICH == 1) {
+        acc[s][0].x += double(wv.x) * y.x + double(wv.y) * y.y;
+        acc[s][0].y += double(wv.x) * y.y - double(wv.y) * y.x;
+      } else {
+        acc[s][0].x += double(y.x) * y.x + double(y.y) * y.y;
+        acc[s][1].x += double(y.x) * wv.x + double(y.y) * wv.y;
+        acc[s][1].y += double(y.x) * wv.y - double(y.y) * wv.x;
+      }
+    }
+  });
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int si = grp * S + s;
+    if (si >= a.nact) continue;   // uniform over the block
+    block_sum<2, BS>(acc[s], lds[s]);
+    if (threadIdx.x == 0) {
+      a.partials[((int64_t)si * a.G + g) * 2 + 0] = acc[s][0];
+      a.partials[((int64_t)si * a.G + g) * 2 + 1] = acc[s][1];
+    }
+  }
+}
+
 }  // namespace mt

@@ -364,8 +364,10 @@ class Context:
         return v.value
 
     def set_spmv_variant(self, variant, layers_per_segment=0):
-        """0 row-parallel SpMV, 2 k-marching, 1 (default) k-marching in FP32
-        solves and row-parallel in FP64 ones."""
+        """0 row-parallel SpMV, 2 k-marching, 1 k-marching in FP32 solves and
+        row-parallel in FP64 ones, 3 (default) multi-system rows in FP32 solves
+        (layers_per_segment = systems per block, 4 or 8) and row-parallel in
+        FP64 ones."""
         self._chk(lib().mt_set_spmv_variant(self._h, int(variant), int(layers_per_segment)))
 
     def set_dilu_variant(self, variant):

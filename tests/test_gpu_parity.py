@@ -219,15 +219,16 @@ def test_batch_dilu_rb_converges(gpu_ctx, precision):
 @pytest.mark.parametrize("precision", [mtsp2.PREC_FP64, mtsp2.PREC_MIXED])
 def test_spmv_variants_agree(gpu_ctx, precision):
     """The k-marching SpMV (LDS layers of an (i,j) tile, partial tiles in i and
-    j, several k segments) and the row-parallel SpMV drive the solver to the
-    same solutions; both meet the FP64 true residual."""
+    j, several k segments), the multi-system row SpMV (4 or 8 systems per
+    block, partly filled groups) and the row-parallel SpMV drive the solver to
+    the same solutions; all meet the FP64 true residual."""
     print(f"case spmv variants precision={precision}", flush=True)
     o, g = _setup(gpu_ctx, 14, nx=37, ny=13, nzAir=4, nzEarth=15)
     grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
     om, B, X0 = P.forward_systems(gpu_ctx, grid, [0.05, 20.0])
     res = {}
     try:
-        for var, ks in ((0, 0), (2, 3), (2, 16)):
+        for var, ks in ((0, 0), (2, 3), (2, 16), (3, 4), (3, 8)):
             gpu_ctx.set_spmv_variant(var, ks)
             X, st = gpu_ctx.solve_batch(om, B, X0, tol=1e-10, maxit=3000, precision=precision)
             for s in range(len(om)):
@@ -237,6 +238,8 @@ def test_spmv_variants_agree(gpu_ctx, precision):
                 assert np.linalg.norm(r) / np.linalg.norm(B[s]) <= 1e-10 * 1.0001
             res[(var, ks)] = X
     finally:
-        gpu_ctx.set_spmv_variant(1, 16)
+        gpu_ctx.set_spmv_variant(3, 4)
     assert rel(res[(2, 3)], res[(0, 0)]) <= 1e-7
     assert rel(res[(2, 16)], res[(0, 0)]) <= 1e-7
+    assert rel(res[(3, 4)], res[(0, 0)]) <= 1e-7
+    assert rel(res[(3, 8)], res[(0, 0)]) <= 1e-7
