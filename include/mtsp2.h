@@ -170,6 +170,13 @@ int mt_precond_apply(mt_ctx *ctx, double omega, int precond, const mt_zdouble *x
 /* Forward RHS b = -CCib e_b from boundary values (all-edge vector e_all, only
  * its boundary entries are read).  Requires mt_assemble. */
 int mt_rhs_fwd(mt_ctx *ctx, const mt_zdouble *e_all, mt_zdouble *b);
+/* The same for a batch on the device, with the initial guess: for each of
+ * nsys all-edge boundary fields (e_dev, [nsys][n_edge] complex), b = -CCib e_b
+ * (Mult_Aib, modelOperator3D.f90:438-459) and, when x0_dev is not NULL,
+ * x0 = e(EDGEi), the initial guess FWDsolve3D takes from the boundary field
+ * (EMsolve3D.f90:217-274); b_dev, x0_dev: [nsys][n_int] in EDGEi order, ready
+ * for mt_solve_batch_dev.  Requires mt_assemble. */
+int mt_rhs_fwd_dev(mt_ctx *ctx, int nsys, const void *e_dev, void *b_dev, void *x0_dev);
 /* Transposed (adjoint) RHS b = s_i + V_E GDii(s_i (i*ISIGN*omega*mu0)^-1 / V_E). */
 int mt_rhs_trn(mt_ctx *ctx, double omega, const mt_zdouble *s_all, mt_zdouble *b);
 /* TRN post-processing: e(EDGEi)=ei, e = V_E e, e(EDGEb) = -CCib^T ei. */

@@ -98,6 +98,7 @@ struct mt_ctx {
                                 // 3 multi-system rows for FP32, row-parallel FP64
   int km_ks = 16;               // k-marching SpMV: layers per k segment
   int ms_S = 4;                 // multi-system row SpMV (variant 3): systems per block
+  bool rhs_ready = false;       // CCib uploaded for mt_rhs_fwd_dev (workspaces rhs_*)
   int rb_variant = 4;           // red-black DILU: 1 persistent single-system
                                 // workgroups; 4 four systems per workgroup
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
@@ -268,6 +269,7 @@ static int upload_operator(mt_ctx* ctx) {
   // Padded vectors rely on zero boundary entries and a zero tail; buffers laid
   // out for another grid would carry stale values there, so drop them all.
   free_workspaces(ctx);
+  ctx->rhs_ready = false;
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   ctx->dc = DivcorData();
@@ -1420,6 +1422,55 @@ int mt_rhs_fwd(mt_ctx* ctx, const mt_zdouble* e_all, mt_zdouble* b) {
     for (int q = H.ccib_row[r]; q < H.ccib_row[r + 1]; ++q) s += H.ccib_val[q] * e_all[H.ccib_col[q]];
     b[r] = -s;
   }
+  return MT_OK;
+}
+
+// Mult_Aib for a batch on the device: one thread per (interior row, system),
+// b = -sum over the row's CCib entries in their stored order; x0 = e(EDGEi).
+__global__ void k_rhs_fwd(int nsys, int64_t nInt, int64_t nE, const int* __restrict__ row,
+                          const int* __restrict__ col, const double* __restrict__ val,
+                          const int* __restrict__ imap, const Z* __restrict__ e, Z* __restrict__ b,
+                          Z* __restrict__ x0) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nInt * nsys) return;
+  const int s = int(t / nInt);
+  const int64_t r = t - int64_t(s) * nInt;
+  const Z* __restrict__ es = e + int64_t(s) * nE;
+  Z acc = {0.0, 0.0};
+  for (int q = row[r]; q < row[r + 1]; ++q) {
+    const Z v = es[col[q]];
+    acc.x += val[q] * v.x;
+    acc.y += val[q] * v.y;
+  }
+  b[t] = {-acc.x, -acc.y};
+  if (x0) x0[t] = es[imap[r]];
+}
+
+int mt_rhs_fwd_dev(mt_ctx* ctx, int nsys, const void* e_dev, void* b_dev, void* x0_dev) {
+  if (!ctx || !e_dev || !b_dev || nsys < 1) return mt_fail(ctx, MT_E_ARG, "mt_rhs_fwd_dev: bad arguments");
+  const HostOperator& H = ctx->hop;
+  if (!H.have_model || !ctx->dop.ready) return mt_fail(ctx, MT_E_STATE, "mt_rhs_fwd_dev needs mt_assemble");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  int *drow, *dcol;
+  double* dval;
+  int rc;
+  if ((rc = ws(ctx, "rhs_row", H.ccib_row.size(), &drow)) ||
+      (rc = ws(ctx, "rhs_col", std::max<size_t>(1, H.ccib_col.size()), &dcol)) ||
+      (rc = ws(ctx, "rhs_val", std::max<size_t>(1, H.ccib_val.size()), &dval)))
+    return rc;
+  if (!ctx->rhs_ready) {
+    MT_HIP_CHECK(ctx, hipMemcpy(drow, H.ccib_row.data(), H.ccib_row.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!H.ccib_col.empty()) {
+      MT_HIP_CHECK(ctx, hipMemcpy(dcol, H.ccib_col.data(), H.ccib_col.size() * sizeof(int), hipMemcpyHostToDevice));
+      MT_HIP_CHECK(ctx, hipMemcpy(dval, H.ccib_val.data(), H.ccib_val.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    ctx->rhs_ready = true;
+  }
+  const int64_t nt = H.g.nInt * nsys;
+  k_rhs_fwd<<<unsigned((nt + 255) / 256), 256, 0, ctx->stream>>>(
+      nsys, H.g.nInt, H.g.nE, drow, dcol, dval, ctx->dop.imap, (const Z*)e_dev, (Z*)b_dev, (Z*)x0_dev);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MT_OK;
 }
 

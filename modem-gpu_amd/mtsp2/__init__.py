@@ -80,6 +80,7 @@ _SIGS = {
     "mt_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
     "mt_precond_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
     "mt_rhs_fwd": (_I, [_P, _PZ, _PZ]),
+    "mt_rhs_fwd_dev": (_I, [_P, _I, _P, _P, _P]),
     "mt_rhs_trn": (_I, [_P, _D, _PZ, _PZ]),
     "mt_post_trn": (_I, [_P, _PZ, _PZ]),
     "mt_divcor": (_I, [_P, _D, _PZ, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
@@ -322,6 +323,11 @@ class Context:
         b = np.zeros(self.n_int, np.complex128)
         self._chk(lib().mt_rhs_fwd(self._h, np.ascontiguousarray(e_all, np.complex128), b))
         return b
+
+    def rhs_fwd_dev(self, nsys, e_ptr, b_ptr, x0_ptr=None):
+        """b = -CCib e_b (and x0 = e(EDGEi)) for nsys device all-edge fields."""
+        self._chk(lib().mt_rhs_fwd_dev(self._h, int(nsys), C.c_void_p(e_ptr), C.c_void_p(b_ptr),
+                                       None if x0_ptr is None else C.c_void_p(x0_ptr)))
 
     def rhs_trn(self, omega, s_all):
         b = np.zeros(self.n_int, np.complex128)
