@@ -111,6 +111,18 @@ int mt_operator_set_csr(mt_ctx *ctx, int n_int, int nnz, const int *rowptr, cons
 int mt_assemble(mt_ctx *ctx, int nx, int ny, int nzAir, int nzEarth, const double *dx,
                 const double *dy, const double *dz, const double *logcond, double air_logcond);
 
+/* UpdateCond (modelOperator3D.f90:167-213) on the device: a new LOGE model
+ * (logcond, nx*ny*nzEarth, Fortran order, and the air value) on the grid of
+ * the last mt_assemble.  sigma_E / sigma_N (cell averages, ModelMap.inc:88-277),
+ * AAii = CCii + GDii(sigma) (GradDivSetup2 with airNIndex,
+ * modelOperator3D.f90:916-1130) and vms1 are recomputed on the GPU; the
+ * curl-curl part is the assembly's.  The device operator equals what
+ * mt_assemble builds for the same model, bit for bit; host copies used by the
+ * host-side entry points are rebuilt on their first use.  Replaces the
+ * updateCond(sigma) call that runs on the CPU before every transmitter's
+ * forward solve (ForwardSolver.f90:261). */
+int mt_update_cond(mt_ctx *ctx, const double *logcond, double air_logcond);
+
 /* Host-only assembly (no device needed): the same construction as
  * mt_assemble, returned as interior CSR (0-based, row-sorted) plus vms1.
  * Call with rowptr == NULL to get *n_int and *nnz, then with arrays. */

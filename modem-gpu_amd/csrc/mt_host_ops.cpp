@@ -48,6 +48,20 @@ const SlotTable& slots() {
 
 }  // namespace
 
+void graddiv_slots(int slot[3][2][6]) {
+  const SlotTable& ST = slots();
+  // lattice offset of the node's edge q from the node
+  const int qc[6] = {0, 0, 1, 1, 2, 2};
+  const int qd[6][3] = {{-1, 0, 0}, {0, 0, 0}, {0, -1, 0}, {0, 0, 0}, {0, 0, -1}, {0, 0, 0}};
+  for (int c = 0; c < 3; ++c)
+    for (int side = 0; side < 2; ++side)
+      for (int q = 0; q < 6; ++q) {
+        int d[3] = {qd[q][0], qd[q][1], qd[q][2]};
+        d[c] += side;   // the second node is one step along the edge
+        slot[c][side][q] = ST.get(c, qc[q], d[0], d[1], d[2]);
+      }
+}
+
 void build_index_maps(HostOperator& op) {
   const Grid& g = op.g;
   op.imap.resize(g.nInt);
@@ -239,6 +253,8 @@ Status assemble(HostOperator& op, int nx, int ny, int nzAir, int nzEarth, const 
             }
         }
   }
+
+  op.coefCC = op.coef;   // the sigma-free part, for updates of sigma on the device
 
   // --- grad-div: each interior node's 6 edges (G = -1 at an edge's first
   // node, +1 at its second, setGradTopology spOpTopology_SG.f90:173-266)

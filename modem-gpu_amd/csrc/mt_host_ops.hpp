@@ -18,6 +18,7 @@ struct HostOperator {
   Grid g;
   bool have_model = false;             // true after assemble()
   std::vector<double> coef;            // [15][nE] AAii, interior rows only
+  std::vector<double> coefCC;          // [15][nE] its curl-curl part (sigma-free, assemble only)
   std::vector<double> vms1;            // [nE] mu0*sigma_E*V_E on interior edges
   std::vector<double> Vedge;           // [nE] (assemble only)
   std::vector<int> imap;               // [nInt] interior (EDGEi order) -> all-edge index
@@ -42,6 +43,12 @@ struct HostOperator {
   std::vector<char> nodeEarth;         // [nN] earth class of interior nodes (airNIndex)
   std::vector<double> Vcell_ghost_node;  // [nN] sum of adjacent cell volumes
 };
+
+// Slots of the grad-div couplings of a row of component c through the node
+// at its first (side 0) or second (side 1) end: slot[c][side][q] for the
+// node's six edges q in the order of assemble()'s grad-div loop
+// (x(i-1), x(i), y(j-1), y(j), z(k-1), z(k) around node (i, j, k)).
+void graddiv_slots(int slot[3][2][6]);
 
 // Gradient of Re(sum_pol lam_pol^T A(sigma) e_pol) w.r.t. the LOGE cell
 // parameters, for one frequency: e_all = forward fields on all edges

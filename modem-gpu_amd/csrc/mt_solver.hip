@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "mt_cond.hpp"
 #include "mt_datafunc.hpp"
 #include "mt_dilu.hpp"
 #include "mt_divcor.hpp"
@@ -99,6 +100,11 @@ struct mt_ctx {
   int km_ks = 16;               // k-marching SpMV: layers per k segment
   int ms_S = 4;                 // multi-system row SpMV (variant 3): systems per block
   bool rhs_ready = false;       // CCib uploaded for mt_rhs_fwd_dev (workspaces rhs_*)
+  // mt_update_cond: the device operator is newer than the host copies in hop
+  // (refreshed by ensure_host before host-side use); cond_ready: the
+  // sigma-free assembly data are on the device (workspaces cond_*)
+  bool host_stale = false;
+  bool cond_ready = false;
   int rb_variant = 4;           // red-black DILU: 1 persistent single-system
                                 // workgroups; 4 four systems per workgroup
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
@@ -270,6 +276,8 @@ static int upload_operator(mt_ctx* ctx) {
   // out for another grid would carry stale values there, so drop them all.
   free_workspaces(ctx);
   ctx->rhs_ready = false;
+  ctx->cond_ready = false;
+  ctx->host_stale = false;
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   ctx->dc = DivcorData();
@@ -312,8 +320,24 @@ static int upload_operator(mt_ctx* ctx) {
   return build_tile_plan(ctx, ctx->tiles_rb, 3, lat, true);
 }
 
+// After mt_update_cond the host copies of the sigma-dependent operator data
+// (coefficients, sigma_E/N, the RHS-form GDii, the earth classes) are rebuilt
+// once, on first host-side use, by the host assembly of the current model
+// (which gives the same coefficients as the device, bit for bit).
+static int ensure_host(mt_ctx* ctx) {
+  if (!ctx->host_stale) return MT_OK;
+  HostOperator& H = ctx->hop;
+  const std::vector<double> dx = H.dx, dy = H.dy, dz = H.dz, lc = H.logcond;
+  Status st = assemble(H, H.g.nx, H.g.ny, H.nzAir, H.nzEarth, dx.data(), dy.data(), dz.data(),
+                       lc.data(), H.airLog);
+  if (st.code) return mt_fail(ctx, st.code, st.msg);
+  ctx->host_stale = false;
+  return MT_OK;
+}
+
 static int ensure_transpose(mt_ctx* ctx) {
   if (ctx->dop.coefT64) return MT_OK;
+  { int rc = ensure_host(ctx); if (rc) return rc; }
   const Grid& g = ctx->hop.g;
   std::vector<double> cT;
   transpose_stencil(ctx->hop, cT);
@@ -504,6 +528,7 @@ static void dilu_apply(mt_ctx* ctx, bool transpose, const int* act, int nact, co
 
 static int64_t stencil_nnz(mt_ctx* ctx) {
   if (ctx->nnz_stencil < 0) {
+    if (ensure_host(ctx)) return 0;
     const Grid& g = ctx->hop.g;
     int64_t z = 0;
     for (int e : ctx->hop.imap)
@@ -1203,6 +1228,7 @@ int mt_operator_size(const mt_ctx* ctx, int* n_int, int* n_edge, int* nnz) {
   if (n_int) *n_int = int(g.nInt);
   if (n_edge) *n_edge = int(g.nE);
   if (nnz) {
+    if (ensure_host(const_cast<mt_ctx*>(ctx))) return MT_E_STATE;
     int64_t z = 0;
     for (int e : ctx->hop.imap)
       for (int s = 0; s < kSlots; ++s) z += ctx->hop.coef[size_t(s) * g.nE + e] != 0.0;
@@ -1214,6 +1240,7 @@ int mt_operator_size(const mt_ctx* ctx, int* n_int, int* n_edge, int* nnz) {
 int mt_operator_get_csr(mt_ctx* ctx, int* rowptr, int* col, double* val, double* vms1) {
   int rc = need_op(ctx);
   if (rc) return rc;
+  if ((rc = ensure_host(ctx))) return rc;
   std::vector<int> r, c;
   std::vector<double> v;
   stencil_to_csr(ctx->hop, r, c, v);
@@ -1478,6 +1505,7 @@ int mt_rhs_trn(mt_ctx* ctx, double omega, const mt_zdouble* s_all, mt_zdouble* b
   if (!ctx || !s_all || !b) return mt_fail(ctx, MT_E_ARG, "mt_rhs_trn: null argument");
   const HostOperator& H = ctx->hop;
   if (!H.have_model) return mt_fail(ctx, MT_E_STATE, "mt_rhs_trn needs mt_assemble");
+  { int rc = ensure_host(ctx); if (rc) return rc; }
   const mt_zdouble iOmegaMuInv = 1.0 / mt_zdouble(0.0, kIsign * omega * kMu0);
   std::vector<mt_zdouble> si(H.g.nE, 0.0);
   for (int64_t r = 0; r < H.g.nInt; ++r) {
@@ -1513,6 +1541,7 @@ int mt_pmultT(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e0, const m
   if (!ctx || !e0 || !lam || !dm || npol < 1) return mt_fail(ctx, MT_E_ARG, "mt_pmultT: bad arguments");
   const HostOperator& H = ctx->hop;
   if (!H.have_model) return mt_fail(ctx, MT_E_STATE, "mt_pmultT needs mt_assemble");
+  { int rc = ensure_host(ctx); if (rc) return rc; }
   const int64_t nE = H.g.nE;
   // minus_i_omega_mu = cmplx(0, -ISIGN*MU_0*omega) (SolverSens.f90:95)
   const mt_zdouble c(0.0, -kIsign * kMu0 * omega);
@@ -1532,6 +1561,7 @@ int mt_pmultT(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e0, const m
 static int ensure_divcor(mt_ctx* ctx) {
   DivcorData& dc = ctx->dc;
   if (dc.ready) return MT_OK;
+  { int rc = ensure_host(ctx); if (rc) return rc; }
   const HostOperator& H = ctx->hop;
   const Grid& g = H.g;
   std::vector<double> coefN, wE, dinv;
@@ -1565,6 +1595,7 @@ int mt_gradient(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e_all,
   if (!ctx || !e_all || !lam || !dm || npol < 1 || mode < 0 || mode > 1)
     return mt_fail(ctx, MT_E_ARG, "mt_gradient: bad arguments");
   if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_gradient needs mt_assemble");
+  { int rc = ensure_host(ctx); if (rc) return rc; }
   grad_assemble(ctx->hop, omega, npol, reinterpret_cast<const zc*>(e_all),
                 reinterpret_cast<const zc*>(lam), mode, dm);
   return MT_OK;
@@ -1994,4 +2025,93 @@ int mt_lmultT(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const voi
               const double* d, const unsigned char* exist, void* comb_dev) {
   if (!ctx || !d || !comb_dev) return mt_fail(ctx, MT_E_ARG, "mt_lmultT: null argument");
   return site_launch(ctx, dtype, nfreq, omegas, e_dev, nullptr, d, exist, comb_dev);
+}
+
+// ---------------------------------------------------------------------------
+// UpdateCond on the device
+// ---------------------------------------------------------------------------
+int mt_update_cond(mt_ctx* ctx, const double* logcond, double air_logcond) {
+  if (!ctx || !logcond) return mt_fail(ctx, MT_E_ARG, "mt_update_cond: null argument");
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  HostOperator& H = ctx->hop;
+  if (!H.have_model || H.coefCC.empty())
+    return mt_fail(ctx, MT_E_STATE, "mt_update_cond needs mt_assemble on this grid");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const Grid& g = H.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nza = H.nzAir;
+  const int64_t nN = int64_t(nx + 1) * (ny + 1) * (nz + 1), ncell = int64_t(nx) * ny * nz;
+  // cell conductivities as the host assembly forms them (exp on the host)
+  std::vector<double> cs(ncell);
+  const double airC = std::exp(air_logcond);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+        cs[i + int64_t(nx) * (j + int64_t(ny) * k)] =
+            k < nza ? airC : std::exp(logcond[i + int64_t(nx) * (j + int64_t(ny) * (k - nza))]);
+  CondArgs a;
+  a.nx = nx; a.ny = ny; a.nz = nz;
+  a.nXe = g.nXe; a.nYe = g.nYe; a.nE = g.nE; a.nInt = g.nInt;
+  double *ddx, *ddy, *ddz, *dlx, *dly, *dlz, *dcc, *dcs, *dsE, *dsN, *dm2;
+  int* dslot;
+  unsigned char* dearth;
+  if ((rc = ws(ctx, "cond_dx", nx, &ddx)) || (rc = ws(ctx, "cond_dy", ny, &ddy)) ||
+      (rc = ws(ctx, "cond_dz", nz, &ddz)) || (rc = ws(ctx, "cond_delx", nx + 1, &dlx)) ||
+      (rc = ws(ctx, "cond_dely", ny + 1, &dly)) || (rc = ws(ctx, "cond_delz", nz + 1, &dlz)) ||
+      (rc = ws(ctx, "cond_cc", size_t(kSlots) * g.nE, &dcc)) || (rc = ws(ctx, "cond_slot", 36, &dslot)) ||
+      (rc = ws(ctx, "cond_csig", ncell, &dcs)) || (rc = ws(ctx, "cond_sige", g.nE, &dsE)) ||
+      (rc = ws(ctx, "cond_earth", g.nE, &dearth)) || (rc = ws(ctx, "cond_sign", nN, &dsN)) ||
+      (rc = ws(ctx, "cond_m2", nN, &dm2)))
+    return rc;
+  if (!ctx->cond_ready) {
+    auto half = [](const std::vector<double>& d) {
+      const int n = int(d.size());
+      std::vector<double> h(n + 1);
+      h[0] = d[0];
+      for (int i = 1; i < n; ++i) h[i] = d[i - 1] + d[i];
+      h[n] = d[n - 1];
+      for (auto& v : h) v /= 2.0;
+      return h;
+    };
+    const std::vector<double> hx = half(H.dx), hy = half(H.dy), hz = half(H.dz);
+    int slot[3][2][6];
+    graddiv_slots(slot);
+    MT_HIP_CHECK(ctx, hipMemcpy(ddx, H.dx.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(ddy, H.dy.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(ddz, H.dz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(dlx, hx.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(dly, hy.data(), (ny + 1) * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(dlz, hz.data(), (nz + 1) * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(dcc, H.coefCC.data(), size_t(kSlots) * g.nE * sizeof(double), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(dslot, &slot[0][0][0], 36 * sizeof(int), hipMemcpyHostToDevice));
+    ctx->cond_ready = true;
+  }
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(dcs, cs.data(), ncell * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  DeviceOperator& d = ctx->dop;
+  a.dx = ddx; a.dy = ddy; a.dz = ddz; a.delX = dlx; a.delY = dly; a.delZ = dlz;
+  a.csig = dcs; a.imap = d.imap; a.coefCC = dcc; a.slot = dslot;
+  a.sigE = dsE; a.earthE = dearth; a.sigN = dsN; a.m2 = dm2;
+  a.coef = d.coef64; a.vms1 = d.vms64;
+  k_cond_edges<<<2048, 256, 0, ctx->stream>>>(a);
+  k_cond_nodes<<<1024, 256, 0, ctx->stream>>>(a);
+  k_cond_rows<<<2048, 256, 0, ctx->stream>>>(a);
+  const int64_t nc = int64_t(kSlots) * g.nE;
+  k_cast_array<float><<<4096, 256, 0, ctx->stream>>>(d.coef64, d.coef32, nc);
+  k_cast_array<float><<<1024, 256, 0, ctx->stream>>>(d.vms64, d.vms32, g.nE);
+  k_rows_from_slots<double><<<2048, 256, 0, ctx->stream>>>(d.coef64, d.vms64, g.nE, d.row64);
+  k_rows_from_slots<float><<<2048, 256, 0, ctx->stream>>>(d.coef32, d.vms32, g.nE, d.row32);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  // what was derived from the previous model
+  hipFree(d.coefT64); hipFree(d.coefT32); hipFree(d.rowT64); hipFree(d.rowT32);
+  d.coefT64 = nullptr; d.coefT32 = nullptr; d.rowT64 = nullptr; d.rowT32 = nullptr;
+  hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
+  hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
+  ctx->dc = DivcorData();
+  ctx->nnz_stencil = -1;
+  H.logcond.assign(logcond, logcond + int64_t(nx) * ny * H.nzEarth);
+  H.airLog = air_logcond;
+  ctx->host_stale = true;
+  return MT_OK;
 }

@@ -66,6 +66,7 @@ _SIGS = {
     "mt_set_grid": (_I, [_P, _I, _I, _I]),
     "mt_operator_set_csr": (_I, [_P, _I, _I, _PI, _PI, _I, _PD, _PD]),
     "mt_assemble": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _PD, _D]),
+    "mt_update_cond": (_I, [_P, _PD, _D]),
     "mt_host_assemble": (_I, [_I, _I, _I, _I, _PD, _PD, _PD, _PD, _D, C.POINTER(_I),
                               C.POINTER(_I), _P, _P, _P, _P]),
     "mt_host_dilu_pivots": (_I, [_I, _I, _I, _I, _I, _PI, _PI, _I, _PD, _PD, _D, _PZ]),
@@ -245,7 +246,15 @@ class Context:
                                     np.ascontiguousarray(lc.ravel(order="F")), float(air_log)))
         self._sizes()
 
+    def update_cond(self, logcond, air_log=np.log(1e-10)):
+        """New LOGE model (nx, ny, nzEarth) on the assembled grid, on the GPU."""
+        lc = np.ascontiguousarray(np.asarray(logcond, np.float64).ravel(order="F"))
+        self._chk(lib().mt_update_cond(self._h, lc, float(air_log)))
+        self.nnz = None   # the stored pattern may change; counted again when asked
+
     def operator_csr(self):
+        if self.nnz is None:
+            self._sizes()
         row = np.zeros(self.n_int + 1, np.int32)
         col = np.zeros(self.nnz, np.int32)
         val = np.zeros(self.nnz)
