@@ -73,6 +73,22 @@ module mtsp2_fortran
          integer(c_int), intent(in) :: rowptr(*), col(*)
          real(c_double), intent(in) :: val(*), vms1(*)
       end function
+      integer(c_int) function mt_assemble(ctx, nx, ny, nzAir, nzEarth, dx, dy, dz, logcond, &
+            air_logcond) bind(C, name='mt_assemble')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nx, ny, nzAir, nzEarth
+         real(c_double), intent(in) :: dx(*), dy(*), dz(*), logcond(*)
+         real(c_double), value :: air_logcond
+      end function
+      ! updateCond (ForwardSolver.f90:261) on the device for the assembled grid
+      integer(c_int) function mt_update_cond(ctx, logcond, air_logcond) &
+            bind(C, name='mt_update_cond')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(in) :: logcond(*)
+         real(c_double), value :: air_logcond
+      end function
       integer(c_int) function mt_bicg(ctx, omega, b, x, maxit, tol, precond, niter, rerr, failed) &
             bind(C, name='mt_bicg')
          import :: c_int, c_ptr, c_double, c_double_complex
