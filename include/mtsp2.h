@@ -42,6 +42,8 @@ typedef double _Complex mt_zdouble;
 #define MT_E_MAXIT 6      /* not converged within maxit (best iterate returned)*/
 #define MT_E_STATE 7      /* operator / preconditioner not set                 */
 #define MT_E_STENCIL 8    /* CSR does not fit the staggered-grid stencil       */
+#define MT_E_PIVOT 9      /* a 2-D boundary slice where the reference's banded LU
+                             would interchange rows (mt_bc_x0_dev)              */
 
 /* ---- preconditioners and precisions ---------------------------------- */
 #define MT_PC_DILU 0      /* block diagonal-ILU, the reference's PC_setup      */
@@ -189,6 +191,20 @@ int mt_rhs_fwd(mt_ctx *ctx, const mt_zdouble *e_all, mt_zdouble *b);
  * (EMsolve3D.f90:217-274); b_dev, x0_dev: [nsys][n_int] in EDGEi order, ready
  * for mt_solve_batch_dev.  Requires mt_assemble. */
 int mt_rhs_fwd_dev(mt_ctx *ctx, int nsys, const void *e_dev, void *b_dev, void *x0_dev);
+
+/* BC_x0_WS (boundary_ws.f90:21-171, called through ComputeBC,
+ * FWD_SP2/modelOperator3D.f90:292-317, from fwdSetup, ForwardSolver.f90:
+ * 333-462) on the device, for a batch: the boundary values and initial guess
+ * of nsys forward systems (omega_s, mode_s; mode 1 = Ey polarisation, one
+ * 2-D TE problem per y-slice, mode 2 = Ex, one per x-slice) from the model
+ * of the last mt_assemble / mt_update_cond.  e_dev ([nsys][n_edge] complex,
+ * device) receives the field FWDsolve3D starts from: E0 on the interior
+ * edges and the boundary conditions on the boundary edges (E0 there, with
+ * the bottom-face x and y edges zero), ready for mt_rhs_fwd_dev.  Equal to
+ * the reference's fields bit for bit while its partial pivoting keeps the
+ * diagonal; MT_E_PIVOT (fields still written) where it would not.
+ * Needs nz <= about 100 (the slice window lives in LDS). */
+int mt_bc_x0_dev(mt_ctx *ctx, int nsys, const double *omegas, const int *modes, void *e_dev);
 /* Transposed (adjoint) RHS b = s_i + V_E GDii(s_i (i*ISIGN*omega*mu0)^-1 / V_E). */
 int mt_rhs_trn(mt_ctx *ctx, double omega, const mt_zdouble *s_all, mt_zdouble *b);
 /* TRN post-processing: e(EDGEi)=ei, e = V_E e, e(EDGEb) = -CCib^T ei. */

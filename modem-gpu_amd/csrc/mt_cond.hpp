@@ -61,7 +61,7 @@ __device__ __forceinline__ bool cond_interior(const CondArgs& a, int c, int i, i
 // sigma_E, earth class, vms1 of every edge
 __global__ void k_cond_edges(CondArgs a) {
 #pragma clang fp contract(off)
-  const double thr = double(1.1f) * 1e-10;
+  const double thr = double(1.1f) * kSigmaAir;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < a.nE;
        e += (int64_t)gridDim.x * blockDim.x) {
     int c, i, j, k;

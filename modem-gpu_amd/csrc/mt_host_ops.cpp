@@ -181,7 +181,7 @@ Status assemble(HostOperator& op, int nx, int ny, int nzAir, int nzEarth, const 
   for (int e : op.bmap) sigE[e] = 0.0;
 
   // --- air / earth node classes (airNIndex, modelOperator3D.f90:1086-1130)
-  const double thr = double(1.1f) * 1e-10;
+  const double thr = double(1.1f) * kSigmaAir;
   std::vector<char> eearth(nE, 0);
   for (int64_t e = 0; e < nE; ++e) eearth[e] = (sigE[e] != 0.0) && !(sigE[e] < thr);
   auto node_inner = [&](int i, int j, int k) {

@@ -37,6 +37,8 @@ namespace mt {
 constexpr int kSlots = 15;
 constexpr double kPi = 3.14159265357898;  // the reference's constant (math_constants.f90:12)
 constexpr double kMu0 = kPi * 4e-7;
+// SIGMA_AIR (math_constants.f90:19) is a default-real literal widened to real(8)
+constexpr double kSigmaAir = double(1.0e-10f);
 constexpr int kIsign = -1;
 
 struct Grid {

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "mt_bc2d.hpp"
 #include "mt_cond.hpp"
 #include "mt_datafunc.hpp"
 #include "mt_dilu.hpp"
@@ -1498,6 +1499,104 @@ int mt_rhs_fwd_dev(mt_ctx* ctx, int nsys, const void* e_dev, void* b_dev, void* 
       nsys, H.g.nInt, H.g.nE, drow, dcol, dval, ctx->dop.imap, (const Z*)e_dev, (Z*)b_dev, (Z*)x0_dev);
   MT_HIP_CHECK(ctx, hipGetLastError());
   MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_bc_x0_dev(mt_ctx* ctx, int nsys, const double* omegas, const int* modes, void* e_dev) {
+  if (!ctx || !omegas || !modes || !e_dev || nsys < 1)
+    return mt_fail(ctx, MT_E_ARG, "mt_bc_x0_dev: bad arguments");
+  const HostOperator& H = ctx->hop;
+  if (!H.have_model) return mt_fail(ctx, MT_E_STATE, "mt_bc_x0_dev needs mt_assemble");
+  const Grid& g = H.g;
+  const int nx = g.nx, ny = g.ny, nz = g.nz, nza = H.nzAir;
+  if (nx < 2 || ny < 2 || nz < 3 || H.nzEarth < 1)
+    return mt_fail(ctx, MT_E_ARG, "mt_bc_x0_dev: the grid needs nx, ny >= 2 and nz >= 3");
+  for (int s = 0; s < nsys; ++s)
+    if ((modes[s] != 1 && modes[s] != 2) || !(omegas[s] > 0.0))
+      return mt_fail(ctx, MT_E_ARG, "mt_bc_x0_dev: mode must be 1 or 2 and omega > 0");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const size_t lds = bc_lds_bytes(nz);
+  int maxlds = 0;
+  MT_HIP_CHECK(ctx, hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, ctx->device));
+  if (lds + 64 > size_t(maxlds))
+    return mt_fail(ctx, MT_E_ARG, "mt_bc_x0_dev: nz too large for the slice window in LDS");
+  // host side: earth conductivities as ModelParamToCell forms them, block distances, items
+  const int64_t nce = int64_t(nx) * ny * H.nzEarth;
+  std::vector<double> sig(nce);
+  for (int64_t q = 0; q < nce; ++q) sig[q] = std::exp(H.logcond[q]);
+  auto cdist = [](const std::vector<double>& d) {
+    const int n = int(d.size());
+    std::vector<double> c(n + 1);
+    for (int i = 1; i < n; ++i) c[i] = d[i] + d[i - 1];
+    c[0] = d[0];
+    c[n] = d[n - 1];
+    return c;
+  };
+  const std::vector<double> cx = cdist(H.dx), cy = cdist(H.dy), cz = cdist(H.dz);
+  std::vector<int2> items;
+  for (int s = 0; s < nsys; ++s) {
+    const int nsl = modes[s] == 2 ? nx : ny;
+    for (int k = 1; k <= nsl; ++k) items.push_back(make_int2(s, k));
+  }
+  const int nitem = int(items.size());
+  const int nhmax = std::max(nx, ny), b = nz - 1;
+  const int64_t nmax = int64_t(nhmax - 1) * b;
+  const int per_cu = 2 * (lds + 64) <= size_t(160 * 1024) ? 2 : 1;   // workgroups per CU by LDS (160 KB)
+  const int G = std::max(1, std::min(nitem, ctx->num_cu * per_cu));
+  BcArgs a;
+  a.nx = nx; a.ny = ny; a.nz = nz; a.nza = nza;
+  a.nXe = g.nXe; a.nE = g.nE;
+  a.nitem = nitem;
+  a.ncolMax = nhmax + 1;
+  a.mmbMax = 2 * nhmax + 2 * nz + 1;
+  a.uStride = nmax * nz;
+  a.yStride = nmax;
+  double *ddx, *ddy, *ddz, *dcx, *dcy, *dcz, *dsig, *dom;
+  int *dmode, *dflag;
+  int2* ditem;
+  double2 *dexb, *dside, *du, *dy;
+  int rc;
+  if ((rc = ws(ctx, "bc_dx", nx, &ddx)) || (rc = ws(ctx, "bc_dy", ny, &ddy)) ||
+      (rc = ws(ctx, "bc_dz", nz, &ddz)) || (rc = ws(ctx, "bc_cx", nx + 1, &dcx)) ||
+      (rc = ws(ctx, "bc_cy", ny + 1, &dcy)) || (rc = ws(ctx, "bc_cz", nz + 1, &dcz)) ||
+      (rc = ws(ctx, "bc_sig", nce, &dsig)) || (rc = ws(ctx, "bc_om", nsys, &dom)) ||
+      (rc = ws(ctx, "bc_mode", nsys, &dmode)) || (rc = ws(ctx, "bc_item", nitem, &ditem)) ||
+      (rc = ws(ctx, "bc_flag", nitem, &dflag)) ||
+      (rc = ws(ctx, "bc_exb", size_t(nitem) * a.mmbMax, &dexb)) ||
+      (rc = ws(ctx, "bc_side", size_t(nitem) * 4 * nz, &dside)) ||
+      (rc = ws(ctx, "bc_u", size_t(G) * a.uStride, &du)) ||
+      (rc = ws(ctx, "bc_y", size_t(G) * a.yStride, &dy)))
+    return rc;
+  MT_HIP_CHECK(ctx, hipMemcpy(ddx, H.dx.data(), nx * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(ddy, H.dy.data(), ny * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(ddz, H.dz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dcx, cx.data(), (nx + 1) * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dcy, cy.data(), (ny + 1) * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dcz, cz.data(), (nz + 1) * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dsig, sig.data(), nce * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dom, omegas, nsys * sizeof(double), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(dmode, modes, nsys * sizeof(int), hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpy(ditem, items.data(), nitem * sizeof(int2), hipMemcpyHostToDevice));
+  a.dx = ddx; a.dy = ddy; a.dz = ddz; a.cx = dcx; a.cy = dcy; a.cz = dcz;
+  a.sig = dsig; a.omega = dom; a.mode = dmode; a.item = ditem;
+  a.exb = dexb; a.side = dside; a.ucol = du; a.yx = dy; a.flag = dflag;
+  a.e = static_cast<double2*>(e_dev);
+  MT_HIP_CHECK(ctx, hipMemsetAsync(e_dev, 0, size_t(nsys) * g.nE * sizeof(double2), ctx->stream));
+  const int64_t n1 = int64_t(nitem) * a.ncolMax;
+  k_bc_te1d<<<unsigned((n1 + 255) / 256), 256, 0, ctx->stream>>>(a);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bc_te2d),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  k_bc_te2d<<<unsigned(G), 256, lds, ctx->stream>>>(a);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  std::vector<int> flags(nitem);
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(flags.data(), dflag, nitem * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < nitem; ++k)
+    if (flags[k])
+      return mt_fail(ctx, MT_E_PIVOT, "mt_bc_x0_dev: a 2-D slice needs row interchanges (system " +
+                                          std::to_string(items[k].x) + ", slice " +
+                                          std::to_string(items[k].y) + "); fields written without them");
   return MT_OK;
 }
 

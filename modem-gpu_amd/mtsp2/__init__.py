@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "lib", "libmtsp2.so")
 
 MT_OK, MT_E_ARG, MT_E_DEVICE, MT_E_OOM, MT_E_NAN_RHS = 0, 1, 2, 3, 4
-MT_E_BREAKDOWN, MT_E_MAXIT, MT_E_STATE, MT_E_STENCIL = 5, 6, 7, 8
+MT_E_BREAKDOWN, MT_E_MAXIT, MT_E_STATE, MT_E_STENCIL, MT_E_PIVOT = 5, 6, 7, 8, 9
 PC_DILU, PC_JACOBI, PC_NONE, PC_DILU_RB = 0, 1, 2, 3
 PREC_FP64, PREC_MIXED = 0, 1
 KT_SPMV, KT_PRECOND = 0, 1
@@ -82,6 +82,7 @@ _SIGS = {
     "mt_precond_apply": (_I, [_P, _D, _I, _PZ, _PZ]),
     "mt_rhs_fwd": (_I, [_P, _PZ, _PZ]),
     "mt_rhs_fwd_dev": (_I, [_P, _I, _P, _P, _P]),
+    "mt_bc_x0_dev": (_I, [_P, _I, _PD, _PI, _P]),
     "mt_rhs_trn": (_I, [_P, _D, _PZ, _PZ]),
     "mt_post_trn": (_I, [_P, _PZ, _PZ]),
     "mt_divcor": (_I, [_P, _D, _PZ, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
@@ -337,6 +338,28 @@ class Context:
         """b = -CCib e_b (and x0 = e(EDGEi)) for nsys device all-edge fields."""
         self._chk(lib().mt_rhs_fwd_dev(self._h, int(nsys), C.c_void_p(e_ptr), C.c_void_p(b_ptr),
                                        None if x0_ptr is None else C.c_void_p(x0_ptr)))
+
+    def bc_x0_dev(self, omegas, modes, e_ptr):
+        """BC_x0_WS on the device: the starting fields (E0 inside, boundary
+        conditions outside) of len(omegas) forward systems into e_ptr
+        ([nsys][n_edge] complex, device)."""
+        om = np.ascontiguousarray(omegas, np.float64)
+        md = np.ascontiguousarray(modes, np.int32)
+        if om.shape != md.shape:
+            raise ValueError("omegas and modes differ in length")
+        self._chk(lib().mt_bc_x0_dev(self._h, len(om), om, md, C.c_void_p(e_ptr)))
+
+    def bc_x0(self, omegas, modes):
+        """bc_x0_dev into a temporary device buffer, returned as (nsys, n_edge)."""
+        ns = len(omegas)
+        if self.n_edge is None:
+            self._sizes()
+        p = self.dev_alloc(ns * self.n_edge * 16)
+        try:
+            self.bc_x0_dev(omegas, modes, p)
+            return self.download(p, np.zeros((ns, self.n_edge), np.complex128))
+        finally:
+            self.dev_free(p)
 
     def rhs_trn(self, omega, s_all):
         b = np.zeros(self.n_int, np.complex128)

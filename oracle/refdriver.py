@@ -18,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_BIN = os.path.join(HERE, "_ref", "modem_ref")
 DF_BIN = os.path.join(HERE, "_ref", "ref_datafunc")
 DF_DROPIN_BIN = os.path.join(HERE, "_ref", "ref_dropin_datafunc")
+BCX0_BIN = os.path.join(HERE, "_ref", "ref_bcx0")
 
 
 def available() -> bool:
@@ -196,3 +197,38 @@ def run_datafunc(grid, origin, omega, dtype, x, e1, e2, d, r=None, az=None, exis
         out = out + (lresp, lcomb[:ne], lcomb[ne:], st)
     _cleanup(wd, True)
     return out
+
+
+def bcx0_available() -> bool:
+    return os.access(BCX0_BIN, os.X_OK)
+
+
+def run_bcx0(grid, logcond, air_log, systems, timeout=300):
+    """Runs the reference's ComputeBC -> BC_x0_WS (driver
+    oracle/ref/ref_bcx0.f90) for systems = [(omega, imode), ...].  Returns
+    (e0, bc): both (nsys, nEdge) complex, E0 and the boundary conditions as
+    setBC writes them into a zero edge vector."""
+    dx = np.asarray(grid["dx"], np.float64)
+    dy = np.asarray(grid["dy"], np.float64)
+    dz = np.asarray(grid["dz"], np.float64)
+    nx, ny, nz = len(dx), len(dy), len(dz)
+    nza = int(grid["nzAir"])
+    lc = np.asarray(logcond, np.float64).reshape(nx, ny, nz - nza)
+    wd = tempfile.mkdtemp(prefix="mtbc_")
+    inp = os.path.join(wd, "input.bin")
+    with open(inp, "wb") as f:
+        np.array([nx, ny, nza, nz - nza], np.int32).tofile(f)
+        dx.tofile(f); dy.tofile(f); dz.tofile(f)
+        lc.ravel(order="F").tofile(f)
+        np.array([air_log], np.float64).tofile(f)
+        np.array([len(systems)], np.int32).tofile(f)
+        for om, mode in systems:
+            np.array([om], np.float64).tofile(f)
+            np.array([mode], np.int32).tofile(f)
+    subprocess.run([BCX0_BIN, inp, wd], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    ns = len(systems)
+    e0 = np.fromfile(os.path.join(wd, "e0.bin"), np.complex128).reshape(ns, -1)
+    bc = np.fromfile(os.path.join(wd, "bc.bin"), np.complex128).reshape(ns, -1)
+    _cleanup(wd, True)
+    return e0, bc

@@ -34,7 +34,8 @@
 #define O_PI 3.14159265357898
 #define O_MU0 (O_PI * 4e-7)
 #define O_ISIGN (-1)
-#define O_SIGMA_AIR 1e-10
+/* SIGMA_AIR is a default-real literal widened to real(8) */
+#define O_SIGMA_AIR ((double)1.0e-10f)
 
 
 /* Complex division as the reference's flang build performs it: flang lowers
@@ -2030,4 +2031,296 @@ int sp2o_lmultT(const sp2o_problem *p, int dt, double omega, const double *origi
   free(S);
   o_coords_free(&co);
   return nc;
+}
+
+/* ================================================================ 2-D TE boundary fields
+ * BC_x0_WS (3D_MT/FWD/boundary_ws.f90:21-171): the boundary values and the
+ * initial guess of a forward solve from one 2-D TE problem per grid slice,
+ * solved by the Siripunvaraporn finite-difference code (3D_MT/FWD/Mod2d/).
+ * Mode 1 (Ey polarisation): one problem per y-slice over (x, z), written to
+ * the y-edges; mode 2 (Ex): one per x-slice over (y, z), written to the
+ * x-edges.  All other edges are zero.
+ *
+ * The dense banded algebra is LAPACK's ZGBTRF/ZGBTRS, which the reference
+ * links as a library; the vendored copy it ships (LAPACK/wsLAPACK.f90,
+ * LAPACK/wsBLAS.f90) takes the unblocked ZGBTF2 path when KU <= 64 (ILAENV,
+ * wsLAPACK.f90:5298-5311), i.e. for every 1-D solve and for 2-D slices with
+ * nz <= 65.  The restatement below follows the published unblocked
+ * algorithm (partial pivoting on |re|+|im|, first maximum) and the reference
+ * BLAS operation order, so on those sizes it reproduces the vendored LAPACK
+ * bit for bit; on larger nz the blocked factorisation rounds differently. */
+
+static inline int o_imin(int a, int b) { return a < b ? a : b; }
+static inline int o_imax(int a, int b) { return a > b ? a : b; }
+static inline zdouble o_zmul(zdouble a, zdouble b) { /* textbook product, no contraction */
+  return CMPLX(creal(a) * creal(b) - cimag(a) * cimag(b), creal(a) * cimag(b) + cimag(a) * creal(b));
+}
+static inline double o_cabs1(zdouble z) { return fabs(creal(z)) + fabs(cimag(z)); }
+
+/* Banded LU with partial pivoting, unblocked (the ZGBTF2 algorithm).  ab is
+ * column-major band storage with leading dimension ldab >= 2*kl+ku+1:
+ * A(i,j) (1-based) sits at row kl+ku+1+i-j of column j.  Returns the first
+ * zero pivot (0 if none); *nswap counts row interchanges. */
+static int o_gbtf2(int m, int n, int kl, int ku, zdouble *ab, int ldab, int *ipiv, int *nswap) {
+#define AB(i, j) ab[(size_t)((j) - 1) * ldab + ((i) - 1)]
+  const int kv = ku + kl;
+  int info = 0;
+  for (int j = ku + 2; j <= o_imin(kv, n); ++j)
+    for (int i = kv - j + 2; i <= kl; ++i) AB(i, j) = 0;
+  int ju = 1;
+  for (int j = 1; j <= o_imin(m, n); ++j) {
+    if (j + kv <= n)
+      for (int i = 1; i <= kl; ++i) AB(i, j + kv) = 0;
+    const int km = o_imin(kl, m - j);
+    int jp = 1;
+    double smax = o_cabs1(AB(kv + 1, j));
+    for (int i = 2; i <= km + 1; ++i) {
+      const double v = o_cabs1(AB(kv + i, j));
+      if (!(v <= smax)) { jp = i; smax = v; }
+    }
+    ipiv[j - 1] = jp + j - 1;
+    if (AB(kv + jp, j) != 0) {
+      ju = o_imax(ju, o_imin(j + ku + jp - 1, n));
+      if (jp != 1) {
+        /* swap matrix rows j and j+jp-1 over columns j..ju (band stride ldab-1) */
+        zdouble *a = &AB(kv + jp, j), *b = &AB(kv + 1, j);
+        for (int q = 0; q < ju - j + 1; ++q) {
+          const zdouble t = a[(size_t)q * (ldab - 1)];
+          a[(size_t)q * (ldab - 1)] = b[(size_t)q * (ldab - 1)];
+          b[(size_t)q * (ldab - 1)] = t;
+        }
+        ++*nswap;
+      }
+      if (km > 0) {
+        const zdouble r = zdiv(CMPLX(1.0, 0.0), AB(kv + 1, j));
+        for (int i = 1; i <= km; ++i) AB(kv + 1 + i, j) = o_zmul(r, AB(kv + 1 + i, j));
+        if (ju > j) {
+          /* rank-1 update of the trailing band: A(j+i, j+c) -= l(i) u(c) */
+          const zdouble *x = &AB(kv + 2, j), *y = &AB(kv, j + 1);
+          zdouble *A = &AB(kv + 1, j + 1);
+          for (int c = 0; c < ju - j; ++c) {
+            const zdouble yc = y[(size_t)c * (ldab - 1)];
+            if (yc != 0) {
+              const zdouble t = o_zmul(CMPLX(-1.0, 0.0), yc);
+              for (int i = 0; i < km; ++i)
+                A[(size_t)c * (ldab - 1) + i] = A[(size_t)c * (ldab - 1) + i] + o_zmul(x[i], t);
+            }
+          }
+        }
+      }
+    } else if (info == 0) {
+      info = j;
+    }
+  }
+  return info;
+}
+
+/* Solve A x = b with the factors of o_gbtf2 (the ZGBTRS 'N' algorithm with
+ * one right-hand side: L with the interchanges, then the banded triangular
+ * solve of U, ZTBSV order). */
+static void o_gbtrs(int n, int kl, int ku, const zdouble *ab, int ldab, const int *ipiv, zdouble *b) {
+  const int kd = ku + kl + 1;
+  if (kl > 0)
+    for (int j = 1; j <= n - 1; ++j) {
+      const int lm = o_imin(kl, n - j), l = ipiv[j - 1];
+      if (l != j) { const zdouble t = b[l - 1]; b[l - 1] = b[j - 1]; b[j - 1] = t; }
+      if (b[j - 1] != 0) {
+        const zdouble t = o_zmul(CMPLX(-1.0, 0.0), b[j - 1]);
+        for (int i = 1; i <= lm; ++i) b[j - 1 + i] = b[j - 1 + i] + o_zmul(AB(kd + i, j), t);
+      }
+    }
+  const int k = kl + ku, kp1 = k + 1;
+  for (int j = n; j >= 1; --j)
+    if (b[j - 1] != 0) {
+      const int l = kp1 - j;
+      b[j - 1] = zdiv(b[j - 1], AB(kp1, j));
+      const zdouble t = b[j - 1];
+      for (int i = j - 1; i >= o_imax(1, j - k); --i) b[i - 1] = b[i - 1] - o_zmul(t, AB(l + i, j));
+    }
+#undef AB
+}
+
+/* DistanceBetweenBlocks (WSutils.f90:182-193): c(1) = d(1), c(i) = d(i) + d(i-1),
+ * c(n+1) = d(n).  c has n+1 entries. */
+static void ws_cdist(int n, const double *d, double *c) {
+  for (int i = 2; i <= n; ++i) c[i - 1] = d[i - 1] + d[i - 2];
+  c[0] = d[0];
+  c[n] = d[n - 1];
+}
+
+/* Fwd1D_TE (WSfwd1Dmod.f90:79-141) with Solve1D (:145-166): the 1-D TE field
+ * of a layered column, 1 at the top node, over nz layers plus one layer of
+ * the bottom skin depth.  s1: the nz-nza earth conductivities; x1d: nz+1. */
+static void ws_fwd1d_te(double per, int nza, int nz, const double *dz, const double *s1, zdouble *x1d,
+                        int *nswap) {
+  const double omega = (2.0 * O_PI) / per, omue = omega * O_MU0;
+  double *s1d = XALLOC(double, nz + 1), *dz1 = XALLOC(double, nz + 1);
+  for (int iz = 1; iz <= nza; ++iz) s1d[iz - 1] = O_SIGMA_AIR;
+  for (int iz = nza + 1; iz <= nz; ++iz) s1d[iz - 1] = s1[iz - nza - 1];
+  s1d[nz] = s1d[nz - 1];
+  const double skd = sqrt(2.0 / (s1d[nz] * omue));
+  for (int iz = 1; iz <= nz; ++iz) dz1[iz - 1] = dz[iz - 1];
+  dz1[nz] = skd;
+  /* tridiagonal in band storage (kl = ku = 1, ldab = 4) */
+  zdouble *a = XALLOC(zdouble, 4 * (size_t)nz), *xb = XALLOC(zdouble, nz);
+  double *ad = XALLOC(double, nz);
+  double au1 = 0.0;
+  for (int jj = 1; jj <= nz; ++jj) {
+    const int iz = jj + 1;
+    const double au = 2.0 / dz1[iz - 2], adv = 2.0 / dz1[iz - 1];
+    const double ss = s1d[iz - 1] * dz1[iz - 1] + s1d[iz - 2] * dz1[iz - 2];
+    ad[jj - 1] = adv;
+    a[(size_t)(jj - 1) * 4 + 2] = CMPLX((0.0 - au) - adv, omue * ss);
+    if (jj == 1) au1 = au;
+  }
+  for (int jj = 2; jj <= nz; ++jj) a[(size_t)(jj - 1) * 4 + 1] = ad[jj - 2];
+  for (int jj = 1; jj <= nz - 1; ++jj) a[(size_t)(jj - 1) * 4 + 3] = ad[jj - 1];
+  xb[0] = CMPLX(-au1, 0.0);
+  int *ipiv = XALLOC(int, nz);
+  o_gbtf2(nz, nz, 1, 1, a, 4, ipiv, nswap);
+  o_gbtrs(nz, 1, 1, a, 4, ipiv, xb);
+  x1d[0] = 1.0;
+  for (int i = 0; i < nz; ++i) x1d[i + 1] = xb[i];
+  free(s1d); free(dz1); free(a); free(xb); free(ad); free(ipiv);
+}
+
+/* One 2-D TE problem (FwdTEmod.f90 Fwd2DsetupTE / UpdateCondTE / UpdateFreqTE /
+ * SetBoundTE / Fwd2DsolveTE, with SetupA_TE, FormAII, SetBound2D_TE and
+ * MulAibWithXb of WSfwd2Dmod.f90).  ny horizontal cells dy, nz vertical cells
+ * dz of which nza are air; cond(ih, iz) at cond[(ih-1) + ny*(iz-1)] (the air
+ * rows are not read: the 2-D code uses SIGMA_AIR there).  esol((ny+1), (nz+1))
+ * at esol[(iy-1) + (ny+1)*(iz-1)]. */
+static void ws_te2d(double per, int ny, int nz, int nza, const double *dy, const double *dz,
+                    const double *cond, zdouble *esol, int *nswap) {
+#define CC(iz, iy) ccon[(size_t)((iy) - 1) * nz + ((iz) - 1)]
+  double *cz = XALLOC(double, nz + 1), *cy = XALLOC(double, ny + 1);
+  ws_cdist(nz, dz, cz);
+  ws_cdist(ny, dy, cy);
+  /* SetupA_TE (WSfwd2Dmod.f90:345-438) */
+  double *ccon = XALLOC(double, (size_t)nz * ny);
+  for (int iy = 1; iy <= ny; ++iy)
+    for (int iz = 1; iz <= nz; ++iz)
+      CC(iz, iy) = iz <= nza ? O_SIGMA_AIR : cond[(size_t)(iy - 1) + (size_t)ny * (iz - 1)];
+  const int mmi = (ny - 1) * (nz - 1), mmb = 2 * ny + 2 * nz;
+  double *a1 = XALLOC(double, mmi), *a2 = XALLOC(double, mmi), *a3 = XALLOC(double, mmi),
+         *a4 = XALLOC(double, mmi), *bte = XALLOC(double, mmb + 1);
+  int jj = 0;
+  for (int iy = 2; iy <= ny; ++iy)
+    for (int iz = 2; iz <= nz; ++iz, ++jj) {
+      const double s00 = CC(iz, iy), s10 = CC(iz - 1, iy), s01 = CC(iz, iy - 1), s11 = CC(iz - 1, iy - 1);
+      const double s2d = (s00 * dz[iz - 1] * dy[iy - 1] + s01 * dz[iz - 1] * dy[iy - 2] +
+                          s10 * dz[iz - 2] * dy[iy - 1] + s11 * dz[iz - 2] * dy[iy - 2]) /
+                         (dz[iz - 1] * dy[iy - 1] + dz[iz - 1] * dy[iy - 2] + dz[iz - 2] * dy[iy - 1] +
+                          dz[iz - 2] * dy[iy - 2]);
+      const double ar = 2.0 * cz[iz - 1] / dy[iy - 1], al = 2.0 * cz[iz - 1] / dy[iy - 2];
+      const double adv = 2.0 * cy[iy - 1] / dz[iz - 1], au = 2.0 * cy[iy - 1] / dz[iz - 2];
+      a1[jj] = -ar - al - adv - au;
+      a4[jj] = cy[iy - 1] * cz[iz - 1] * s2d;
+      a2[jj] = iz == nz ? 0.0 : adv;
+      a3[jj] = iy < ny ? ar : 0.0;
+    }
+  /* boundary couplings, 1-based positions as in the reference */
+  for (int iz = 2; iz <= nz; ++iz) bte[iz] = 2.0 * cz[iz - 1] / dy[0];
+  for (int iy = 2; iy <= ny; ++iy) bte[nz + iy] = 2.0 * cy[iy - 1] / dz[0];
+  for (int iy = 2; iy <= ny; ++iy) bte[nz + ny + iy - 1] = 2.0 * cy[iy - 1] / dz[nz - 1];
+  for (int iz = 2; iz <= nz; ++iz) bte[nz + 2 * ny - 1 + iz] = 2.0 * cz[iz - 1] / dy[ny - 1];
+  /* FormAII (WSfwd2Dmod.f90:31-85): band storage and factorisation */
+  const double omega = (2.0 * O_PI) / per, omue = omega * O_MU0;
+  const int kl = nz - 1, ku = nz - 1, kc = kl + ku + 1, ldab = 2 * kl + ku + 1;
+  zdouble *aii = XALLOC(zdouble, (size_t)ldab * mmi);
+#define AII(i, j) aii[(size_t)((j) - 1) * ldab + ((i) - 1)]
+  for (int j = 1; j <= mmi; ++j) AII(kc, j) = CMPLX(a1[j - 1], omue * a4[j - 1]);
+  for (int j = 1; j <= mmi - 1; ++j) { AII(kc - 1, j + 1) = a2[j - 1]; AII(kc + 1, j) = a2[j - 1]; }
+  for (int j = 1; j <= mmi - (nz - 1); ++j) {
+    AII(kc - (nz - 1), j + (nz - 1)) = a3[j - 1];
+    AII(kc + (nz - 1), j) = a3[j - 1];
+  }
+#undef AII
+  int *ipiv = XALLOC(int, mmi);
+  o_gbtf2(mmi, mmi, kl, ku, aii, ldab, ipiv, nswap);
+  /* SetBound2D_TE (WSfwd2Dmod.f90:262-341): 1-D columns at the sides and,
+   * dy-averaged, under every interior node of the bottom */
+  zdouble *exb = XALLOC(zdouble, mmb + 1), *x1d = XALLOC(zdouble, nz + 1);
+  double *s1 = XALLOC(double, nz);
+  const int ne = nz - nza;
+  for (int iz = 1; iz <= ne; ++iz) s1[iz - 1] = CC(iz + nza, 1);
+  ws_fwd1d_te(per, nza, nz, dz, s1, x1d, nswap);
+  for (int i = 1; i <= nz + 1; ++i) exb[i] = x1d[i - 1];
+  for (int i = nz + 2; i <= nz + ny; ++i) exb[i] = 1.0;
+  for (int iy = 2; iy <= ny; ++iy) {
+    for (int iz = 1; iz <= ne; ++iz)
+      s1[iz - 1] = (CC(iz + nza, iy - 1) * dy[iy - 2] + CC(iz + nza, iy) * dy[iy - 1]) / (dy[iy - 2] + dy[iy - 1]);
+    ws_fwd1d_te(per, nza, nz, dz, s1, x1d, nswap);
+    exb[nz + ny - 1 + iy] = x1d[nz];
+  }
+  for (int iz = 1; iz <= ne; ++iz) s1[iz - 1] = CC(iz + nza, ny);
+  ws_fwd1d_te(per, nza, nz, dz, s1, x1d, nswap);
+  for (int i = 0; i <= nz; ++i) exb[nz + 2 * ny + i] = x1d[i];
+  /* MulAibWithXb (WSfwd2Dmod.f90:89-140): b = -A_ib x_b (the right side
+   * overwrites, the top and bottom accumulate) */
+  zdouble *exi = XALLOC(zdouble, mmi);
+#define RL(a, x) CMPLX((a) * creal(x), (a) * cimag(x))
+  for (int iz = 2; iz <= nz; ++iz) exi[iz - 2] = -RL(bte[iz], exb[iz]);
+  for (int iz = 2, izz = 2 * ny + nz + 1; iz <= nz; ++iz, ++izz)
+    exi[(ny - 2) * (nz - 1) + iz - 2] = -RL(bte[izz], exb[izz]);
+  for (int iy = 2, izz = nz + 2; iy <= ny; ++iy, ++izz) {
+    const int q = (iy - 2) * (nz - 1);
+    exi[q] = exi[q] - RL(bte[izz], exb[izz]);
+  }
+  for (int iy = 2, izz = nz + ny + 1; iy <= ny; ++iy, ++izz) {
+    const int q = (iy - 2) * (nz - 1) + nz - 2;
+    exi[q] = exi[q] - RL(bte[izz], exb[izz]);
+  }
+#undef RL
+  o_gbtrs(mmi, kl, ku, aii, ldab, ipiv, exi);
+  /* Fwd2DsolveTE (FwdTEmod.f90:110-169): interior, then the four sides */
+#define ES(iy, iz) esol[(size_t)((iy) - 1) + (size_t)(ny + 1) * ((iz) - 1)]
+  int is = 0;
+  for (int iy = 2; iy <= ny; ++iy)
+    for (int iz = 2; iz <= nz; ++iz) ES(iy, iz) = exi[is++];
+  for (int iz = 1; iz <= nz + 1; ++iz) ES(1, iz) = exb[iz];
+  for (int iy = 2; iy <= ny; ++iy) ES(iy, 1) = exb[iy + nz];
+  for (int iy = 2; iy <= ny; ++iy) ES(iy, nz + 1) = exb[iy + nz + ny - 1];
+  for (int iz = 1; iz <= nz + 1; ++iz) ES(ny + 1, iz) = exb[nz + 2 * ny - 1 + iz];
+#undef ES
+#undef CC
+  free(cz); free(cy); free(ccon); free(a1); free(a2); free(a3); free(a4); free(bte);
+  free(aii); free(ipiv); free(exb); free(x1d); free(s1); free(exi);
+}
+
+int sp2o_bc_x0(int nx, int ny, int nzAir, int nzEarth, const double *dx, const double *dy,
+               const double *dz, const double *logcond, double omega, int imode, zdouble *e0,
+               int *nswap) {
+  const int nz = nzAir + nzEarth;
+  const int nXe = nx * (ny + 1) * (nz + 1), nYe = (nx + 1) * ny * (nz + 1), nZe = (nx + 1) * (ny + 1) * nz;
+  memset(e0, 0, sizeof(zdouble) * (size_t)(nXe + nYe + nZe));
+  *nswap = 0;
+  if (imode != 1 && imode != 2) return -1;
+  /* ComputeBC (FWD_SP2/modelOperator3D.f90:292-317) passes the period 2*PI/omega */
+  const double per = (2.0 * O_PI) / omega;
+  const int nh = imode == 2 ? ny : nx, nslice = imode == 2 ? nx : ny;
+  const double *dh = imode == 2 ? dy : dx;
+  double *cond = XALLOC(double, (size_t)nh * nz);
+  zdouble *es = XALLOC(zdouble, (size_t)(nh + 1) * (nz + 1));
+  for (int s = 1; s <= nslice; ++s) {
+    /* sigma_C of the earth cells (ModelParamToCell, ModelMap.inc:26-66) */
+    for (int ih = 1; ih <= nh; ++ih)
+      for (int iz = nzAir + 1; iz <= nz; ++iz) {
+        const int i = imode == 2 ? s : ih, j = imode == 2 ? ih : s;
+        cond[(size_t)(ih - 1) + (size_t)nh * (iz - 1)] =
+            exp(logcond[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (iz - nzAir - 1))]);
+      }
+    ws_te2d(per, nh, nz, nzAir, dh, dz, cond, es, nswap);
+    for (int ih = 1; ih <= nh + 1; ++ih)
+      for (int iz = 1; iz <= nz + 1; ++iz) {
+        const zdouble v = es[(size_t)(ih - 1) + (size_t)(nh + 1) * (iz - 1)];
+        if (imode == 2) /* E0%x(iSlice, ih, iz) */
+          e0[(size_t)(s - 1) + (size_t)nx * ((ih - 1) + (size_t)(ny + 1) * (iz - 1))] = v;
+        else /* E0%y(ih, iSlice, iz) */
+          e0[(size_t)nXe + (size_t)(ih - 1) + (size_t)(nx + 1) * ((s - 1) + (size_t)ny * (iz - 1))] = v;
+      }
+  }
+  free(cond);
+  free(es);
+  return 0;
 }

@@ -108,4 +108,15 @@ int sp2o_lmultT(const sp2o_problem *p, int dt, double omega, const double *origi
                 const unsigned char *exist, const zdouble *e1, const zdouble *e2, zdouble *comb1,
                 zdouble *comb2);
 
+/* BC_x0_WS (3D_MT/FWD/boundary_ws.f90:21-171): the all-edge field E0 of the
+ * 2-D TE slice solves for mode 1 (Ey polarisation) or 2 (Ex) at angular
+ * frequency omega; the cell model is LOGE (air cells use SIGMA_AIR, as the
+ * 2-D code does).  nswap receives the number of row interchanges the banded
+ * LU made over all slices.  The forward problem's boundary data are E0 on
+ * the boundary edges with the bottom-face x and y edges set to zero
+ * (BC%xZMax = BC%yZMax = 0). */
+int sp2o_bc_x0(int nx, int ny, int nzAir, int nzEarth, const double *dx, const double *dy,
+               const double *dz, const double *logcond, double omega, int imode, zdouble *e0,
+               int *nswap);
+
 #endif

@@ -63,6 +63,8 @@ def lib():
             "sp2o_data_resp": (_I, [_P, _I, _D, _PD, _I, _PD, _P, _P, _PZ, _PZ, _PD]),
             "sp2o_lmultT": (_I, [_P, _I, _D, _PD, _I, _PD, _P, _P, _PD, _P, _PZ, _PZ,
                                  _PZ, _PZ]),
+            "sp2o_bc_x0": (_I, [_I, _I, _I, _I, _PD, _PD, _PD, _PD, _D, _I, _PZ,
+                                C.POINTER(_I)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -260,3 +262,32 @@ class Problem:
         if nc < 0:
             raise ValueError(f"unsupported data type {dtype}")
         return c1, c2
+
+
+def bc_x0(dx, dy, dz, nza, logcond, omega, imode):
+    """BC_x0_WS: the all-edge E0 of the 2-D TE slice solves for mode imode
+    (1: Ey polarisation, 2: Ex) and the number of pivot interchanges."""
+    dx, dy, dz = (np.ascontiguousarray(a, np.float64) for a in (dx, dy, dz))
+    nx, ny, nz = len(dx), len(dy), len(dz)
+    lc = np.ascontiguousarray(np.asarray(logcond, np.float64).reshape(nx, ny, nz - nza).ravel(order="F"))
+    ne = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
+    e0 = np.zeros(ne, np.complex128)
+    nsw = _I()
+    rc = lib().sp2o_bc_x0(nx, ny, nza, nz - nza, dx, dy, dz, lc, float(omega), int(imode), e0,
+                          C.byref(nsw))
+    if rc:
+        raise ValueError(f"bad mode {imode}")
+    return e0, nsw.value
+
+
+def bc_form(e0, nx, ny, nz):
+    """The field FWDsolve3D starts from: E0 with the bottom-face x and y edges
+    zeroed (BC_x0_WS sets BC%xZMax = BC%yZMax = 0 and setBC writes the z
+    faces last)."""
+    e = np.array(e0, np.complex128)
+    nXe = nx * (ny + 1) * (nz + 1)
+    ex = e[:nXe].reshape(nz + 1, ny + 1, nx)
+    ey = e[nXe:nXe + (nx + 1) * ny * (nz + 1)].reshape(nz + 1, ny, nx + 1)
+    ex[nz] = 0
+    ey[nz] = 0
+    return e

@@ -1,10 +1,13 @@
 """Writes tests/golden/ref_small.npz: what the reference's own CPU SP2 path
 (the flang build of the reference sources, oracle/ref/Makefile) computes on a
-small seeded grid, and tests/golden/ref_datafunc.npz: its dataResp and LmultT
-for every data type on the same grid.  Inputs and outputs only -- data, no
-reference code.
+small seeded grid, tests/golden/ref_datafunc.npz: its dataResp and LmultT
+for every data type on the same grid, and tests/golden/ref_bcx0.npz: the 2-D
+TE boundary fields of BC_x0_WS (through ComputeBC) on that grid and on a deep
+one whose bandwidth takes LAPACK's blocked factorisation.  Inputs and outputs
+only -- data, no reference code.
 
-Run where the reference is mounted:  python tests/golden/make_ref_fixtures.py
+Run where the reference is mounted:
+    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0]
 """
 import os
 import sys
@@ -97,6 +100,45 @@ def datafunc_main():
     print("wrote", os.path.join(HERE, "ref_datafunc.npz"))
 
 
+BC_SYSTEMS = [(2 * np.pi * 0.05, 1), (2 * np.pi * 0.05, 2), (2 * np.pi * 20.0, 1),
+              (2 * np.pi * 1e-3, 2)]
+
+
+def bcx0_deep_case():
+    """nz = 70 (10 air + 60 earth): bandwidth 69 > 64, LAPACK's blocked ZGBTRF."""
+    rng = np.random.default_rng(31)
+    nx, ny, nza, nze = 4, 3, 10, 60
+    dx = rng.uniform(200, 2000, nx)
+    dy = rng.uniform(200, 2000, ny)
+    dz = np.concatenate([1000.0 * 1.6 ** np.arange(nza)[::-1], 40.0 * 1.08 ** np.arange(nze)])
+    lc = np.log(rng.uniform(0.001, 1.0, (nx, ny, nze)))
+    return dx, dy, dz, nza, lc
+
+
+def bcx0_main():
+    _, dx, dy, dz, nza, lc = case()
+    air = np.log(1e-10)
+    e0, bc = refdriver.run_bcx0(dict(dx=dx, dy=dy, dz=dz, nzAir=nza), lc, air, BC_SYSTEMS)
+    out = dict(dx=dx, dy=dy, dz=dz, nzAir=nza, logcond=lc, air=air,
+               omega=np.array([w for w, _ in BC_SYSTEMS]), mode=np.array([m for _, m in BC_SYSTEMS]),
+               e0=e0, bc=bc)
+    ddx, ddy, ddz, dnza, dlc = bcx0_deep_case()
+    sy = [(2 * np.pi * 1e-2, 1), (2 * np.pi * 3.0, 2)]
+    de0, _ = refdriver.run_bcx0(dict(dx=ddx, dy=ddy, dz=ddz, nzAir=dnza), dlc, air, sy)
+    nx, ny, nz = len(ddx), len(ddy), len(ddz)
+    nxe, nye = nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1)
+    out.update(deep_dx=ddx, deep_dy=ddy, deep_dz=ddz, deep_nzAir=dnza, deep_logcond=dlc,
+               deep_omega=np.array([w for w, _ in sy]), deep_mode=np.array([m for _, m in sy]),
+               deep_ey=de0[0][nxe:nxe + nye], deep_ex=de0[1][:nxe])   # the only non-zero parts
+    np.savez_compressed(os.path.join(HERE, "ref_bcx0.npz"), **out)
+    print("wrote", os.path.join(HERE, "ref_bcx0.npz"))
+
+
 if __name__ == "__main__":
-    main()
-    datafunc_main()
+    which = sys.argv[1:] or ["small", "datafunc", "bcx0"]
+    if "small" in which:
+        main()
+    if "datafunc" in which:
+        datafunc_main()
+    if "bcx0" in which:
+        bcx0_main()
