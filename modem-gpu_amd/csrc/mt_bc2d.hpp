@@ -21,7 +21,9 @@
 // LAPACK form it (textbook products, C99 Annex G division as __divdc3 does
 // it, no contraction) and every entry receives its updates in the same
 // order, so the fields equal the reference's bit for bit while its partial
-// pivoting keeps the diagonal.  A column where it would interchange rows (or
+// pivoting keeps the diagonal.  (ZGERU's a + x*(-1*y) is formed as
+// a - x*y: negation is exact and round-to-nearest symmetric, so the two
+// agree in value and differ at most in the sign of a zero.)  A column where it would interchange rows (or
 // a zero pivot) is flagged, and the entry point reports it.
 #pragma once
 
@@ -256,9 +258,18 @@ __global__ void k_bc_te1d(BcArgs a) {
     for (int ii = Nz + 2; ii <= Nz + S.nh; ++ii) exb[ii] = make_double2(1.0, 0.0);   // the surface
 }
 
+// A workgroup barrier that orders LDS only: the factorisation's global
+// stores (rows of U, the eliminated rhs) are not read back until after a full
+// barrier, so the steps need not wait for them (__syncthreads would).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __host__ __device__ inline size_t bc_lds_bytes(int nz) {
   const size_t B1 = size_t(nz);
-  return (B1 * B1 + B1 + 2 * kBcChunk) * sizeof(double2) + 2 * kBcChunk * sizeof(double);
+  return (B1 * B1 + B1 + 2 * kBcChunk + 1) * sizeof(double2) + 2 * kBcChunk * sizeof(double);
 }
 
 // One slice per workgroup, persistent over the items.  Window slot of node r
@@ -274,10 +285,13 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
   double2* Er = Ed + kBcChunk;
   double* Ea2 = reinterpret_cast<double*>(Er + kBcChunk);
   double* Ea3 = Ea2 + kBcChunk;
+  double2* Rp = reinterpret_cast<double2*>(Ea3 + kBcChunk);   // 1 / next pivot
   const int tid = threadIdx.x, nt = blockDim.x;
   double2* U = a.ucol + (int64_t)blockIdx.x * a.uStride;
   double2* Y = a.yx + (int64_t)blockIdx.x * a.yStride;
-  const double2 zero = make_double2(0.0, 0.0), mone = make_double2(-1.0, 0.0);
+  // (selects below are written as assignments: a ?: between a local double2
+  // and a memory operand becomes a select of addresses, which keeps the
+  // local in scratch)
 
   for (int it = blockIdx.x; it < a.nitem; it += gridDim.x) {
     const BcSlice S = bc_slice(a, it);
@@ -285,7 +299,7 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
     const double2* exb = a.exb + (int64_t)it * a.mmbMax;
     if (tid == 0) sflag = 0;
     int myflag = 0;
-    for (int q = tid; q < B1 * B1; q += nt) W[q] = zero;
+    for (int q = tid; q < B1 * B1; q += nt) W[q] = make_double2(0.0, 0.0);
     __syncthreads();
     // the window of step 1: nodes 1..min(b, n)
     const int n0 = min(b, n);
@@ -321,7 +335,7 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
           }
         }
       }
-      __syncthreads();
+      lds_barrier();
       const int m = min(b, n - j), sj = j % B1, q = j + b, sq = q % B1;
       const double2 p = W[sj * B1 + sj];
       const double2 yv = Bv[sj];
@@ -331,15 +345,20 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
         const int i = tid;
         int si = sj + i;
         if (si >= B1) si -= B1;
-        const double2 ai = i == b ? a3j : W[sj * B1 + si];
+        double2 ai = a3j;
+        if (i != b) ai = W[sj * B1 + si];
         if (!(bz_abs1(ai) <= bz_abs1(p))) myflag = 1;
-        const double2 r = bz_div(make_double2(1.0, 0.0), p);
+        double2 r;
+        if (j == 1) r = bz_div(make_double2(1.0, 0.0), p);
+        else r = Rp[0];
         const double2 l = bz_mul(r, ai);
         W[sj * B1 + si] = l;
-        double2 base = i == b ? Er[cs] : Bv[si];
+        double2 base;
+        if (i == b) base = Er[cs];
+        else base = Bv[si];
         if (bz_nz(yv)) {
-          const double2 u = bz_mul(l, bz_mul(mone, yv));
-          base = make_double2(base.x + u.x, base.y + u.y);
+          const double2 u = bz_mul(l, yv);
+          base = make_double2(base.x - u.x, base.y - u.y);
         }
         Bv[si] = base;
       }
@@ -348,7 +367,10 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
         const int c = tid;
         int sc = sj + c;
         if (sc >= B1) sc -= B1;
-        const double2 u = c == 0 ? p : (c == b ? a3j : W[sc * B1 + sj]);
+        double2 u;
+        if (c == 0) u = p;
+        else if (c == b) u = a3j;
+        else u = W[sc * B1 + sj];
         U[(int64_t)(j + c - 1) * B1 + c] = u;
       }
       if (tid == 0) {
@@ -359,7 +381,7 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
         if (tid <= b) {
           // the entering column q: rows j..q
           const int r = j + tid;
-          double2 v = zero;
+          double2 v = make_double2(0.0, 0.0);
           if (r == j) v = a3j;
           else if (r == q - 1) v = make_double2(Ea2[cs], 0.0);
           else if (r == q) v = Ed[cs];
@@ -368,26 +390,49 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
         if (tid >= 1 && tid <= b - 1) {
           // the entering row q: columns j+1..q-1
           const int c = j + tid;
-          W[(c % B1) * B1 + sq] = c == q - 1 ? make_double2(Ea2[cs], 0.0) : zero;
+          W[(c % B1) * B1 + sq] = make_double2(c == q - 1 ? Ea2[cs] : 0.0, 0.0);
         }
       }
-      __syncthreads();
-      // rank-1 update of the trailing m x m block (ZGERU order per entry)
+      lds_barrier();
+      // rank-1 update of the trailing m x m block (ZGERU order per entry):
+      // thread (g, i) owns row i and every G-th column from 1+g, so its
+      // multiplier is read once; four columns are in flight at a time.  The
+      // owner of the next pivot (row 1, column 1: thread 0) also forms its
+      // reciprocal for the next step.
       if (m > 0) {
-        const int tot = m * m;
-        int i = 1 + tid % m, c = 1 + tid / m;
-        const int di = nt % m, dc = nt / m;
-        for (int e = tid; e < tot; e += nt) {
-          int sc = sj + c, si = sj + i;
-          if (sc >= B1) sc -= B1;
+        const int G = nt / m, g = tid / m, i = 1 + tid % m;
+        if (g < G) {
+          int si = sj + i;
           if (si >= B1) si -= B1;
-          const double2 t = bz_mul(mone, W[sc * B1 + sj]);
-          const double2 u = bz_mul(W[sj * B1 + si], t);
-          const double2 w = W[sc * B1 + si];
-          W[sc * B1 + si] = make_double2(w.x + u.x, w.y + u.y);
-          i += di;
-          c += dc;
-          if (i > m) { i -= m; ++c; }
+          const double2 l = W[sj * B1 + si];
+          int c = 1 + g;
+          for (; c + 3 * G <= m; c += 4 * G) {
+            int sc[4];
+            double2 y[4], w[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              sc[k] = sj + c + k * G;
+              if (sc[k] >= B1) sc[k] -= B1;
+              y[k] = W[sc[k] * B1 + sj];
+              w[k] = W[sc[k] * B1 + si];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+              const double2 u = bz_mul(l, y[k]);
+              w[k] = make_double2(w[k].x - u.x, w[k].y - u.y);
+              W[sc[k] * B1 + si] = w[k];
+            }
+            if (tid == 0 && c == 1) Rp[0] = bz_div(make_double2(1.0, 0.0), w[0]);
+          }
+          for (; c <= m; c += G) {
+            int sc = sj + c;
+            if (sc >= B1) sc -= B1;
+            const double2 u = bz_mul(l, W[sc * B1 + sj]);
+            const double2 w0 = W[sc * B1 + si];
+            const double2 wn = make_double2(w0.x - u.x, w0.y - u.y);
+            W[sc * B1 + si] = wn;
+            if (tid == 0 && c == 1) Rp[0] = bz_div(make_double2(1.0, 0.0), wn);
+          }
         }
       }
     }
@@ -403,11 +448,15 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
       // stage U columns j0..j1 into W[(jj-j0)*B1 + o] and the entering y_{jj-b} into W[b*B1 + jj-j0]
       for (int e = tid; e < cnt * B1; e += nt) {
         const int jj = j0 + e / B1, o = e % B1;
-        W[(jj - j0) * B1 + o] = o <= jj - 1 ? U[(int64_t)(jj - 1) * B1 + o] : zero;
+        double2 v = make_double2(0.0, 0.0);
+        if (o <= jj - 1) v = U[(int64_t)(jj - 1) * B1 + o];
+        W[(jj - j0) * B1 + o] = v;
       }
       for (int e = tid; e < cnt; e += nt) {
         const int jj = j0 + e;
-        W[b * B1 + e] = jj - b >= 1 ? Y[jj - b - 1] : zero;
+        double2 v = make_double2(0.0, 0.0);
+        if (jj - b >= 1) v = Y[jj - b - 1];
+        W[b * B1 + e] = v;
       }
       __syncthreads();
       if (tid < 64) {
@@ -415,11 +464,14 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
           const double2* Uc = W + (j - j0) * B1;
           const double2 yj = Bv[j % B1];
           const bool go = bz_nz(yj);
-          const double2 xj = go ? bz_div(yj, Uc[0]) : yj;
+          double2 xj = yj;
+          if (go) xj = bz_div(yj, Uc[0]);
           const int om = min(b, j - 1);
           for (int o = 1 + tid; o <= om; o += 64) {
             const int sr = (j - o) % B1;
-            double2 v = o == b ? W[b * B1 + (j - j0)] : Bv[sr];
+            double2 v;
+            if (o == b) v = W[b * B1 + (j - j0)];
+            else v = Bv[sr];
             if (go) {
               const double2 u = bz_mul(xj, Uc[o]);
               v = make_double2(v.x - u.x, v.y - u.y);
@@ -439,7 +491,7 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
     for (int e = tid; e < nhp * nzp; e += nt) {
       const int iz = 1 + e % nzp, ih = 1 + e / nzp;
       double2 v;
-      if (iz == nzp) v = zero;
+      if (iz == nzp) v = make_double2(0.0, 0.0);
       else if (ih == 1) v = exb[iz];
       else if (ih == nhp) v = exb[a.nz + 2 * Ny - 1 + iz];
       else if (iz == 1) v = exb[a.nz + ih];
