@@ -62,7 +62,21 @@ def cpu_baseline(grid, lc, om, B, X0, iters_per_solve, sample_iters=12):
         per_it = (time.time() - t1) / sample_iters
         kind, total = "port", time.time() - t0
     solves_per_s = 1.0 / (per_it * iters_per_solve)
+    bc_s = None
+    try:
+        import refdriver
+        if refdriver.bcx0_available():
+            # the reference's own BC_x0_WS for one system of the workload
+            t0 = time.time()
+            refdriver.run_bcx0(grid, lc, np.log(1e-10), [(om[0], 1)], timeout=600)
+            bc_s = time.time() - t0
+    except (ImportError, OSError):
+        bc_s = None
     return {"value": solves_per_s, "unit": "solves/s", "cores": 1, "kind": kind,
+            "bc_x0_reference_s_per_system": bc_s,
+            "bc_x0_reference_note": ("wall time of the reference's ComputeBC -> BC_x0_WS for one "
+                                     "(frequency, mode) of the workload, including the driver's "
+                                     "grid/operator set-up"),
             "sample": (f"{sample_iters} BiCGStab iterations of one system (f={om[0]/(2*np.pi):.3g} Hz) "
                        f"of the same 100x100x60+10 workload on 1 host core: {per_it:.3f} s/iteration, "
                        f"scaled by the mean FP64 iterations per solve ({iters_per_solve:.1f}); "
@@ -96,6 +110,9 @@ def main():
                     help="systems per block of the multi-system row SpMV (4 or 8)")
     ap.add_argument("--dilu-variant", type=int, default=4,
                     help="red-black DILU launch shape: 1 single-system persistent, 4 systems per workgroup")
+    ap.add_argument("--bc", choices=("2d", "1d"), default="2d",
+                    help="boundary data and initial guess: 2d = the reference's 2-D TE slice "
+                         "solves (BC_x0_WS) on the device; 1d = a plane wave over a halfspace")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
     args = ap.parse_args()
@@ -118,9 +135,36 @@ def main():
     t0 = time.time()
     ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
     log(f"assembled n_int={ctx.n_int} nnz={ctx.nnz} in {time.time()-t0:.1f}s")
-    om, B, X0 = P.forward_systems(ctx, grid, fhz)
-    nsys, n = B.shape
-    zb = nsys * n * 16
+    bc_info = None
+    if args.bc == "2d" and not args.kernels_only:
+        # the reference's start of a forward solve, on the device: fwdSetup ->
+        # ComputeBC -> BC_x0_WS (mode 1, 2 per frequency), then FWDsolve3D's
+        # b = -CCib e_b and x0 = e(EDGEi)
+        om = np.repeat(2 * P.PI * np.asarray(fhz, np.float64), 2)
+        modes = np.tile(np.array([1, 2], np.int32), len(fhz))
+        nsys, n = len(om), ctx.n_int
+        zb = nsys * n * 16
+        d_b, d_x0, d_x = ctx.dev_alloc(zb), ctx.dev_alloc(zb), ctx.dev_alloc(zb)
+        d_e = ctx.dev_alloc(nsys * ctx.n_edge * 16)
+        ctx.bc_x0_dev(om, modes, d_e)                   # sizes the scratch
+        t0 = time.time()
+        ctx.bc_x0_dev(om, modes, d_e)
+        t_bc = time.time() - t0
+        t0 = time.time()
+        ctx.rhs_fwd_dev(nsys, d_e, d_b, d_x0)
+        t_rhs = time.time() - t0
+        ctx.dev_free(d_e)
+        bc_info = {"bc_x0_ms": 1e3 * t_bc, "rhs_fwd_ms": 1e3 * t_rhs, "systems": nsys}
+        log(f"boundary fields of {nsys} systems on the device: BC_x0_WS {1e3 * t_bc:.1f} ms, "
+            f"Mult_Aib {1e3 * t_rhs:.1f} ms")
+        B = X0 = None
+        if not args.no_cpu:
+            B = ctx.download(d_b, np.zeros((nsys, n), np.complex128))
+            X0 = ctx.download(d_x0, np.zeros((nsys, n), np.complex128))
+    else:
+        om, B, X0 = P.forward_systems(ctx, grid, fhz)
+        nsys, n = B.shape
+        zb = nsys * n * 16
     if args.kernels_only:
         for prec, ns in ((mtsp2.PREC_MIXED, nsys), (mtsp2.PREC_FP64, 1)):
             kms, kby = ctx.bench_spmv(ns, prec, reps=5)
@@ -141,9 +185,10 @@ def main():
         ctx.dev_free(dst)
         ctx.close()
         return
-    d_b, d_x0, d_x = ctx.dev_alloc(zb), ctx.dev_alloc(zb), ctx.dev_alloc(zb)
-    ctx.upload(d_b, B)
-    ctx.upload(d_x0, X0)
+    if bc_info is None:
+        d_b, d_x0, d_x = ctx.dev_alloc(zb), ctx.dev_alloc(zb), ctx.dev_alloc(zb)
+        ctx.upload(d_b, B)
+        ctx.upload(d_x0, X0)
     tuned = ctx.autotune(nsys)
     log(f"autotune: {tuned} row-tile groups per system")
     if args.spmv_variant is not None:
@@ -257,7 +302,10 @@ def main():
         "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "c32 inner / c64 refinement",
-        "data": "synthetic (seeded blocky model, plane-wave boundary data)",
+        "data": ("synthetic (seeded blocky model; boundary data and initial guess from the "
+                 "2-D TE slice solves of BC_x0_WS, on the device)") if bc_info else
+                "synthetic (seeded blocky model, plane-wave boundary data)",
+        "boundary_fields": bc_info,
         "config": {"workload": f"{args.config}: 100x100x60+10 air, 2052270 interior edges, "
                                "8 f x 2 pol forward systems, tol 1e-7 FP64 true residual",
                    "nsys": nsys, "seed": args.seed, "inner_tol": args.inner_tol,
