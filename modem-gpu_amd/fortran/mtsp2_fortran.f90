@@ -15,7 +15,7 @@ module mtsp2_fortran
 
    integer(c_int), parameter, public :: MT_OK = 0, MT_E_ARG = 1, MT_E_DEVICE = 2, &
         MT_E_OOM = 3, MT_E_NAN_RHS = 4, MT_E_BREAKDOWN = 5, MT_E_MAXIT = 6, &
-        MT_E_STATE = 7, MT_E_STENCIL = 8
+        MT_E_STATE = 7, MT_E_STENCIL = 8, MT_E_PIVOT = 9
    integer(c_int), parameter, public :: MT_PC_DILU = 0, MT_PC_JACOBI = 1, MT_PC_NONE = 2
    integer(c_int), parameter, public :: MT_PREC_FP64 = 0, MT_PREC_MIXED = 1
 
@@ -35,7 +35,8 @@ module mtsp2_fortran
    public :: mt_default_opts, mt_ctx_create, mt_ctx_destroy, mt_device_count, mt_set_grid, &
              mt_operator_set_csr, mt_bicg, mt_solve_batch, mt_apply, mt_precond_apply, &
              mt_divcor, mt_pmultT, mt_last_error_string, mt_receivers_set, mt_data_resp, &
-             mt_lmultT, mt_dev_alloc, mt_dev_free, mt_dev_copy
+             mt_lmultT, mt_dev_alloc, mt_dev_free, mt_dev_copy, mt_assemble, mt_update_cond, &
+             mt_bc_x0_dev, mt_rhs_fwd_dev
    public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, mtsp2_data_resp
 
    interface
@@ -88,6 +89,22 @@ module mtsp2_fortran
          type(c_ptr), value :: ctx
          real(c_double), intent(in) :: logcond(*)
          real(c_double), value :: air_logcond
+      end function
+      ! ComputeBC -> BC_x0_WS (FWD_SP2/modelOperator3D.f90:292-317) for a device batch
+      integer(c_int) function mt_bc_x0_dev(ctx, nsys, omegas, modes, e_dev) &
+            bind(C, name='mt_bc_x0_dev')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx, e_dev
+         integer(c_int), value :: nsys
+         real(c_double), intent(in) :: omegas(*)
+         integer(c_int), intent(in) :: modes(*)
+      end function
+      ! b = -CCib e_b and x0 = e(EDGEi) (EMsolve3D.f90:217-274) for a device batch
+      integer(c_int) function mt_rhs_fwd_dev(ctx, nsys, e_dev, b_dev, x0_dev) &
+            bind(C, name='mt_rhs_fwd_dev')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, e_dev, b_dev, x0_dev
+         integer(c_int), value :: nsys
       end function
       integer(c_int) function mt_bicg(ctx, omega, b, x, maxit, tol, precond, niter, rerr, failed) &
             bind(C, name='mt_bicg')
