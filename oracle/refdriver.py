@@ -232,3 +232,89 @@ def run_bcx0(grid, logcond, air_log, systems, timeout=300):
     bc = np.fromfile(os.path.join(wd, "bc.bin"), np.complex128).reshape(ns, -1)
     _cleanup(wd, True)
     return e0, bc
+
+
+IO_BIN = os.path.join(HERE, "_ref", "ref_io")
+
+
+def io_available() -> bool:
+    return os.access(IO_BIN, os.X_OK)
+
+
+def run_io_ws(path, air=None, timeout=120):
+    """The reference's read_modelParam_WS of a model file (driver
+    oracle/ref/ref_io.f90); air = (method, nz, max_height_km, min_top_dz_km,
+    alpha) as a control file sets them, or None for the defaults.  Returns
+    dict(dx, dy, dz, nzAir, logcond (nx, ny, nzEarth) LOGE, origin, rotdeg,
+    air, param_type)."""
+    wd = tempfile.mkdtemp(prefix="mtio_")
+    out = os.path.join(wd, "ws.bin")
+    extra = [] if air is None else [str(air[0])] + [repr(v) for v in air[1:]]
+    subprocess.run([IO_BIN, "ws", os.path.abspath(path), out] + extra, check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=wd)
+    b = open(out, "rb").read()
+    nx, ny, nza, nze = (int(v) for v in np.frombuffer(b[:16], np.int32))
+    o = [16]
+
+    def take(n):
+        a = np.frombuffer(b[o[0]:o[0] + 8 * n], np.float64).copy()
+        o[0] += 8 * n
+        return a
+    dx, dy, dz = take(nx), take(ny), take(nza + nze)
+    ox, oy, oz, rot = take(4)
+    lc = take(nx * ny * nze).reshape(nze, ny, nx).transpose(2, 1, 0).copy()
+    air = float(take(1)[0])
+    pt = b[o[0]:o[0] + 80].decode().strip()
+    _cleanup(wd, True)
+    return dict(dx=dx, dy=dy, dz=dz, nzAir=nza, logcond=lc, origin=(ox, oy, oz), rotdeg=rot,
+                air=air, param_type=pt)
+
+
+def run_io_soln_write(path, dx, dy, dz, nzAir, omegas, fields, origin=(0.0, 0.0, 0.0), rotdeg=0.0,
+                      timeout=120):
+    """The reference's FileWriteInit + EfileWrite (write_solnVectorMTX's
+    records) of fields (nPer, 2, nEdge) into path."""
+    dx, dy, dz = (np.asarray(a, np.float64) for a in (dx, dy, dz))
+    wd = tempfile.mkdtemp(prefix="mtio_")
+    inp = os.path.join(wd, "in.bin")
+    with open(inp, "wb") as f:
+        np.array([len(dx), len(dy), nzAir, len(dz) - nzAir], np.int32).tofile(f)
+        dx.tofile(f); dy.tofile(f); dz.tofile(f)
+        np.array(list(origin) + [rotdeg], np.float64).tofile(f)
+        np.array([len(omegas)], np.int32).tofile(f)
+        for j, w in enumerate(omegas):
+            np.array([w], np.float64).tofile(f)
+            for k in range(2):
+                np.asarray(fields[j][k], np.complex128).tofile(f)
+    subprocess.run([IO_BIN, "solnw", inp, os.path.abspath(path)], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=wd)
+    _cleanup(wd, True)
+
+
+def run_io_soln_read(path, timeout=120):
+    """The reference's FileReadInit + EfileRead of a solution file: dict(dx,
+    dy, dz, nzAir, origin, rotdeg, omegas, fields (nPer, nMode, nEdge))."""
+    wd = tempfile.mkdtemp(prefix="mtio_")
+    out = os.path.join(wd, "soln.bin")
+    subprocess.run([IO_BIN, "solnr", os.path.abspath(path), out], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=wd)
+    b = open(out, "rb").read()
+    nper, nmode, nx, ny, nz, nza = (int(v) for v in np.frombuffer(b[:24], np.int32))
+    o = 24
+    ox, oy, oz, rot = np.frombuffer(b[o:o + 32], np.float64)
+    o += 32
+    dx = np.frombuffer(b[o:o + 8 * nx], np.float64).copy(); o += 8 * nx
+    dy = np.frombuffer(b[o:o + 8 * ny], np.float64).copy(); o += 8 * ny
+    dz = np.frombuffer(b[o:o + 8 * nz], np.float64).copy(); o += 8 * nz
+    ne = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
+    om = np.zeros(nper)
+    fields = np.zeros((nper, nmode, ne), np.complex128)
+    for j in range(nper):
+        for k in range(nmode):
+            om[j] = np.frombuffer(b[o:o + 8], np.float64)[0]
+            o += 8
+            fields[j, k] = np.frombuffer(b[o:o + 16 * ne], np.complex128)
+            o += 16 * ne
+    _cleanup(wd, True)
+    return dict(dx=dx, dy=dy, dz=dz, nzAir=nza, origin=(ox, oy, oz), rotdeg=rot, omegas=om,
+                fields=fields)

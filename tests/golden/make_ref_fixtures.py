@@ -3,11 +3,13 @@
 small seeded grid, tests/golden/ref_datafunc.npz: its dataResp and LmultT
 for every data type on the same grid, and tests/golden/ref_bcx0.npz: the 2-D
 TE boundary fields of BC_x0_WS (through ComputeBC) on that grid and on a deep
-one whose bandwidth takes LAPACK's blocked factorisation.  Inputs and outputs
-only -- data, no reference code.
+one whose bandwidth takes LAPACK's blocked factorisation, and
+tests/golden/io/: small WS model files with what the reference's reader reads
+from them (ref_io.npz), and a solution file written by the reference's
+writer with its inputs.  Inputs and outputs only -- data, no reference code.
 
 Run where the reference is mounted:
-    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0]
+    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0] [io]
 """
 import os
 import sys
@@ -134,11 +136,86 @@ def bcx0_main():
     print("wrote", os.path.join(HERE, "ref_bcx0.npz"))
 
 
+# a WS file written by hand with list-directed freedoms: a value list split
+# over lines, commas, r*c repeats, D exponents, surplus values at the end of
+# a record, and no origin or rotation lines
+WS_QUIRKS = """# hand-written test model (10 earth layers: the default air layers mirror them)
+2 2 10 0 LOG10
+ 1.0D+03, 500.0
+ 250.5
+ 300. 200. 999.0 888.0
+ 2*10.0, 12.5 15.0
+ 20.0 25.0 3*30.0
+ 40.0 99.0
+ 1.5 , 2.5, 3.5 7.7
+ 2*1.0 0.5
+ 0.25D0 -0.5
+ 2*-1.0
+ 1.0 2.0
+ 3.0 4.0
+ 0.5 0.5
+ 0.1 0.2
+ 1.25 2.75
+ 3*2.0
+ -1.5 1.5
+ 2.2 3.3 4.4
+ 1.0
+ 0.0
+ 5.0 6.0
+ 7.0 8.0
+ 9.0 1.0
+"""
+
+AIR_FIXED = ("fixed height", 12, 1000.0, 30.0, 3.0)
+
+
+def io_main():
+    d = os.path.join(HERE, "io")
+    os.makedirs(d, exist_ok=True)
+    sys.path.insert(0, os.path.join(ROOT, "modem-gpu_amd"))
+    from mtsp2 import formats
+    rng = np.random.default_rng(42)
+    nx, ny, nze = 5, 4, 12     # the default air layers mirror the top 10 earth layers
+    dx = np.round(rng.uniform(100, 900, nx), 3)
+    dy = np.round(rng.uniform(100, 900, ny), 3)
+    dze = np.round(25.0 * 1.3 ** np.arange(nze), 3)
+    lc = np.log(10.0 ** rng.uniform(-3, 0, (nx, ny, nze)))
+    files = {}
+    for pt in ("LOGE", "LOG10", "LINEAR"):
+        name = f"ws_{pt.lower()}.rho"
+        formats.write_ws_model(os.path.join(d, name), dx, dy, dze, lc, pt,
+                               origin=(-1500.0, -1200.0, 0.0), rotdeg=7.5)
+        files[name] = None
+    with open(os.path.join(d, "ws_quirks.rho"), "w") as f:
+        f.write(WS_QUIRKS)
+    files["ws_quirks.rho"] = None
+    out = {}
+    for name in files:
+        for tag, air in (("default", None), ("fixed", AIR_FIXED)):
+            r = refdriver.run_io_ws(os.path.join(d, name), air=air)
+            key = f"{name[:-4]}_{tag}"
+            out.update({f"{key}_{k}": np.asarray(v) for k, v in r.items()})
+    # a solution file from the reference's own writer
+    srng = np.random.default_rng(43)
+    sdx, sdy = srng.uniform(100, 500, 4), srng.uniform(100, 500, 3)
+    sdz = np.concatenate([[3000.0, 800.0], srng.uniform(50, 300, 3)])
+    ne = sum(formats.edge_counts(4, 3, 5))
+    om = np.array([2 * np.pi * 0.1, 2 * np.pi * 7.0])
+    F = srng.standard_normal((2, 2, ne)) + 1j * srng.standard_normal((2, 2, ne))
+    refdriver.run_io_soln_write(os.path.join(d, "ref_small.soln"), sdx, sdy, sdz, 2, om, F,
+                                origin=(-600.0, -450.0, 0.0), rotdeg=12.5)
+    out.update(soln_dx=sdx, soln_dy=sdy, soln_dz=sdz, soln_omegas=om, soln_fields=F)
+    np.savez_compressed(os.path.join(d, "ref_io.npz"), **out)
+    print("wrote", d)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["small", "datafunc", "bcx0"]
+    which = sys.argv[1:] or ["small", "datafunc", "bcx0", "io"]
     if "small" in which:
         main()
     if "datafunc" in which:
         datafunc_main()
     if "bcx0" in which:
         bcx0_main()
+    if "io" in which:
+        io_main()
