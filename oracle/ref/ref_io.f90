@@ -16,6 +16,12 @@
 !                                     write_solnVectorMTX writes
 !       in:  int32 nx, ny, nzAir, nzEarth ; real64 dx, dy, dz, ox, oy, oz, rotdeg ;
 !            int32 nPer ; nPer x { real64 omega ; 2 x complex128 E (x, y, z edges) }
+!   ref_io zlist <data file> <out>    read_dataVectorMTX = read_Z_list
+!                                     (3D_MT/DataIO.f90:552-1209)
+!       out: int32 nTx ; real64 periods ; int32 nRx ; nRx x { char(50) id ; real64 x(3) } ;
+!            int32 nd ; nd x { int32 tx, nDt ; nDt x { int32 dataType, nComp, nSite,
+!            isComplex ; int32 rx(nSite) ; real64 value(nComp, nSite), error ;
+!            int32 exist ; real64 (Hx, Hy, Ex, Ey, Hx_ref, Hy_ref)(nSite) } }
 !   ref_io solnr <soln file> <out>    FileReadInit + EfileRead (ioAscii.f90)
 !       out: int32 nPer, nMode, nx, ny, nz, nzAir ; real64 ox, oy, oz, rotdeg,
 !            dx, dy, dz ; nPer*nMode x { real64 omega ; complex128 E }
@@ -27,10 +33,16 @@ program ref_io
    use ModelSpace
    use file_units
    use ioAscii
+   use dataspace
+   use transmitters
+   use receivers
+   use DataIO
    implicit none
 
    character(len=512)           :: cmd, a1, a2, a3
    real(kind=prec)              :: rv
+   type(dataVectorMTX_t)        :: alld
+   integer                      :: i, q
    integer                      :: fin, fo, nx, ny, nza, nze, nper, j, k, ios, fper, fmode
    type(grid_t), target         :: grid
    type(airLayers_t)            :: air
@@ -108,8 +120,34 @@ program ref_io
       end do
       close(fo)
       close(ioE)
+   case ('zlist')
+      call read_dataVectorMTX(alld, trim(a1))
+      open(newunit=fo, file=trim(a2), access='stream', form='unformatted', status='replace')
+      write(fo) int(size(txDict), 4)
+      write(fo) (txDict(i)%period, i = 1, size(txDict))
+      write(fo) int(size(rxDict), 4)
+      do i = 1, size(rxDict)
+         write(fo) rxDict(i)%id, rxDict(i)%x
+      end do
+      write(fo) int(alld%nTx, 4)
+      do i = 1, alld%nTx
+         write(fo) int(alld%d(i)%tx, 4), int(alld%d(i)%nDt, 4)
+         do j = 1, alld%d(i)%nDt
+            associate (bl => alld%d(i)%data(j))
+               write(fo) int(bl%dataType, 4), int(bl%nComp, 4), int(bl%nSite, 4), &
+                         int(merge(1, 0, bl%isComplex), 4)
+               write(fo) (int(bl%rx(k), 4), k = 1, bl%nSite)
+               write(fo) bl%value, bl%error
+               write(fo) ((int(merge(1, 0, bl%exist(q, k)), 4), q = 1, bl%nComp), k = 1, bl%nSite)
+               write(fo) (bl%orient(k)%azimuth%Hx, bl%orient(k)%azimuth%Hy, bl%orient(k)%azimuth%Ex, &
+                          bl%orient(k)%azimuth%Ey, bl%orient(k)%azimuth%Hx_ref, &
+                          bl%orient(k)%azimuth%Hy_ref, k = 1, bl%nSite)
+            end associate
+         end do
+      end do
+      close(fo)
    case default
-      write(0, *) 'usage: ref_io ws|solnw|solnr <in> <out>'
+      write(0, *) 'usage: ref_io ws|solnw|solnr|zlist <in> <out>'
       stop 2
    end select
 end program ref_io

@@ -318,3 +318,50 @@ def run_io_soln_read(path, timeout=120):
     _cleanup(wd, True)
     return dict(dx=dx, dy=dy, dz=dz, nzAir=nza, origin=(ox, oy, oz), rotdeg=rot, omegas=om,
                 fields=fields)
+
+
+def run_io_zlist(path, timeout=120):
+    """The reference's read_dataVectorMTX (read_Z_list) of a list data file:
+    dict(periods, receivers [(id, x)], data [dict(tx, blocks [dict(type,
+    ncomp, complex, rx, value, error, exist, azimuth)])])."""
+    wd = tempfile.mkdtemp(prefix="mtio_")
+    out = os.path.join(wd, "zl.bin")
+    subprocess.run([IO_BIN, "zlist", os.path.abspath(path), out], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=wd)
+    b = open(out, "rb").read()
+    o = [0]
+
+    def i32(n=1):
+        a = np.frombuffer(b[o[0]:o[0] + 4 * n], np.int32).copy()
+        o[0] += 4 * n
+        return a
+
+    def f64(n=1):
+        a = np.frombuffer(b[o[0]:o[0] + 8 * n], np.float64).copy()
+        o[0] += 8 * n
+        return a
+    ntx = int(i32()[0])
+    periods = f64(ntx)
+    nrx = int(i32()[0])
+    receivers = []
+    for _ in range(nrx):
+        rid = b[o[0]:o[0] + 50].decode().rstrip()
+        o[0] += 50
+        receivers.append((rid, f64(3)))
+    nd = int(i32()[0])
+    data = []
+    for _ in range(nd):
+        tx, ndt = (int(v) for v in i32(2))
+        blocks = []
+        for _ in range(ndt):
+            dt, nc, ns, cx = (int(v) for v in i32(4))
+            rx = i32(ns)
+            v = f64(nc * ns).reshape(ns, nc).T.copy()
+            e = f64(nc * ns).reshape(ns, nc).T.copy()
+            ex = i32(nc * ns).reshape(ns, nc).T.astype(bool)
+            az = f64(6 * ns).reshape(ns, 6)
+            blocks.append(dict(type=dt, ncomp=nc, complex=bool(cx), rx=rx, value=v, error=e, exist=ex,
+                               azimuth=az))
+        data.append(dict(tx=tx, blocks=blocks))
+    _cleanup(wd, True)
+    return dict(periods=periods, receivers=receivers, data=data)

@@ -169,6 +169,66 @@ WS_QUIRKS = """# hand-written test model (10 earth layers: the default air layer
 AIR_FIXED = ("fixed height", 12, 1000.0, 30.0, 3.0)
 
 
+def zlist_text():
+    """A list data file in five blocks exercising read_Z_list's MT branches:
+    azimuth columns (12-15 and 10-14), both sign conventions, Ohm and
+    [mV/km]/[nT] units, a geographic orientation, transmitters and sites
+    shared across blocks."""
+    rng = np.random.default_rng(44)
+    sites = [("S01", 45.123, -97.5, 100.0, -2000.0, 0.0), ("S02", 45.2, -97.61, 2100.5, 350.0, 12.0),
+             ("LONGSITENAME-0003", 44.9, -98.0, -1500.0, 800.0, 0.0)]
+    hdr = "# Period(s) Code GG_Lat GG_Lon X(m) Y(m) Z(m) Component Real Imag Error"
+
+    def cplx_block(name, comps, sign, units, orient, periods, extra):
+        out = [f"# synthetic {name}", hdr, f"> {name}", f"> exp({sign}i\\omega t)", f"> {units}",
+               f">   {orient:.2f}", ">   45.000  -97.000", f">  {len(periods)}  {len(sites)}"]
+        for p in periods:
+            for si, (c, la, lo, x, y, z) in enumerate(sites):
+                for ci, cp in enumerate(comps):
+                    re_, im_, er = rng.standard_normal(2).tolist() + [abs(rng.standard_normal()) + 0.1]
+                    az = " ".join(f"{a:.2f}" for a in extra[(si + ci) % len(extra)])
+                    out.append(f"{p:.6E} {c} {la:.3f} {lo:.3f} {x:.3f} {y:.3f} {z:.3f} {cp} "
+                               f"{re_:.6E} {im_:.6E} {er:.6E} {az}".rstrip())
+        return out
+
+    def real_block(name, comps, sign, orient, periods, extra):
+        out = [f"# synthetic {name}", hdr, f"> {name}", f"> exp({sign}i\\omega t)", "> []",
+               f">   {orient:.2f}", ">   45.000  -97.000", f">  {len(periods)}  {len(sites)}"]
+        for p in periods:
+            for si, (c, la, lo, x, y, z) in enumerate(sites):
+                for ci, cp in enumerate(comps):
+                    v = 10.0 ** rng.uniform(0, 3) if cp.startswith("RHO") else rng.uniform(5, 80)
+                    er = 0.05 * v
+                    az = " ".join(f"{a:.2f}" for a in extra[(si + ci) % len(extra)])
+                    out.append(f"{p:.6E} {c} {la:.3f} {lo:.3f} {x:.3f} {y:.3f} {z:.3f} {cp} "
+                               f"{v:.6E} {er:.6E} {az}".rstrip())
+        return out
+    ext = [(), (12.0,), (12.0, 98.0), (12.0, 98.0, 8.0), (12.0, 98.0, 8.0, 101.0)]
+    lines = []
+    lines += cplx_block("Full_Impedance", ["ZXX", "ZXY", "ZYX", "ZYY"], "-", "[V/m]/[A/m]", 10.0,
+                        [1.0, 100.0], ext)
+    lines += real_block("Off_Diagonal_Rho_Phase", ["RHOXY", "PHSXY", "RHOYX", "PHSYX"], "+", 0.0,
+                        [100.0, 3.16], ext)
+    lines += cplx_block("Full_Vertical_Components", ["TX", "TY"], "+", "[]", 0.0, [1.0], ext[:1])
+    lines += real_block("Phase_Tensor", ["PTXX", "PTXY", "PTYX", "PTYY"], "-", -5.0, [3.16], ext)
+    lines += cplx_block("Off_Diagonal_Impedance", ["ZXY", "ZYX"], "+", "[mV/km]/[nT]", 0.0,
+                        [1000.0], ext[:1])
+    return "\n".join(lines) + "\n"
+
+
+def _pack_zlist(prefix, z, out):
+    out[prefix + "periods"] = np.asarray(z["periods"])
+    out[prefix + "rx_ids"] = np.array([r[0] for r in z["receivers"]])
+    out[prefix + "rx_x"] = np.array([r[1] for r in z["receivers"]])
+    out[prefix + "nd"] = np.array(len(z["data"]))
+    for i, d in enumerate(z["data"]):
+        out[f"{prefix}d{i}_tx"] = np.array(d["tx"])
+        out[f"{prefix}d{i}_nb"] = np.array(len(d["blocks"]))
+        for j, b in enumerate(d["blocks"]):
+            for k in ("type", "ncomp", "rx", "value", "error", "exist", "azimuth"):
+                out[f"{prefix}d{i}_b{j}_{k}"] = np.asarray(b[k])
+
+
 def io_main():
     d = os.path.join(HERE, "io")
     os.makedirs(d, exist_ok=True)
@@ -205,6 +265,11 @@ def io_main():
     refdriver.run_io_soln_write(os.path.join(d, "ref_small.soln"), sdx, sdy, sdz, 2, om, F,
                                 origin=(-600.0, -450.0, 0.0), rotdeg=12.5)
     out.update(soln_dx=sdx, soln_dy=sdy, soln_dz=sdz, soln_omegas=om, soln_fields=F)
+    # a list data file read by the reference's read_Z_list
+    zpath = os.path.join(d, "zlist_mixed.dat")
+    with open(zpath, "w") as f:
+        f.write(zlist_text())
+    _pack_zlist("zl_", refdriver.run_io_zlist(zpath), out)
     np.savez_compressed(os.path.join(d, "ref_io.npz"), **out)
     print("wrote", d)
 

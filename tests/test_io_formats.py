@@ -105,3 +105,50 @@ def test_ws_reference_models_live(name):
     """The reference's own test models (48 x 68 x 44) read alike."""
     path = os.path.join(REF_UT, name)
     _same(F.read_ws_model(path), refdriver.run_io_ws(path))
+
+
+def _same_list(m, r):
+    assert np.array_equal(m["periods"], r["periods"])
+    assert [a for a, _ in m["receivers"]] == [a for a, _ in r["receivers"]]
+    assert all(np.array_equal(xa, xb) for (_, xa), (_, xb) in zip(m["receivers"], r["receivers"]))
+    assert len(m["data"]) == len(r["data"])
+    for dm, dr in zip(m["data"], r["data"]):
+        assert dm["tx"] == dr["tx"] and len(dm["blocks"]) == len(dr["blocks"])
+        for bm, br in zip(dm["blocks"], dr["blocks"]):
+            assert bm["type"] == br["type"] and bm["ncomp"] == br["ncomp"]
+            for k in ("rx", "value", "error", "exist", "azimuth"):
+                assert np.array_equal(np.asarray(bm[k]), np.asarray(br[k])), (bm["type"], k)
+
+
+def _unpack_list(d, p):
+    z = dict(periods=d[p + "periods"],
+             receivers=list(zip([str(s) for s in d[p + "rx_ids"]], d[p + "rx_x"])), data=[])
+    for i in range(int(d[p + "nd"])):
+        blocks = [{k: d[f"{p}d{i}_b{j}_{k}"] for k in
+                   ("type", "ncomp", "rx", "value", "error", "exist", "azimuth")}
+                  for j in range(int(d[f"{p}d{i}_nb"]))]
+        for b in blocks:
+            b["type"], b["ncomp"] = int(b["type"]), int(b["ncomp"])
+        z["data"].append(dict(tx=int(d[f"{p}d{i}_tx"]), blocks=blocks))
+    return z
+
+
+def test_list_reader_fixture():
+    """read_Z_list's MT branches: impedances with 11-15 columns (azimuths,
+    the geographic orientation added), rho/phase (log10 of rho with its error
+    propagation, phase to radians with the sign convention), phase tensor,
+    vertical TFs, Ohm and [mV/km]/[nT] units (ImpUnits' default-real
+    literals), conjugation of exp(+i omega t) data, transmitters and sites
+    shared across blocks (and the reference's two site-id formats)."""
+    d = np.load(os.path.join(GOLD, "ref_io.npz"))
+    _same_list(F.read_data_list(os.path.join(GOLD, "zlist_mixed.dat")), _unpack_list(d, "zl_"))
+
+
+@pytest.mark.skipif(not refdriver.io_available(), reason="reference build (oracle/_ref/ref_io) absent")
+def test_list_reader_live():
+    files = [os.path.join(GOLD, "zlist_mixed.dat")]
+    if os.path.isdir(REF_UT):
+        files += [os.path.join(REF_UT, f) for f in
+                  ("CylinderModel_1per.dat", "CylinderModel_1per_Template.dat")]
+    for p in files:
+        _same_list(F.read_data_list(p), refdriver.run_io_zlist(p))
