@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--config", default="cfg1")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-7)
-    ap.add_argument("--inner-tol", type=float, default=1e-4,
+    ap.add_argument("--inner-tol", type=float, default=3e-4,
                     help="least residual reduction asked of each FP32 inner solve")
     ap.add_argument("--restart", type=int, default=120,
                     help="BiCGStab restart interval of the inner solves (the reference's 120)")

@@ -67,7 +67,7 @@ typedef struct {
   int precision;        /* MT_PREC_*                                           */
   int restart;          /* restart interval; 0 -> 120 (the reference's)        */
   int best_iterate;     /* 1: return the smallest-residual iterate on failure  */
-  double inner_tol;     /* mixed: inner relative reduction per refinement step */
+  double inner_tol;     /* mixed: inner relative reduction per refinement step (3e-4) */
   int max_outer;        /* mixed: max refinement steps                         */
   int op_transpose;     /* 1: solve with A^T (north-star adjoint); 0: A        */
   int chunk;            /* iterations per host status check / graph replay     */

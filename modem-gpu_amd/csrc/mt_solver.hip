@@ -966,7 +966,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   oa.d32 = d32;
   oa.partials = a.partials;
   oa.tol = o.tol;
-  oa.inner_tol = o.inner_tol > 0 ? o.inner_tol : 1e-4;
+  oa.inner_tol = o.inner_tol > 0 ? o.inner_tol : 3e-4;
   oa.max_outer = o.max_outer > 0 ? o.max_outer : 30;
   oa.maxit_total = o.maxit;
   const int rlen = oa.max_outer + 2;
@@ -1095,7 +1095,7 @@ void mt_default_opts(mt_solve_opts* o) {
   o->precision = MT_PREC_FP64;
   o->restart = 120;
   o->best_iterate = 1;
-  o->inner_tol = 1e-4;
+  o->inner_tol = 3e-4;
   o->max_outer = 30;
   o->op_transpose = 0;
   o->chunk = 10;
