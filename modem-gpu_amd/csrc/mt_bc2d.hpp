@@ -395,42 +395,58 @@ __global__ __launch_bounds__(256) void k_bc_te2d(BcArgs a) {
       }
       lds_barrier();
       // rank-1 update of the trailing m x m block (ZGERU order per entry):
-      // thread (g, i) owns row i and every G-th column from 1+g, so its
-      // multiplier is read once; four columns are in flight at a time.  The
-      // owner of the next pivot (row 1, column 1: thread 0) also forms its
-      // reciprocal for the next step.
+      // thread (g, i) owns rows i and i + mh and every G-th column from 1+g,
+      // so each U-row value it reads serves two entries and its multipliers
+      // are read once; four columns are in flight at a time.  The owner of
+      // the next pivot (row 1, column 1: thread 0) also forms its reciprocal
+      // for the next step.
       if (m > 0) {
-        const int G = nt / m, g = tid / m, i = 1 + tid % m;
+        const int mh = (m + 1) / 2, G = nt / mh, g = tid / mh, i = 1 + tid % mh, i2 = i + mh;
         if (g < G) {
-          int si = sj + i;
+          const bool two = i2 <= m;
+          int si = sj + i, si2 = sj + i2;
           if (si >= B1) si -= B1;
+          if (si2 >= B1) si2 -= B1;
           const double2 l = W[sj * B1 + si];
+          double2 l2 = make_double2(0.0, 0.0);
+          if (two) l2 = W[sj * B1 + si2];
           int c = 1 + g;
           for (; c + 3 * G <= m; c += 4 * G) {
             int sc[4];
-            double2 y[4], w[4];
+            double2 y[4], w[4], w2[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
               sc[k] = sj + c + k * G;
               if (sc[k] >= B1) sc[k] -= B1;
               y[k] = W[sc[k] * B1 + sj];
               w[k] = W[sc[k] * B1 + si];
+              if (two) w2[k] = W[sc[k] * B1 + si2];
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
               const double2 u = bz_mul(l, y[k]);
               w[k] = make_double2(w[k].x - u.x, w[k].y - u.y);
               W[sc[k] * B1 + si] = w[k];
+              if (two) {
+                const double2 u2 = bz_mul(l2, y[k]);
+                W[sc[k] * B1 + si2] = make_double2(w2[k].x - u2.x, w2[k].y - u2.y);
+              }
             }
             if (tid == 0 && c == 1) Rp[0] = bz_div(make_double2(1.0, 0.0), w[0]);
           }
           for (; c <= m; c += G) {
             int sc = sj + c;
             if (sc >= B1) sc -= B1;
-            const double2 u = bz_mul(l, W[sc * B1 + sj]);
+            const double2 yv2 = W[sc * B1 + sj];
+            const double2 u = bz_mul(l, yv2);
             const double2 w0 = W[sc * B1 + si];
             const double2 wn = make_double2(w0.x - u.x, w0.y - u.y);
             W[sc * B1 + si] = wn;
+            if (two) {
+              const double2 u2 = bz_mul(l2, yv2);
+              const double2 w1 = W[sc * B1 + si2];
+              W[sc * B1 + si2] = make_double2(w1.x - u2.x, w1.y - u2.y);
+            }
             if (tid == 0 && c == 1) Rp[0] = bz_div(make_double2(1.0, 0.0), wn);
           }
         }
