@@ -37,7 +37,8 @@ module mtsp2_fortran
              mt_divcor, mt_pmultT, mt_last_error_string, mt_receivers_set, mt_data_resp, &
              mt_lmultT, mt_dev_alloc, mt_dev_free, mt_dev_copy, mt_assemble, mt_update_cond, &
              mt_bc_x0_dev, mt_rhs_fwd_dev
-   public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, mtsp2_data_resp
+   public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, mtsp2_data_resp, &
+             mtsp2_bc_x0
 
    interface
       subroutine mt_default_opts(opts) bind(C, name='mt_default_opts')
@@ -426,6 +427,38 @@ contains
                                             0_c_intptr_t) + int(nb, c_intptr_t), c_dev), nb, 1_c_int)
       rc = mt_dev_free(ctx, e_dev)
       rc = mt_dev_free(ctx, c_dev)
+   end subroutine
+
+
+   ! Drop-in for the ComputeBC -> BC_x0_WS calls of fwdSetup
+   ! (FWD_SP2/modelOperator3D.f90:292-317, ForwardSolver.f90:333-462) for a
+   ! batch of (omega, mode) systems: e(nEdge, nsys) receives what FWDsolve3D
+   ! starts from -- E0 on the interior edges, the boundary conditions (setBC
+   ! of BC) on the boundary edges.  The model is the LOGE cell model of the
+   ! grid (logcond nx*ny*nzEarth, Fortran order, and the air value).
+   subroutine mtsp2_bc_x0(device_idx, nx, ny, nzAir, nzEarth, dx, dy, dz, logcond, air_logcond, &
+                          nsys, omegas, modes, e, ierr)
+      integer, intent(in)                    :: device_idx, nx, ny, nzAir, nzEarth, nsys
+      real(c_double), intent(in)             :: dx(:), dy(:), dz(:), logcond(:), air_logcond
+      real(c_double), intent(in)             :: omegas(:)
+      integer(c_int), intent(in)             :: modes(:)
+      complex(c_double_complex), intent(out), target :: e(:,:)
+      integer, intent(out)                   :: ierr
+      type(c_ptr)       :: ctx, e_dev
+      integer(c_size_t) :: nb
+      call dev_ctx(device_idx, ctx, ierr)
+      if (ierr /= MT_OK) return
+      ierr = mt_assemble(ctx, int(nx, c_int), int(ny, c_int), int(nzAir, c_int), &
+                         int(nzEarth, c_int), dx, dy, dz, logcond, air_logcond)
+      if (ierr /= MT_OK) return
+      nb = int(size(e, 1), c_size_t) * int(nsys, c_size_t) * 16_c_size_t
+      ierr = mt_dev_alloc(ctx, nb, e_dev)
+      if (ierr /= MT_OK) return
+      ierr = mt_bc_x0_dev(ctx, int(nsys, c_int), omegas, modes, e_dev)
+      if (ierr == MT_OK .or. ierr == MT_E_PIVOT) then
+         if (mt_dev_copy(ctx, c_loc(e), e_dev, nb, 1_c_int) /= MT_OK) ierr = MT_E_DEVICE
+      end if
+      if (mt_dev_free(ctx, e_dev) /= MT_OK .and. ierr == MT_OK) ierr = MT_E_DEVICE
    end subroutine
 
 end module mtsp2_fortran

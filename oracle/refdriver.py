@@ -19,6 +19,7 @@ REF_BIN = os.path.join(HERE, "_ref", "modem_ref")
 DF_BIN = os.path.join(HERE, "_ref", "ref_datafunc")
 DF_DROPIN_BIN = os.path.join(HERE, "_ref", "ref_dropin_datafunc")
 BCX0_BIN = os.path.join(HERE, "_ref", "ref_bcx0")
+BC_DROPIN_BIN = os.path.join(HERE, "_ref", "ref_dropin_bc")
 
 
 def available() -> bool:
@@ -365,3 +366,39 @@ def run_io_zlist(path, timeout=120):
         data.append(dict(tx=tx, blocks=blocks))
     _cleanup(wd, True)
     return dict(periods=periods, receivers=receivers, data=data)
+
+
+def dropin_bc_available() -> bool:
+    return os.access(BC_DROPIN_BIN, os.X_OK)
+
+
+def run_dropin_bc(grid, logcond, air_log, systems, timeout=300):
+    """One Fortran host (oracle/ref/ref_dropin_bc.f90) runs the reference's
+    ComputeBC and the library's mtsp2_bc_x0 for systems = [(omega, mode)].
+    Returns (ref, lib, status): (nsys, nEdge) fields in the form FWDsolve3D
+    starts from, and the binding's status."""
+    dx = np.asarray(grid["dx"], np.float64)
+    dy = np.asarray(grid["dy"], np.float64)
+    dz = np.asarray(grid["dz"], np.float64)
+    nx, ny, nz = len(dx), len(dy), len(dz)
+    nza = int(grid["nzAir"])
+    lc = np.asarray(logcond, np.float64).reshape(nx, ny, nz - nza)
+    wd = tempfile.mkdtemp(prefix="mtbc_")
+    inp = os.path.join(wd, "input.bin")
+    with open(inp, "wb") as f:
+        np.array([nx, ny, nza, nz - nza], np.int32).tofile(f)
+        dx.tofile(f); dy.tofile(f); dz.tofile(f)
+        lc.ravel(order="F").tofile(f)
+        np.array([air_log], np.float64).tofile(f)
+        np.array([len(systems)], np.int32).tofile(f)
+        for om, mode in systems:
+            np.array([om], np.float64).tofile(f)
+            np.array([mode], np.int32).tofile(f)
+    subprocess.run([BC_DROPIN_BIN, inp, wd], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    ns = len(systems)
+    ref = np.fromfile(os.path.join(wd, "ref.bin"), np.complex128).reshape(ns, -1)
+    lib = np.fromfile(os.path.join(wd, "lib.bin"), np.complex128).reshape(ns, -1)
+    st = int(np.fromfile(os.path.join(wd, "ierr.bin"), np.int32)[0])
+    _cleanup(wd, True)
+    return ref, lib, st

@@ -9,6 +9,8 @@
    libmtsp2 call, next to the reference CPU BICG on the same system.
 3. oracle/_ref/ref_dropin_datafunc (where it was built): the reference's
    dataResp/LmultT next to mtsp2_data_resp/mtsp2_lmultT, one Fortran host.
+4. oracle/_ref/ref_dropin_bc (where it was built): the reference's
+   ComputeBC -> BC_x0_WS next to mtsp2_bc_x0, one Fortran host.
 """
 import os
 import subprocess
@@ -137,3 +139,18 @@ def test_reference_host_with_library_lmultT():
         assert np.max(np.abs(lr - rr)) <= 1e-10 * np.max(np.abs(rr)), dt
         for a, b in ((l1, c1), (l2, c2)):
             assert np.max(np.abs(a - b)) <= 1e-10 * max(np.max(np.abs(b)), 1e-300), dt
+
+
+@pytest.mark.skipif(not refdriver.dropin_bc_available(),
+                    reason="reference boundary-field drop-in not built here")
+def test_reference_host_with_library_bc_x0():
+    """The reference's ComputeBC (the 2-D TE slice solves) and the library's
+    mtsp2_bc_x0, called from one Fortran host on the same grid, model and
+    (omega, mode) list: bit for bit."""
+    print("case reference dropin BC_x0_WS", flush=True)
+    grid = P.make_grid(11, 8, 12, nair=5, npad=3)
+    lc = P.boxes_model(grid, nbox=10, seed=12)
+    sy = [(2 * np.pi * 0.01, 1), (2 * np.pi * 0.01, 2), (2 * np.pi * 40.0, 1), (2 * np.pi * 40.0, 2)]
+    ref, lib, st = refdriver.run_dropin_bc(grid, lc, np.log(1e-10), sy)
+    assert st == 0
+    assert np.array_equal(lib, ref)
