@@ -120,3 +120,19 @@ def test_bc_x0_dev_full_size(gpu_ctx):
     for s in (0, len(om) - 1):
         want = _want(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc, om[s:s + 1], md[s:s + 1])
         assert np.array_equal(e[s], want[0]), s
+
+
+def test_bc_x0_dev_deep_window(gpu_ctx):
+    """nz = 90 (the 120x120x80+10 configuration's depth): the band window
+    takes 131 KB of LDS, one slice per CU; bit for bit against the oracle."""
+    print("case bc_x0 nz=90", flush=True)
+    grid = P.make_grid(5, 4, 80, nair=10, npad=2)
+    lc = P.boxes_model(grid, nbox=6, seed=8)
+    ctx = gpu_ctx
+    ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
+    assert len(grid["dz"]) == 90
+    om = 2 * np.pi * np.array([3e-3, 20.0])
+    md = np.array([1, 2])
+    e = ctx.bc_x0(om, md)
+    want = _want(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc, om, md)
+    assert np.array_equal(e, want)
