@@ -203,7 +203,9 @@ int mt_rhs_fwd_dev(mt_ctx *ctx, int nsys, const void *e_dev, void *b_dev, void *
  * the bottom-face x and y edges zero), ready for mt_rhs_fwd_dev.  Equal to
  * the reference's fields bit for bit while its partial pivoting keeps the
  * diagonal; MT_E_PIVOT (fields still written) where it would not.
- * Needs nz <= about 100 (the slice window lives in LDS). */
+ * Needs nz <= about 100 (the slice window lives in LDS).  Scratch, kept by
+ * the context: the rows of U of the slices in flight, up to
+ * 2*CUs*(nh-1)*(nz-1)*nz complex doubles (3.9 GB at 100x100x70 on 256 CUs). */
 int mt_bc_x0_dev(mt_ctx *ctx, int nsys, const double *omegas, const int *modes, void *e_dev);
 /* Transposed (adjoint) RHS b = s_i + V_E GDii(s_i (i*ISIGN*omega*mu0)^-1 / V_E). */
 int mt_rhs_trn(mt_ctx *ctx, double omega, const mt_zdouble *s_all, mt_zdouble *b);

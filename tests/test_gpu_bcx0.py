@@ -136,3 +136,17 @@ def test_bc_x0_dev_deep_window(gpu_ctx):
     e = ctx.bc_x0(om, md)
     want = _want(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc, om, md)
     assert np.array_equal(e, want)
+
+
+def test_bc_x0_dev_argument_errors(gpu_ctx):
+    """Invalid modes and frequencies are refused before any launch."""
+    print("case bc_x0 errors", flush=True)
+    import mtsp2
+    dx, dy, dz = np.full(4, 300.0), np.full(3, 300.0), np.concatenate([[3000.0, 1000.0], np.full(4, 100.0)])
+    lc = np.full((4, 3, 4), np.log(0.01))
+    ctx = gpu_ctx
+    ctx.assemble(dx, dy, dz, 2, lc)
+    for om, md in (([1.0], [3]), ([0.0], [1]), ([-2.0], [2])):
+        with pytest.raises(mtsp2.MTError) as ei:
+            ctx.bc_x0(np.array(om), np.array(md))
+        assert ei.value.code == mtsp2.MT_E_ARG
