@@ -83,6 +83,31 @@ def cpu_baseline(grid, lc, om, B, X0, iters_per_solve, sample_iters=12):
                        f"{total:.1f} s wall incl. operator set-up")}
 
 
+TRAFFIC_SUMMARY = os.path.join("profiles", "mixed_batch16_100x100x60_step_hbm_counters.md")
+
+
+def counted_traffic(fam, args):
+    """HBM bytes per launch of kernel family `fam` over the launches of one
+    step, as counted by two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE,
+    each scaled by its reading for a 1 GiB device copy) of this benchmark.
+    The counter summary holds one line per family,
+        traffic_bytes_per_launch <family> <bytes> <arg>=<value> ...
+    with the arguments of the profiled run; the figure applies only to a run
+    with the same ones (they fix the launch sequence), else None."""
+    path = os.path.join(ROOT, TRAFFIC_SUMMARY)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if len(w) < 3 or w[0] != "traffic_bytes_per_launch" or w[1] != fam:
+                continue
+            want = dict(kv.split("=", 1) for kv in w[3:])
+            have = {k: str(getattr(args, k)) for k in want}
+            return float(w[2]) if have == want else None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,7 +143,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(systems shard across GPUs by frequency with no exchange; see DESIGN.md)",
+              "(the systems would shard across GPUs by frequency, with no exchange in the solve)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -269,8 +294,13 @@ def main():
     kernels = {"spmv": spmv_kernels[spv],
                "precond": pc_kernels.get(args.precond, "preconditioner")}
     achieved = kby / (kms * 1e-3) / 1e9
+    traffic = counted_traffic(dom, args)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": kernels[dom],
+                "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "kernel": kernels[dom],
+                "traffic_source": (f"{TRAFFIC_SUMMARY}: mean HBM bytes per launch over the "
+                                   "launches of one step, counted (FETCH_SIZE x 2.0 + WRITE_SIZE)")
+                if traffic else None,
+                "traffic_over_algorithmic": traffic / (kby / nl) if traffic else None,
                 "launches": nl, "us_per_launch": 1e3 * kms / nl, "bytes_per_launch": kby / nl,
                 "share_of_step": kms / (ms * args.steps),
                 "timed_families": {k: {"launches": v[0], "ms": v[1],

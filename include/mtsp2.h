@@ -69,7 +69,7 @@ typedef struct {
   int best_iterate;     /* 1: return the smallest-residual iterate on failure  */
   double inner_tol;     /* mixed: inner relative reduction per refinement step (3e-4) */
   int max_outer;        /* mixed: max refinement steps                         */
-  int op_transpose;     /* 1: solve with A^T (north-star adjoint); 0: A        */
+  int op_transpose;     /* 1: solve with A^T (the adjoint systems); 0: A      */
   int chunk;            /* iterations per host status check / graph replay     */
   int use_graph;        /* 1: capture the iteration chunk in a hipGraph        */
 } mt_solve_opts;

@@ -2,7 +2,7 @@
 ! code sets up the operator (ModelDataInit, UpdateCond, UpdateFreq from
 ! f90/3D_MT/FWD_SP2/modelOperator3D.f90, compiled from the reference sources
 ! by oracle/ref/Makefile) and the solve goes through `cuBiCG` below -- the
-! replacement a maintainer puts into solver.f90 (INTEGRATION.md), which hands
+! replacement a maintainer puts into solver.f90, which hands
 ! the reference's module state (AAii, VomegaMuSig, omega, mGrid) to libmtsp2
 ! through modem-gpu_amd/fortran/mtsp2_fortran.f90.  The reference CPU BICG
 ! (solver.f90:720-926) solves the same system for comparison.
