@@ -133,8 +133,9 @@ def main():
                     help="SpMV kernel of the solves: 0 row, 1 k-marching (FP32), 3 multi-system row")
     ap.add_argument("--spmv-systems", type=int, default=4,
                     help="systems per block of the multi-system row SpMV (4 or 8)")
-    ap.add_argument("--dilu-variant", type=int, default=4,
-                    help="red-black DILU launch shape: 1 single-system persistent, 4 systems per workgroup")
+    ap.add_argument("--dilu-variant", type=int, default=5, choices=(1, 4, 5),
+                    help="red-black DILU launch shape: 1 single-system persistent, 4 four systems per "
+                         "workgroup, 5 the same with the lean LDS layout (three workgroups per CU)")
     ap.add_argument("--bc", choices=("2d", "1d"), default="2d",
                     help="boundary data and initial guess: 2d = the reference's 2-D TE slice "
                          "solves (BC_x0_WS) on the device; 1d = a plane wave over a halfspace")
@@ -281,8 +282,9 @@ def main():
 
     dom = max(kt, key=lambda k: kt[k][1])
     nl, kms, kby = kt[dom]
-    rb_kernel = ("k_dilu_rbm<float,float,RED_FWD|BLACK_FB|RED_BWD,4>" if args.dilu_variant == 4
-                 else "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD>")
+    rb_kernel = {4: "k_dilu_rbm<float,float,RED_FWD|BLACK_FB|RED_BWD,4>",
+                 5: "k_dilu_rbl<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (lean LDS)",
+                 1: "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD>"}[args.dilu_variant]
     pc_kernels = {0: "k_dilu_tile<float,float,8,8,8,true|false> (block-DILU level sweeps)",
                   3: rb_kernel + " (red-black tile block-DILU, 3 launches per application)"}
     spv = 3 if args.spmv_variant is None else args.spmv_variant

@@ -317,9 +317,10 @@ int mt_bench_precond(mt_ctx *ctx, int nsys, int precision, int precond, int reps
 int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
 /* Launch shape of the red-black DILU (MT_PC_DILU_RB) sweeps: 1 = one system
  * per single-wave persistent workgroup with the next item prefetched into
- * registers; 4 (default) = four systems of one tile per workgroup sharing the
- * tile's couplings in LDS.  Same arithmetic.  No counterpart in the
- * reference. */
+ * registers; 4 = four systems of one tile per workgroup sharing the tile's
+ * couplings in LDS; 5 (default) = the same with the lean LDS layout (three
+ * workgroups per CU).  The same operator; 5 rounds the black tiles' backward
+ * sweep differently.  No counterpart in the reference. */
 int mt_set_dilu_variant(mt_ctx *ctx, int variant);
 /* Fixes the row-tile group count per system (0: automatic). */
 int mt_set_tuning(mt_ctx *ctx, int groups_per_system);

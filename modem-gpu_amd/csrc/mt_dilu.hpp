@@ -850,6 +850,303 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbm(DiluArgs<T, V> a, const Ti
   }
 }
 
+// ------------------------------ red-black sweeps, four systems, lean LDS
+// The launches of k_dilu_rbm with the LDS of a workgroup cut below a third of
+// a CU's 160 KB, so that three workgroups (12 waves) share a CU: the sweeps
+// are bound by LDS latency, which resident waves hide.
+//  * The right-hand side lives in the box in place: a point's forward step
+//    reads its own box entry, so the per-point data shrink to the pivots
+//    (red forward then keeps y in the pivot's slot, which it no longer needs).
+//  * The box is 9 x 9 x 9: the tile and one pad layer on the low side.  An
+//    upper-neighbour index past the end wraps to the start, which puts the
+//    high faces in low-side pad slots, disjoint from one another and from the
+//    tile.  A mode reads one side's faces from the box (red forward none,
+//    black the low ones, red backward the high ones); the other side's terms
+//    are folded into the right-hand side before the sweeps by the lanes that
+//    loaded the face values, one direction after another, which is the
+//    per-point order of the k_dilu_rbm folds.
+//  * Black backward: z = (dv y) - dv (sum of the upper terms), from the box's
+//    dv y (the same operator as dv (y - sum), rounded differently).  Red
+//    forward and red backward round as k_dilu_rbm does.
+//  * System tables per workgroup.
+template <class T, class V, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const TileShape* shp) {
+  constexpr int TX = 8, NP = 512, NU = 8, NLEV = 22, BX = 9, NB = 729;
+  constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
+  constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
+  constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
+  struct Cpl { V c[3]; };
+  __shared__ C2<T> Wall[NW][NB];                 // per system: the box
+  __shared__ C2<T> DVall[NW][NP];                // per system: pivots (red forward: then y)
+  __shared__ Cpl CL[FWDP ? NP : 1];              // raw lower couplings (slots 1, 3, 5)
+  __shared__ Cpl CH[BWDP ? NP : 1];              // raw upper couplings (slots 2, 4, 6)
+  __shared__ int s_sys[NW], s_ok[NW], s_fidx[NW];
+  __shared__ C2<T> s_al[NW];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int ngrp = (a.nact + NW - 1) / NW;
+  const int nwork = a.ntiles * ngrp;
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3;
+  const int chunk = (nwork + 7) / 8;
+  const int w = xcd * chunk + lb;
+  if (lb >= chunk || w >= nwork) return;   // uniform over the workgroup
+  const int tix = w / ngrp, grp = w - tix * ngrp;
+  if (tid < NW) {
+    const int si = grp * NW + tid;
+    int sy = 0, ok = 0, fi = 0;
+    C2<T> al = {T(0), T(0)};
+    if (si < a.nact) {
+      sy = a.act[si];
+      const SysScal& sc = a.sc[sy];
+      ok = sc.status == a.status_req;
+      fi = sc.fidx;
+      al = ccast<T>(sc.alpha);
+    }
+    s_sys[tid] = sy;
+    s_ok[tid] = ok;
+    s_fidx[tid] = fi;
+    s_al[tid] = al;
+  }
+  const int4 tl = a.tiles[tix];
+  const LatticeInfo L = a.lat[tl.x];
+  const int gi0 = tl.y * TX, gj0 = tl.z * TX, gk0 = tl.w * TX;
+  const int ex = min(TX, L.A - gi0), ey = min(TX, L.B - gj0), ez = min(TX, L.C - gk0);
+  const V* __restrict__ cfs = a.coef;
+  const int64_t cst = a.cstride;
+  const int st[3] = {1, BX, BX * BX};
+  // ---- the tile's couplings, once for the NW systems
+  for (int p = tid; p < NP; p += 64 * NW) {
+    const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+    Cpl c, h;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) c.c[s] = h.c[s] = V(0);
+    if (li < ex && lj < ey && lk < ez) {
+      const int64_t e = lat_e(L, gi0 + li, gj0 + lj, gk0 + lk);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        if (FWDP) c.c[s] = cfs[(int64_t)(1 + 2 * s) * cst + e];
+        if (BWDP) h.c[s] = cfs[(int64_t)(2 + 2 * s) * cst + e];
+      }
+    }
+    if (FWDP) CL[p] = c;
+    if (BWDP) CH[p] = h;
+  }
+  __syncthreads();   // s_ tables
+  const bool mine = s_ok[wv] != 0;   // 0 for the slots past nact
+  const int sys = s_sys[wv];
+  C2<T>* __restrict__ W = Wall[wv];
+  C2<T>* __restrict__ DV = DVall[wv];
+  const int64_t o = (int64_t)sys * a.vstride;
+  const C2<T>* __restrict__ pout = a.out + o;
+  // face values kept for the fold (black: the high faces; red backward: the
+  // low faces and the lower couplings of the tile points next to them)
+  C2<T> fold_v[3];
+  T fold_c[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    fold_v[s] = {T(0), T(0)};
+    fold_c[s] = T(0);
+  }
+  if (mine) {
+    for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
+    const C2<T>* __restrict__ pin = a.in + o;
+    const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
+    const C2<T>* __restrict__ dinv = a.dinv + (int64_t)s_fidx[wv] * cst;
+    const C2<T> al = s_al[wv];
+    C2<T> rv[NU], v2[NU], dd[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      rv[u] = v2[u] = dd[u] = {T(0), T(0)};
+      if (li < ex && lj < ey && lk < ez) {
+        const int64_t e = lat_e(L, gi0 + li, gj0 + lj, gk0 + lk);
+        rv[u] = FWDP ? pin[e] : pout[e];
+        if (FWDP && pin2) v2[u] = pin2[e];
+        dd[u] = dinv[e];
+      }
+    }
+    C2<T> fv[6], fd[6];
+    if (HALO) {
+      const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int ti, tj, tk;   // tile point next to the face
+        if (dir == 0) { ti = hi ? 7 : 0; tj = fa; tk = fb; }
+        else if (dir == 1) { ti = fa; tj = hi ? 7 : 0; tk = fb; }
+        else { ti = fa; tj = fb; tk = hi ? 7 : 0; }
+        const int ni = gi0 + ti + (dir == 0 ? (hi ? 1 : -1) : 0);
+        const int nj = gj0 + tj + (dir == 1 ? (hi ? 1 : -1) : 0);
+        const int nk = gk0 + tk + (dir == 2 ? (hi ? 1 : -1) : 0);
+        fv[f] = fd[f] = {T(0), T(0)};
+        const bool pin_ok = ti < ex && tj < ey && tk < ez;
+        const bool ok = pin_ok && ni >= 0 && nj >= 0 && nk >= 0 && ni < L.A && nj < L.B && nk < L.C;
+        if (ok) {
+          if (dir == 0) {
+            // the neighbour tile's facing i-face, packed (entry tj + 8*tk = lane)
+            const int tn = L.tbase + (tl.y + (hi ? 1 : -1)) + L.ntx * (tl.z + L.nty * tl.w);
+            fv[f] = a.xf[(MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys +
+                         (int64_t)tn * 128 + (hi ? 0 : 64) + lane];
+          } else {
+            const int64_t l = lat_e(L, ni, nj, nk);
+            fv[f] = pout[l];
+            if (MODE == RB_BLACK_FB) fd[f] = dinv[l];
+          }
+        }
+        if (MODE == RB_RED_BWD && !hi && pin_ok)
+          fold_c[dir] = T(cfs[(int64_t)(1 + 2 * dir) * cst +
+                              lat_e(L, gi0 + ti, gj0 + tj, gk0 + tk)]);
+      }
+    }
+    // ---- stage into LDS: the right-hand side in the box, the pivots
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+      C2<T> r = rv[u];
+      if (FWDP && pin2) r = csub(r, cmul(al, v2[u]));
+      W[q] = r;
+      DV[p] = dd[u];
+    }
+    if (HALO) {
+      const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int ti, tj, tk;
+        if (dir == 0) { ti = hi ? 7 : 0; tj = fa; tk = fb; }
+        else if (dir == 1) { ti = fa; tj = hi ? 7 : 0; tk = fb; }
+        else { ti = fa; tj = fb; tk = hi ? 7 : 0; }
+        // black: red neighbours enter as dinv*y (the packed i-faces already are)
+        const C2<T> v = (MODE == RB_BLACK_FB && dir != 0) ? cmul(fd[f], fv[f]) : fv[f];
+        const bool inbox = (MODE == RB_BLACK_FB) ? !hi : hi;
+        if (!inbox) {
+          fold_v[dir] = v;
+          continue;
+        }
+        if (!(ti < ex && tj < ey && tk < ez)) continue;
+        const int qp = (ti + 1) + BX * ((tj + 1) + BX * (tk + 1));
+        int qs = hi ? qp + st[dir] : qp - st[dir];
+        if (qs >= NB) qs -= NB;
+        W[qs] = v;
+      }
+    }
+  }
+  __syncthreads();   // the shared couplings; the rest is per wave
+  if (!mine) return;
+  // fold the face terms the sweeps do not read, one direction after another:
+  // black forward, the high faces (red neighbours above, upper couplings);
+  // red backward, the low faces (black neighbours below, lower couplings)
+  if (HALO) {
+    const int fa = lane & 7, fb = lane >> 3;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const int t0 = (MODE == RB_BLACK_FB) ? TX - 1 : 0;
+      int ti, tj, tk;
+      if (s == 0) { ti = t0; tj = fa; tk = fb; }
+      else if (s == 1) { ti = fa; tj = t0; tk = fb; }
+      else { ti = fa; tj = fb; tk = t0; }
+      if (ti < ex && tj < ey && tk < ez) {
+        const int p = ti + TX * (tj + TX * tk);
+        const int q = (ti + 1) + BX * ((tj + 1) + BX * (tk + 1));
+        const T c = (MODE == RB_BLACK_FB) ? T(CH[p].c[s]) : fold_c[s];
+        const C2<T> v = fold_v[s];
+        C2<T> r = W[q];
+        r.x -= c * v.x;
+        r.y -= c * v.y;
+        W[q] = r;
+      }
+      wave_sync();
+    }
+  }
+  // this lane's point on every hyperplane of the tile (at most 48 per plane)
+  unsigned int pk[NLEV];
+#pragma unroll
+  for (int h = 0; h < NLEV; ++h) {
+    const int b = shp->lvl[h], e = shp->lvl[h + 1];
+    pk[h] = (b + lane < e) ? shp->pts[b + lane] : 0xffffffffu;
+  }
+  if (FWDP) {
+#pragma unroll
+    for (int h = 0; h < NLEV; ++h) {
+      const unsigned int pt = pk[h];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+      if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
+        const int p = li + TX * (lj + TX * lk);
+        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        const C2<T> dvp = DV[p];
+        const Cpl c4 = CL[p];
+        // red: face neighbours are upper, so only in-tile lower couplings;
+        // black: every lower-side coupling (in tile or across the face)
+        const bool in[3] = {li > 0, lj > 0, lk > 0};
+        C2<T> y = W[q];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T c = (MODE == RB_BLACK_FB || in[s]) ? T(c4.c[s]) : T(0);
+          const C2<T> v = W[q - st[s]];
+          y.x -= c * v.x;
+          y.y -= c * v.y;
+        }
+        W[q] = cmul(dvp, y);
+        if (MODE == RB_RED_FWD) DV[p] = y;
+      }
+      wave_sync();
+    }
+  }
+  if (BWDP) {
+#pragma unroll
+    for (int h = NLEV - 1; h >= 0; --h) {
+      const unsigned int pt = pk[h];
+      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+      if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
+        const int p = li + TX * (lj + TX * lk);
+        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        const C2<T> dvp = DV[p];
+        const Cpl h4 = CH[p];
+        // black: upper neighbours in the tile only; red: in tile or across
+        const bool up[3] = {li + 1 < ex, lj + 1 < ey, lk + 1 < ez};
+        C2<T> y = (MODE == RB_RED_BWD) ? W[q] : C2<T>{T(0), T(0)};
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T c = (MODE == RB_RED_BWD || up[s]) ? T(h4.c[s]) : T(0);
+          int qn = q + st[s];
+          if (qn >= NB) qn -= NB;
+          const C2<T> z = W[qn];
+          y.x -= c * z.x;
+          y.y -= c * z.y;
+        }
+        if constexpr (MODE == RB_RED_BWD) {
+          W[q] = cmul(dvp, y);
+        } else {
+          // y holds -(sum of the upper terms); the box holds dv*y_forward
+          const C2<T> d = cmul(dvp, y);
+          const C2<T> b = W[q];
+          W[q] = {b.x + d.x, b.y + d.y};
+        }
+      }
+      wave_sync();
+    }
+  }
+  // ---- write the tile back (red forward: y; otherwise z), and the packed
+  // i-faces the other colour reads next (red forward: dinv*y; black: z)
+  C2<T>* __restrict__ dst = a.out + o;
+  C2<T>* __restrict__ xfo =
+      a.xf + (MODE == RB_RED_FWD ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys +
+      (int64_t)(L.tbase + tl.y + L.ntx * (tl.z + L.nty * tl.w)) * 128;
+  for (int p = lane; p < NP; p += 64) {
+    const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+    if (!(li < ex && lj < ey && lk < ez)) continue;
+    const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+    const C2<T> zv = (MODE == RB_RED_FWD) ? DV[p] : W[q];
+    dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = zv;
+    if (MODE != RB_RED_BWD && (li == 0 || li == TX - 1))
+      xfo[(li == 0 ? 0 : 64) + lj + 8 * lk] = W[q];   // red forward: dinv*y; black: z
+  }
+}
+
 // ------------------------------------------------------------- pivots
 // PC_setup's block-DILU pivots on the device (Dilu_Cmplx_AS, spOpTools.f90:
 // 1996-2055, on the x-x, y-y, z-z blocks of A(omega)):

@@ -106,8 +106,9 @@ struct mt_ctx {
   // sigma-free assembly data are on the device (workspaces cond_*)
   bool host_stale = false;
   bool cond_ready = false;
-  int rb_variant = 4;           // red-black DILU: 1 persistent single-system
-                                // workgroups; 4 four systems per workgroup
+  int rb_variant = 5;           // red-black DILU: 1 persistent single-system
+                                // workgroups; 4 four systems per workgroup;
+                                // 5 the same in the lean LDS layout
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
   // SpMV launch and every DILU sweep of the BiCGStab engine
   bool timing = false;
@@ -483,13 +484,19 @@ static void dilu_rb_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
     if (a.ntiles == 0) continue;
-    if (ctx->rb_variant == 4) {
+    if (ctx->rb_variant == 4 || ctx->rb_variant == 5) {
       constexpr int NW = 4;
       const int64_t nwork = int64_t(a.ntiles) * ((a.nact + NW - 1) / NW);
       const int g = int(8 * ((nwork + 7) / 8));
-      if (step == 0) k_dilu_rbm<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-      else if (step == 1) k_dilu_rbm<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-      else k_dilu_rbm<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      if (ctx->rb_variant == 5) {
+        if (step == 0) k_dilu_rbl<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else if (step == 1) k_dilu_rbl<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else k_dilu_rbl<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      } else {
+        if (step == 0) k_dilu_rbm<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else if (step == 1) k_dilu_rbm<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else k_dilu_rbm<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      }
       continue;
     }
     const int g = rb_grid(ctx, a.ntiles * a.nact);
@@ -1936,8 +1943,8 @@ int mt_bench_precond(mt_ctx* ctx, int nsys, int precision, int precond, int reps
 }
 
 int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
-  if (!ctx || (variant != 1 && variant != 4))
-    return mt_fail(ctx, MT_E_ARG, "mt_set_dilu_variant: variant is 1 or 4");
+  if (!ctx || (variant != 1 && variant != 4 && variant != 5))
+    return mt_fail(ctx, MT_E_ARG, "mt_set_dilu_variant: variant is 1, 4 or 5");
   ctx->rb_variant = variant;
   return MT_OK;
 }

@@ -178,13 +178,25 @@ def test_kernel_timing_graph_and_eager(gpu_ctx):
     assert gpu_ctx.kernel_timing(mtsp2.KT_SPMV) == (0, 0.0, 0.0)
 
 
-def test_dilu_rb_apply_inverts_its_factorisation(gpu_ctx):
+@pytest.fixture
+def dilu_variant(gpu_ctx, request):
+    """Launch shape of the red-black DILU for one test (restored after)."""
+    gpu_ctx.set_dilu_variant(request.param)
+    yield request.param
+    gpu_ctx.set_dilu_variant(5)   # the library's default
+
+
+@pytest.mark.parametrize("dilu_variant", [4, 5, 1], indirect=True)
+def test_dilu_rb_apply_inverts_its_factorisation(gpu_ctx, dilu_variant):
     """MT_PC_DILU_RB applies M^-1 with M = (P+E) P^-1 (P+F), E and F the
     strictly lower and upper parts of the x-x, y-y, z-z blocks of A(omega) in
     the red-black tile ordering and P the DILU pivots of that ordering
     (dilu_factor, pinned on the natural ordering against the oracle's
-    PC_Lsolve/PC_Usolve in tests/test_oracle_invariants.py): M z = x."""
-    print("case dilu-rb apply", flush=True)
+    PC_Lsolve/PC_Usolve in tests/test_oracle_invariants.py): M z = x, for
+    every launch shape (1 single-system workgroups, 4 four systems per
+    workgroup, 5 the same with the lean LDS layout).  The grid has partial
+    tiles in every direction."""
+    print(f"case dilu-rb apply variant={dilu_variant}", flush=True)
     dx, dy, dz, nza, lc = small_problem(12, nx=20, ny=18, nzAir=4, nzEarth=10)
     o = sp2oracle.Problem(dx, dy, dz, nza, lc)
     gpu_ctx.assemble(dx, dy, dz, nza, lc)
@@ -197,11 +209,31 @@ def test_dilu_rb_apply_inverts_its_factorisation(gpu_ctx):
     assert rel(dilu_mult(piv, lower, upper, z), x) <= 1e-12
 
 
+def test_dilu_rb_lean_layout_matches(gpu_ctx):
+    """The lean-LDS launches (variant 5) against the four-systems launches
+    (variant 4) on a grid of full and partial tiles: the same operator (the
+    black backward sweep rounds differently), FP64."""
+    print("case dilu-rb variants 4/5", flush=True)
+    dx, dy, dz, nza, lc = small_problem(21, nx=26, ny=17, nzAir=5, nzEarth=12)
+    gpu_ctx.assemble(dx, dy, dz, nza, lc)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(gpu_ctx.n_int) + 1j * rng.standard_normal(gpu_ctx.n_int)
+    try:
+        zs = []
+        for v in (4, 5):
+            gpu_ctx.set_dilu_variant(v)
+            zs.append(gpu_ctx.precond_apply(2 * np.pi * 0.3, x, mtsp2.PC_DILU_RB))
+    finally:
+        gpu_ctx.set_dilu_variant(5)   # the library's default
+    assert rel(zs[1], zs[0]) <= 1e-13
+
+
+@pytest.mark.parametrize("dilu_variant", [4, 5], indirect=True)
 @pytest.mark.parametrize("precision", [mtsp2.PREC_FP64, mtsp2.PREC_MIXED])
-def test_batch_dilu_rb_converges(gpu_ctx, precision):
+def test_batch_dilu_rb_converges(gpu_ctx, precision, dilu_variant):
     """Solves preconditioned with the red-black tile DILU reach the FP64 true
     residual tolerance and agree with the oracle's solution."""
-    print(f"case batch dilu-rb precision={precision}", flush=True)
+    print(f"case batch dilu-rb precision={precision} variant={dilu_variant}", flush=True)
     o, g = _setup(gpu_ctx, 13, nx=20, ny=18, nzAir=4, nzEarth=10)
     grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
     om, B, X0 = P.forward_systems(gpu_ctx, grid, [1e-2, 10.0])
