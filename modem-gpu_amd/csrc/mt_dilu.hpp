@@ -931,6 +931,22 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
     if (BWDP) CH[p] = h;
   }
   __syncthreads();   // s_ tables
+  // this lane's point on every hyperplane of the tile (at most 48 per plane),
+  // requested before the staging loads so that their latencies overlap: the
+  // level bounds in one batch, then unconditional loads (index clamped into
+  // the table) and a select
+  unsigned int pk[NLEV];
+  {
+    int lv[NLEV + 1];
+#pragma unroll
+    for (int h = 0; h <= NLEV; ++h) lv[h] = shp->lvl[h];
+#pragma unroll
+    for (int h = 0; h < NLEV; ++h) {
+      const int idx = lv[h] + lane;
+      const unsigned int v = shp->pts[min(idx, kMaxTilePts - 1)];
+      pk[h] = idx < lv[h + 1] ? v : 0xffffffffu;
+    }
+  }
   const bool mine = s_ok[wv] != 0;   // 0 for the slots past nact
   const int sys = s_sys[wv];
   C2<T>* __restrict__ W = Wall[wv];
@@ -1061,13 +1077,6 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
       }
       wave_sync();
     }
-  }
-  // this lane's point on every hyperplane of the tile (at most 48 per plane)
-  unsigned int pk[NLEV];
-#pragma unroll
-  for (int h = 0; h < NLEV; ++h) {
-    const int b = shp->lvl[h], e = shp->lvl[h + 1];
-    pk[h] = (b + lane < e) ? shp->pts[b + lane] : 0xffffffffu;
   }
   if (FWDP) {
 #pragma unroll
