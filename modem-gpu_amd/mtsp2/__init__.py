@@ -409,6 +409,9 @@ class Context:
         self._chk(lib().mt_set_spmv_variant(self._h, int(variant), int(layers_per_segment)))
 
     def set_dilu_variant(self, variant):
+        """Launch shape of the red-black DILU sweeps: 1 one system per
+        single-wave workgroup, 4 four systems per workgroup, 5 (default) four
+        systems per workgroup in the lean LDS layout."""
         self._chk(lib().mt_set_dilu_variant(self._h, int(variant)))
 
     def set_tuning(self, groups):
