@@ -7,7 +7,8 @@
 ! meaning of cuBiCG(b, x, KSPiter, device_idx) (solver.f90:931-965) that
 ! takes the operator explicitly (AAii as the reference's 1-based CSR, the
 ! frequency-free diagonal VomegaMuSig/omega and omega), keeps one library
-! context per device and re-uploads the operator only when it changes.
+! context per device and re-uploads the operator only when it changes (an
+! exact byte comparison against a host copy of the operator last uploaded).
 module mtsp2_fortran
    use iso_c_binding
    implicit none
@@ -16,7 +17,8 @@ module mtsp2_fortran
    integer(c_int), parameter, public :: MT_OK = 0, MT_E_ARG = 1, MT_E_DEVICE = 2, &
         MT_E_OOM = 3, MT_E_NAN_RHS = 4, MT_E_BREAKDOWN = 5, MT_E_MAXIT = 6, &
         MT_E_STATE = 7, MT_E_STENCIL = 8, MT_E_PIVOT = 9
-   integer(c_int), parameter, public :: MT_PC_DILU = 0, MT_PC_JACOBI = 1, MT_PC_NONE = 2
+   integer(c_int), parameter, public :: MT_PC_DILU = 0, MT_PC_JACOBI = 1, MT_PC_NONE = 2, &
+        MT_PC_DILU_RB = 3
    integer(c_int), parameter, public :: MT_PREC_FP64 = 0, MT_PREC_MIXED = 1
 
    type, bind(C), public :: mt_solve_opts
@@ -38,7 +40,7 @@ module mtsp2_fortran
              mt_lmultT, mt_dev_alloc, mt_dev_free, mt_dev_copy, mt_assemble, mt_update_cond, &
              mt_bc_x0_dev, mt_rhs_fwd_dev
    public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, mtsp2_data_resp, &
-             mtsp2_bc_x0
+             mtsp2_bc_x0, mtsp2_operator_reset
 
    interface
       subroutine mt_default_opts(opts) bind(C, name='mt_default_opts')
@@ -208,12 +210,25 @@ module mtsp2_fortran
          import :: c_size_t, c_ptr
          type(c_ptr), value :: s
       end function
+      integer(c_int) function c_memcmp(a, b, n) bind(C, name='memcmp')
+         import :: c_int, c_ptr, c_size_t
+         type(c_ptr), value :: a, b
+         integer(c_size_t), value :: n
+      end function
    end interface
 
-   ! one context per device, with the identity of the operator last uploaded
+   ! one context per device, with a host copy of the operator last uploaded
+   ! to it (grid, AAii as 1-based CSR, vms1): a solve re-uploads the operator
+   ! unless every byte is unchanged
    integer, parameter :: MAXDEV = 16
    type(c_ptr), save :: ctxs(0:MAXDEV-1) = c_null_ptr
-   integer(c_int64_t), save :: op_key(3, 0:MAXDEV-1) = 0
+   type :: op_copy
+      logical :: valid = .false.
+      integer :: nx = -1, ny = -1, nz = -1
+      integer(c_int), allocatable :: row(:), col(:)
+      real(c_double), allocatable :: val(:), vms1(:)
+   end type
+   type(op_copy), save, target :: ops(0:MAXDEV-1)
 
 contains
 
@@ -233,17 +248,15 @@ contains
    end function
 
    ! Context for device_idx with the operator (AAii, vms1) uploaded.  The
-   ! operator is re-uploaded when its size or contents change (a new sigma).
+   ! operator is re-uploaded when the grid, the CSR pattern, the values or
+   ! vms1 differ in any byte from the copy kept of the last upload.
    subroutine get_ctx(device_idx, nx, ny, nz, row, col, val, vms1, ctx, ierr)
       integer, intent(in)                 :: device_idx, nx, ny, nz
-      integer(c_int), intent(in)          :: row(:), col(:)
-      real(c_double), intent(in)          :: val(:), vms1(:)
+      integer(c_int), intent(in), target  :: row(:), col(:)
+      real(c_double), intent(in), target  :: val(:), vms1(:)
       type(c_ptr), intent(out)            :: ctx
       integer, intent(out)                :: ierr
-      integer(c_int64_t) :: key(3)
       integer(c_int) :: rc
-      real(c_double) :: h
-      integer :: i
       ierr = MT_OK
       if (device_idx < 0 .or. device_idx >= MAXDEV) then
          ierr = MT_E_DEVICE
@@ -257,26 +270,61 @@ contains
          endif
       endif
       ctx = ctxs(device_idx)
-      h = 0.0d0
-      do i = 1, size(val)
-         h = h + val(i) * dble(mod(i, 9973) + 1)
-      enddo
-      do i = 1, size(vms1)
-         h = h + vms1(i) * dble(mod(i, 7919) + 1)
-      enddo
-      key(1) = size(val)
-      key(2) = transfer(h, key(2))
-      key(3) = int(nx, c_int64_t) + 4096_c_int64_t * (int(ny, c_int64_t) + 4096_c_int64_t * nz)
-      if (all(key == op_key(:, device_idx))) return
+      if (same_operator(ops(device_idx), nx, ny, nz, row, col, val, vms1)) return
+      ops(device_idx)%valid = .false.
       rc = mt_set_grid(ctx, int(nx, c_int), int(ny, c_int), int(nz, c_int))
       if (rc == MT_OK) rc = mt_operator_set_csr(ctx, int(size(row) - 1, c_int), int(size(col), c_int), &
                                                row, col, 1_c_int, val, vms1)
       if (rc /= MT_OK) then
          ierr = rc
-         op_key(:, device_idx) = 0
          return
       endif
-      op_key(:, device_idx) = key
+      ops(device_idx)%nx = nx
+      ops(device_idx)%ny = ny
+      ops(device_idx)%nz = nz
+      ops(device_idx)%row = row
+      ops(device_idx)%col = col
+      ops(device_idx)%val = val
+      ops(device_idx)%vms1 = vms1
+      ops(device_idx)%valid = .true.
+   end subroutine
+
+   ! .true. when (grid, row, col, val, vms1) equal the kept copy byte for byte.
+   logical function same_operator(o, nx, ny, nz, row, col, val, vms1)
+      type(op_copy), intent(in), target   :: o
+      integer, intent(in)                 :: nx, ny, nz
+      integer(c_int), intent(in), target  :: row(:), col(:)
+      real(c_double), intent(in), target  :: val(:), vms1(:)
+      same_operator = .false.
+      if (.not. o%valid) return
+      if (o%nx /= nx .or. o%ny /= ny .or. o%nz /= nz) return
+      if (size(o%row) /= size(row) .or. size(o%col) /= size(col) .or. &
+          size(o%val) /= size(val) .or. size(o%vms1) /= size(vms1)) return
+      if (.not. same_bytes(c_loc(o%row), c_loc(row), 4_c_size_t * size(row, kind=c_size_t))) return
+      if (.not. same_bytes(c_loc(o%col), c_loc(col), 4_c_size_t * size(col, kind=c_size_t))) return
+      if (.not. same_bytes(c_loc(o%val), c_loc(val), 8_c_size_t * size(val, kind=c_size_t))) return
+      if (.not. same_bytes(c_loc(o%vms1), c_loc(vms1), 8_c_size_t * size(vms1, kind=c_size_t))) return
+      same_operator = .true.
+   end function
+
+   logical function same_bytes(a, b, n)
+      type(c_ptr), intent(in)       :: a, b
+      integer(c_size_t), intent(in) :: n
+      same_bytes = .true.
+      if (n > 0) same_bytes = c_memcmp(a, b, n) == 0
+   end function
+
+   ! Drops the kept operator copy of device_idx: the next solve uploads the
+   ! operator it is given (for callers that change AAii in place between
+   ! solves and want to say so; the byte comparison catches that as well).
+   subroutine mtsp2_operator_reset(device_idx)
+      integer, intent(in) :: device_idx
+      if (device_idx < 0 .or. device_idx >= MAXDEV) return
+      ops(device_idx)%valid = .false.
+      if (allocated(ops(device_idx)%row)) deallocate(ops(device_idx)%row)
+      if (allocated(ops(device_idx)%col)) deallocate(ops(device_idx)%col)
+      if (allocated(ops(device_idx)%val)) deallocate(ops(device_idx)%val)
+      if (allocated(ops(device_idx)%vms1)) deallocate(ops(device_idx)%vms1)
    end subroutine
 
    ! Drop-in for cuBiCG (solver.f90:931-2009).  Arguments follow solverControl_t
@@ -332,7 +380,7 @@ contains
          if (c_associated(ctxs(d))) then
             rc = mt_ctx_destroy(ctxs(d))
             ctxs(d) = c_null_ptr
-            op_key(:, d) = 0
+            call mtsp2_operator_reset(d)
          endif
       enddo
    end subroutine
@@ -448,6 +496,8 @@ contains
       integer(c_size_t) :: nb
       call dev_ctx(device_idx, ctx, ierr)
       if (ierr /= MT_OK) return
+      ! the context's operator is replaced: a later solve must upload its own
+      ops(device_idx)%valid = .false.
       ierr = mt_assemble(ctx, int(nx, c_int), int(ny, c_int), int(nzAir, c_int), &
                          int(nzEarth, c_int), dx, dy, dz, logcond, air_logcond)
       if (ierr /= MT_OK) return

@@ -32,9 +32,13 @@ def _amat(o, w):
     return A
 
 
+@pytest.mark.parametrize("precond", [mtsp2.PC_DILU, mtsp2.PC_DILU_RB])
 @pytest.mark.parametrize("precision", [mtsp2.PREC_FP64, mtsp2.PREC_MIXED])
-def test_transpose_batch(gpu_ctx, precision):
-    print(f"case transpose batch precision={precision}", flush=True)
+def test_transpose_batch(gpu_ctx, precision, precond):
+    """A^T solves with the block-DILU of A^T's blocks in the reference
+    ordering and in the red-black tile ordering (transposed stencil, pivots
+    of A^T) on a grid of partial 8x8x8 tiles."""
+    print(f"case transpose batch precision={precision} precond={precond}", flush=True)
     dx, dy, dz, nza, lc = small_problem(11, nx=11, ny=10, nzAir=4, nzEarth=8)
     o = sp2oracle.Problem(dx, dy, dz, nza, lc)
     gpu_ctx.assemble(dx, dy, dz, nza, lc)
@@ -42,7 +46,7 @@ def test_transpose_batch(gpu_ctx, precision):
     om = 2 * np.pi * np.array([1e-2, 1.0, 1.0, 50.0])
     S = rng.standard_normal((len(om), o.nInt)) + 1j * rng.standard_normal((len(om), o.nInt))
     L, stats = gpu_ctx.solve_batch(om, S, None, tol=1e-10, maxit=3000, precision=precision,
-                                   op_transpose=1)
+                                   op_transpose=1, precond=precond)
     for s, w in enumerate(om):
         assert stats[s]["status"] == 0, stats[s]
         A = _amat(o, w)

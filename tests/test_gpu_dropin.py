@@ -57,10 +57,11 @@ def test_fortran_binding_solve():
             x0.astype(np.complex128).tofile(f)
         subprocess.run([CHECK, fi, fo], check=True, timeout=120)
         with open(fo, "rb") as f:
-            niter, failed, ierr = np.fromfile(f, np.int32, 3)
+            niter, failed, ierr, same = np.fromfile(f, np.int32, 4)
             rerr = np.fromfile(f, np.float64, maxit)
             x = np.fromfile(f, np.complex128, o.nInt)
     assert ierr == 0 and failed == 0
+    assert same == 1, "a repeat solve with an unchanged operator gave other iterates"
     xo, info = o.bicg(b, x0, maxit=maxit, tol=tol, gpu_semantics=True)
     assert abs(int(niter) - info["niter"]) <= 3
     assert rel(x, xo) <= 1e-8
