@@ -402,3 +402,52 @@ def run_dropin_bc(grid, logcond, air_log, systems, timeout=300):
     st = int(np.fromfile(os.path.join(wd, "ierr.bin"), np.int32)[0])
     _cleanup(wd, True)
     return ref, lib, st
+
+
+JMULT_BIN = os.path.join(HERE, "_ref", "ref_jmultT")
+
+
+def jmultT_available() -> bool:
+    return os.access(JMULT_BIN, os.X_OK)
+
+
+def run_jmultT(grid, logcond, air_log, omega, sites, d, maxit=2000, tol=1e-10,
+               origin=(0.0, 0.0, 0.0), timeout=600):
+    """The reference's sensitivity chain for one MT transmitter (driver
+    oracle/ref/ref_jmultT.f90): forward solves of both modes (BC_x0_WS
+    boundary data, BICG to `tol`), LmultT of full-impedance data d (nsite, 8),
+    the TRN solves (sensSolve) and PmultT, plus JmultT_TX itself.  Returns
+    dict(fwd (2, nEdge), comb (2, nEdge), trn (2, nEdge), dsig, dsig_jt
+    (nx, ny, nzEarth))."""
+    dx = np.asarray(grid["dx"], np.float64)
+    dy = np.asarray(grid["dy"], np.float64)
+    dz = np.asarray(grid["dz"], np.float64)
+    nx, ny, nz = len(dx), len(dy), len(dz)
+    nza = int(grid["nzAir"])
+    nze = nz - nza
+    lc = np.asarray(logcond, np.float64).reshape(nx, ny, nze)
+    x = np.ascontiguousarray(np.asarray(sites, np.float64).reshape(-1, 3))
+    dd = np.ascontiguousarray(np.asarray(d, np.float64).reshape(len(x), 8))
+    wd = tempfile.mkdtemp(prefix="mtjt_")
+    inp = os.path.join(wd, "input.bin")
+    with open(inp, "wb") as f:
+        np.array([nx, ny, nza, nze], np.int32).tofile(f)
+        dx.tofile(f); dy.tofile(f); dz.tofile(f)
+        np.asarray(origin, np.float64).tofile(f)
+        lc.ravel(order="F").tofile(f)
+        np.array([air_log, omega], np.float64).tofile(f)
+        np.array([maxit], np.int32).tofile(f)
+        np.array([tol], np.float64).tofile(f)
+        np.array([len(x)], np.int32).tofile(f)
+        x.tofile(f)          # (nsite, 3) C order = x(3, nsite) Fortran order
+        dd.tofile(f)         # d(8, nsite) Fortran order
+    subprocess.run([JMULT_BIN, inp, wd], check=True, timeout=timeout,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, cwd=wd)
+    out = {}
+    for k in ("fwd", "comb", "trn"):
+        out[k] = np.fromfile(os.path.join(wd, k + ".bin"), np.complex128).reshape(2, -1)
+    for k in ("dsig", "dsig_jt"):
+        out[k] = np.fromfile(os.path.join(wd, k + ".bin"), np.float64).reshape((nx, ny, nze),
+                                                                              order="F")
+    _cleanup(wd, True)
+    return out

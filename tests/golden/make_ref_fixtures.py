@@ -6,10 +6,13 @@ TE boundary fields of BC_x0_WS (through ComputeBC) on that grid and on a deep
 one whose bandwidth takes LAPACK's blocked factorisation, and
 tests/golden/io/: small WS model files with what the reference's reader reads
 from them (ref_io.npz), and a solution file written by the reference's
-writer with its inputs.  Inputs and outputs only -- data, no reference code.
+writer with its inputs, and tests/golden/ref_jmultT.npz: the reference's
+sensitivity chain for one MT transmitter at two frequencies (forward fields,
+LmultT sources, the TRN solves of sensSolve, PmultT and JmultT_TX) on a small
+seeded grid.  Inputs and outputs only -- data, no reference code.
 
 Run where the reference is mounted:
-    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0] [io]
+    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0] [io] [jmultT]
 """
 import os
 import sys
@@ -274,8 +277,37 @@ def io_main():
     print("wrote", d)
 
 
+def jmult_case():
+    """Grid, model, sites and full-impedance data of the sensitivity fixture."""
+    sys.path.insert(0, os.path.join(ROOT, "modem-gpu_amd"))
+    from mtsp2 import problem as P
+    rng = np.random.default_rng(77)
+    grid = P.make_grid(10, 9, 6, nair=3, dcore=300.0, npad=2)
+    nx, ny = len(grid["dx"]), len(grid["dy"])
+    nze = len(grid["dz"]) - grid["nzAir"]
+    lc = np.log(0.01) + 0.8 * rng.standard_normal((nx, ny, nze))
+    sites = P.core_sites(grid, 3, npad=2)
+    omegas = 2 * np.pi * np.array([0.05, 20.0])
+    d = rng.standard_normal((len(omegas), len(sites), 8))
+    return grid, lc, sites, omegas, d
+
+
+def jmult_main():
+    grid, lc, sites, omegas, d = jmult_case()
+    out = dict(dx=grid["dx"], dy=grid["dy"], dz=grid["dz"], nzAir=grid["nzAir"], logcond=lc,
+               sites=sites, omegas=omegas, d=d, tol=1e-10, maxit=2000)
+    for k, w in enumerate(omegas):
+        r = refdriver.run_jmultT(grid, lc, np.log(1e-10), w, sites, d[k], maxit=2000, tol=1e-10)
+        for name, v in r.items():
+            out[f"{name}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "ref_jmultT.npz"), **out)
+    print("wrote", os.path.join(HERE, "ref_jmultT.npz"))
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["small", "datafunc", "bcx0", "io"]
+    which = sys.argv[1:] or ["small", "datafunc", "bcx0", "io", "jmultT"]
+    if "jmultT" in which:
+        jmult_main()
     if "small" in which:
         main()
     if "datafunc" in which:
