@@ -97,6 +97,30 @@ def config_model(name, grid, seed=0):
     return block_model(grid, seed) if name == "cfg0" else boxes_model(grid, seed=seed)
 
 
+# sites per axis on the core: 10 x 10 (cfg0), 20 x 20 = 400 (cfg1, cfg3), 12 x 12 = 144 (cfg4)
+CONFIG_SITES = {"cfg0": 10, "cfg1": 20, "cfg3": 20, "cfg4": 12}
+
+
+def core_sites(grid, nside, npad=8):
+    """nside x nside receivers on the surface (z = 0) spread over the core
+    cells (the padding excluded), in the grid frame with its origin at the
+    grid's corner (x = 0 at the first x node).  Each site sits 0.37 of a cell
+    from its cell's low corner, so no site lies on a node or a cell centre
+    (where the interpolation rule would meet ties)."""
+    out = []
+    for d in (grid["dx"], grid["dy"]):
+        nodes = np.concatenate([[0.0], np.cumsum(d)])
+        ncore = len(d) - 2 * npad
+        cells = npad + ((np.arange(nside) + 0.5) * ncore / nside).astype(int)
+        out.append(nodes[cells] + 0.37 * d[cells])
+    X, Y = np.meshgrid(out[0], out[1], indexing="ij")
+    return np.column_stack([X.ravel(), Y.ravel(), np.zeros(X.size)])
+
+
+def config_sites(name, grid):
+    return core_sites(grid, CONFIG_SITES[name])
+
+
 # ------------------------------------------------------------- edge layout
 def edge_counts(nx, ny, nz):
     nXe, nYe, nZe = nx * (ny + 1) * (nz + 1), (nx + 1) * ny * (nz + 1), (nx + 1) * (ny + 1) * nz
