@@ -207,10 +207,33 @@ int mt_rhs_fwd_dev(mt_ctx *ctx, int nsys, const void *e_dev, void *b_dev, void *
  * the context: the rows of U of the slices in flight, up to
  * 2*CUs*(nh-1)*(nz-1)*nz complex doubles (3.9 GB at 100x100x70 on 256 CUs). */
 int mt_bc_x0_dev(mt_ctx *ctx, int nsys, const double *omegas, const int *modes, void *e_dev);
-/* Transposed (adjoint) RHS b = s_i + V_E GDii(s_i (i*ISIGN*omega*mu0)^-1 / V_E). */
+/* Transposed (adjoint) RHS of FWDsolve3D's TRN branch with a dense source
+ * (EMsolve3D.f90:217-256): b = s_i + V_E GDii(s_i (i*ISIGN*omega*mu0)^-1 / V_E),
+ * s_all on all edges, b on interior edges.  Host buffers; the work runs in
+ * the kernels of mt_rhs_trn_dev.  Requires mt_assemble. */
 int mt_rhs_trn(mt_ctx *ctx, double omega, const mt_zdouble *s_all, mt_zdouble *b);
-/* TRN post-processing: e(EDGEi)=ei, e = V_E e, e(EDGEb) = -CCib^T ei. */
+/* TRN post-processing of FWDsolve3D without boundary data
+ * (EMsolve3D.f90:463-478; the reference's nonzero_BC branch, :473-474, is
+ * not covered: sensSolve never passes boundary data): e(EDGEi) = ei,
+ * e = V_E e, e(EDGEb) = -CCib^T ei.  Host buffers; kernels of
+ * mt_post_trn_dev. */
 int mt_post_trn(mt_ctx *ctx, const mt_zdouble *ei, mt_zdouble *e_all);
+
+/* Device forms for batches (nsys systems, system-major, device pointers):
+ *  mt_post_fwd_dev: FWDsolve3D's forward post-processing (EMsolve3D.f90:463,
+ *    481-485): e(EDGEi) = x on top of the boundary data already in e
+ *    (x [nsys][n_int], e [nsys][n_edge], e.g. from mt_bc_x0_dev).
+ *  mt_rhs_trn_dev: the TRN right-hand side above for each system (omegas[s],
+ *    s_dev [nsys][n_edge], b_dev [nsys][n_int]); gd_correct = 0 gives
+ *    b = s(EDGEi), the right-hand side of the exact transposed system
+ *    A^T lam = s (no GDii term, no omega needed).
+ *  mt_post_trn_dev: the TRN post-processing above (ei_dev [nsys][n_int],
+ *    e_dev [nsys][n_edge]).
+ * All require mt_assemble (sigma from the last mt_update_cond, if any). */
+int mt_post_fwd_dev(mt_ctx *ctx, int nsys, const void *x_dev, void *e_dev);
+int mt_rhs_trn_dev(mt_ctx *ctx, int nsys, const double *omegas, const void *s_dev, void *b_dev,
+                   int gd_correct);
+int mt_post_trn_dev(mt_ctx *ctx, int nsys, const void *ei_dev, void *e_dev);
 
 /* ---- divergence correction (SdivCorr, EMsolve3D.f90:507-613) -------------- */
 /* Node-Laplacian PCG (solver.f90:57-136) on VDsG = VDs*Gai with its DILU
@@ -227,6 +250,14 @@ int mt_divcor(mt_ctx *ctx, double omega, mt_zdouble *ei, const mt_zdouble *phi0,
  * into dm when accumulate != 0.  Requires mt_assemble. */
 int mt_pmultT(mt_ctx *ctx, double omega, int npol, const mt_zdouble *e0_all,
               const mt_zdouble *lam_all, double *dm, int accumulate);
+/* The same on the device for nfreq frequencies (JmultT summed over the
+ * transmitters, SensComp.f90:408-485 per transmitter): e0_dev, lam_dev
+ * [nfreq][npol][n_edge] complex (forward fields and TRN solutions as
+ * FWDsolve3D returns them, all edges), dm_dev nx*ny*nzEarth doubles on the
+ * device, overwritten (accumulate = 0) or added to.  The host form above
+ * stages its arrays through the same kernels. */
+int mt_pmultT_dev(mt_ctx *ctx, int nfreq, const double *omegas, int npol, const void *e0_dev,
+                  const void *lam_dev, void *dm_dev, int accumulate);
 
 /* Gradient of Re(sum_pol lam_pol^T A(omega) e_pol) w.r.t. the LOGE cell
  * parameters (nx*ny*nzEarth, Fortran order), accumulated into dm.  For a
@@ -244,6 +275,14 @@ int mt_pmultT(mt_ctx *ctx, double omega, int npol, const mt_zdouble *e0_all,
 #define MT_GRAD_EXACT 1
 int mt_gradient(mt_ctx *ctx, double omega, int npol, const mt_zdouble *e_all,
                 const mt_zdouble *lam, double *dm, int mode);
+/* The same on the device for nfreq frequencies: e_dev [nfreq][npol][n_edge]
+ * (forward fields, all edges), lam_dev [nfreq][npol][n_int] (adjoint
+ * solutions of A^T lam = s, interior edges), dm_dev nx*ny*nzEarth doubles on
+ * the device, overwritten (accumulate = 0) or added to.  The host form
+ * mt_gradient stages its arrays through the same kernels; mt_host_gradient
+ * is the host-only restatement the kernels are checked against. */
+int mt_gradient_dev(mt_ctx *ctx, int nfreq, const double *omegas, int npol, const void *e_dev,
+                    const void *lam_dev, void *dm_dev, int mode, int accumulate);
 /* Host-only variant (assembles the model itself; for small problems/tests). */
 int mt_host_gradient(int nx, int ny, int nzAir, int nzEarth, const double *dx, const double *dy,
                      const double *dz, const double *logcond, double air_logcond, double omega,

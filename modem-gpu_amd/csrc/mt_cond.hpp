@@ -19,6 +19,7 @@ namespace mt {
 
 struct CondArgs {
   int nx, ny, nz;
+  int nzAir;                  // air layers (the sensitivity kernels' earth cells start below)
   int64_t nXe, nYe, nE, nInt;
   const double *dx, *dy, *dz, *delX, *delY, *delZ;
   const double* csig;         // [nz][ny][nx] cell conductivity (air layers included)

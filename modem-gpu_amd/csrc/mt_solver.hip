@@ -21,6 +21,7 @@
 #include "mt_host_ops.hpp"
 #include "mt_internal.h"
 #include "mt_kernels.hpp"
+#include "mt_sens.hpp"
 #include "mt_sites.hpp"
 #include "mt_spmv.hpp"
 
@@ -106,6 +107,12 @@ struct mt_ctx {
   // sigma-free assembly data are on the device (workspaces cond_*)
   bool host_stale = false;
   bool cond_ready = false;
+  // sensitivity kernels: sens_fields -- sigma_E, sigma_N and the node weights
+  // of the current model are in the cond_* workspaces; sens_geom -- the
+  // sigma-free data (ghost volumes, edge -> row map, CCib transposed) are in
+  // the sens_* workspaces
+  bool sens_fields = false;
+  bool sens_geom = false;
   int rb_variant = 5;           // red-black DILU: 1 persistent single-system
                                 // workgroups; 4 four systems per workgroup;
                                 // 5 the same in the lean LDS layout
@@ -279,7 +286,10 @@ static int upload_operator(mt_ctx* ctx) {
   free_workspaces(ctx);
   ctx->rhs_ready = false;
   ctx->cond_ready = false;
+  ctx->sens_fields = false;
+  ctx->sens_geom = false;
   ctx->host_stale = false;
+  ctx->nnz_stencil = -1;  // counted again for the new operator on first use
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   ctx->dc = DivcorData();
@@ -1172,6 +1182,7 @@ int mt_set_grid(mt_ctx* ctx, int nx, int ny, int nz) {
   ctx->hop = HostOperator();
   ctx->hop.g.set(nx, ny, nz);
   free_dop(ctx->dop);
+  ctx->nnz_stencil = -1;
   return MT_OK;
 }
 
@@ -1607,60 +1618,57 @@ int mt_bc_x0_dev(mt_ctx* ctx, int nsys, const double* omegas, const int* modes, 
   return MT_OK;
 }
 
+// Host-buffer forms of the sensitivity entry points: the arrays are staged
+// through device workspaces and the device kernels do the work.
+static int stage_up(mt_ctx* ctx, const char* name, const void* h, size_t bytes, void** d) {
+  int rc = ws_get(ctx, name, bytes, d);
+  if (rc) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+  return MT_OK;
+}
+
 int mt_rhs_trn(mt_ctx* ctx, double omega, const mt_zdouble* s_all, mt_zdouble* b) {
   if (!ctx || !s_all || !b) return mt_fail(ctx, MT_E_ARG, "mt_rhs_trn: null argument");
-  const HostOperator& H = ctx->hop;
-  if (!H.have_model) return mt_fail(ctx, MT_E_STATE, "mt_rhs_trn needs mt_assemble");
-  { int rc = ensure_host(ctx); if (rc) return rc; }
-  const mt_zdouble iOmegaMuInv = 1.0 / mt_zdouble(0.0, kIsign * omega * kMu0);
-  std::vector<mt_zdouble> si(H.g.nE, 0.0);
-  for (int64_t r = 0; r < H.g.nInt; ++r) {
-    const int e = H.imap[r];
-    si[e] = s_all[e] * iOmegaMuInv / H.Vedge[e];
-  }
-  for (int64_t r = 0; r < H.g.nInt; ++r) {
-    const int e = H.imap[r];
-    mt_zdouble st = 0.0;
-    for (int q = H.gd_row[r]; q < H.gd_row[r + 1]; ++q) st += H.gd_val[q] * si[H.gd_col[q]];
-    b[r] = s_all[e] + st * H.Vedge[e];
-  }
+  if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_rhs_trn needs mt_assemble");
+  const Grid& g = ctx->hop.g;
+  void *ds, *db;
+  int rc;
+  if ((rc = stage_up(ctx, "io_s", s_all, g.nE * sizeof(Z), &ds)) ||
+      (rc = ws_get(ctx, "io_b", g.nInt * sizeof(Z), &db)))
+    return rc;
+  if ((rc = mt_rhs_trn_dev(ctx, 1, &omega, ds, db, 1))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpy(b, db, g.nInt * sizeof(Z), hipMemcpyDeviceToHost));
   return MT_OK;
 }
 
 int mt_post_trn(mt_ctx* ctx, const mt_zdouble* ei, mt_zdouble* e_all) {
   if (!ctx || !ei || !e_all) return mt_fail(ctx, MT_E_ARG, "mt_post_trn: null argument");
-  const HostOperator& H = ctx->hop;
-  if (!H.have_model) return mt_fail(ctx, MT_E_STATE, "mt_post_trn needs mt_assemble");
-  std::vector<mt_zdouble> sb(H.g.nE, 0.0);
-  for (int64_t q = 0; q < H.g.nE; ++q) e_all[q] = 0.0;
-  for (int64_t r = 0; r < H.g.nInt; ++r) {
-    e_all[H.imap[r]] = ei[r];
-    for (int q = H.ccib_row[r]; q < H.ccib_row[r + 1]; ++q) sb[H.ccib_col[q]] += H.ccib_val[q] * ei[r];
-  }
-  for (int64_t q = 0; q < H.g.nE; ++q) e_all[q] = H.Vedge[q] * e_all[q];
-  for (int e : H.bmap) e_all[e] = -sb[e];
+  if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_post_trn needs mt_assemble");
+  const Grid& g = ctx->hop.g;
+  void *di, *de;
+  int rc;
+  if ((rc = stage_up(ctx, "io_b", ei, g.nInt * sizeof(Z), &di)) ||
+      (rc = ws_get(ctx, "io_s", g.nE * sizeof(Z), &de)))
+    return rc;
+  if ((rc = mt_post_trn_dev(ctx, 1, di, de))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpy(e_all, de, g.nE * sizeof(Z), hipMemcpyDeviceToHost));
   return MT_OK;
 }
 
 int mt_pmultT(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e0, const mt_zdouble* lam,
               double* dm, int accumulate) {
   if (!ctx || !e0 || !lam || !dm || npol < 1) return mt_fail(ctx, MT_E_ARG, "mt_pmultT: bad arguments");
+  if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_pmultT needs mt_assemble");
   const HostOperator& H = ctx->hop;
-  if (!H.have_model) return mt_fail(ctx, MT_E_STATE, "mt_pmultT needs mt_assemble");
-  { int rc = ensure_host(ctx); if (rc) return rc; }
-  const int64_t nE = H.g.nE;
-  // minus_i_omega_mu = cmplx(0, -ISIGN*MU_0*omega) (SolverSens.f90:95)
-  const mt_zdouble c(0.0, -kIsign * kMu0 * omega);
-  std::vector<double> re(nE);
-  for (int64_t e = 0; e < nE; ++e) {
-    mt_zdouble acc = e0[e] * lam[e];
-    for (int k = 1; k < npol; ++k) acc += e0[size_t(k) * nE + e] * lam[size_t(k) * nE + e];
-    re[e] = (c * acc).real();
-  }
-  const int64_t nc = int64_t(H.g.nx) * H.g.ny * H.nzEarth;
-  std::vector<double> tmp(nc);
-  edge_to_model(H, re.data(), tmp.data());
-  for (int64_t q = 0; q < nc; ++q) dm[q] = accumulate ? dm[q] + tmp[q] : tmp[q];
+  const size_t nv = size_t(npol) * H.g.nE * sizeof(Z);
+  const size_t nc = size_t(H.g.nx) * H.g.ny * H.nzEarth * sizeof(double);
+  void *de, *dl, *dd;
+  int rc;
+  if ((rc = stage_up(ctx, "io_e", e0, nv, &de)) || (rc = stage_up(ctx, "io_l", lam, nv, &dl)) ||
+      (rc = stage_up(ctx, "io_dm", dm, nc, &dd)))
+    return rc;
+  if ((rc = mt_pmultT_dev(ctx, 1, &omega, npol, de, dl, dd, accumulate))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpy(dm, dd, nc, hipMemcpyDeviceToHost));
   return MT_OK;
 }
 
@@ -1701,9 +1709,16 @@ int mt_gradient(mt_ctx* ctx, double omega, int npol, const mt_zdouble* e_all,
   if (!ctx || !e_all || !lam || !dm || npol < 1 || mode < 0 || mode > 1)
     return mt_fail(ctx, MT_E_ARG, "mt_gradient: bad arguments");
   if (!ctx->hop.have_model) return mt_fail(ctx, MT_E_STATE, "mt_gradient needs mt_assemble");
-  { int rc = ensure_host(ctx); if (rc) return rc; }
-  grad_assemble(ctx->hop, omega, npol, reinterpret_cast<const zc*>(e_all),
-                reinterpret_cast<const zc*>(lam), mode, dm);
+  const HostOperator& H = ctx->hop;
+  const size_t nc = size_t(H.g.nx) * H.g.ny * H.nzEarth * sizeof(double);
+  void *de, *dl, *dd;
+  int rc;
+  if ((rc = stage_up(ctx, "io_e", e_all, size_t(npol) * H.g.nE * sizeof(Z), &de)) ||
+      (rc = stage_up(ctx, "io_l", lam, size_t(npol) * H.g.nInt * sizeof(Z), &dl)) ||
+      (rc = stage_up(ctx, "io_dm", dm, nc, &dd)))
+    return rc;
+  if ((rc = mt_gradient_dev(ctx, 1, &omega, npol, de, dl, dd, mode, 1))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpy(dm, dd, nc, hipMemcpyDeviceToHost));
   return MT_OK;
 }
 
@@ -2136,28 +2151,17 @@ int mt_lmultT(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const voi
 // ---------------------------------------------------------------------------
 // UpdateCond on the device
 // ---------------------------------------------------------------------------
-int mt_update_cond(mt_ctx* ctx, const double* logcond, double air_logcond) {
-  if (!ctx || !logcond) return mt_fail(ctx, MT_E_ARG, "mt_update_cond: null argument");
-  int rc = need_op(ctx);
-  if (rc) return rc;
+// The UpdateCond kernels' arguments over the cond_* workspaces: the grid's
+// geometry (uploaded once per grid) and, when logcond is not null, the cell
+// conductivities of that model (exp on the host, as the host assembly forms
+// them; air layers at exp(air_logcond)).
+static int cond_args(mt_ctx* ctx, const double* logcond, double air_logcond, CondArgs& a) {
   HostOperator& H = ctx->hop;
-  if (!H.have_model || H.coefCC.empty())
-    return mt_fail(ctx, MT_E_STATE, "mt_update_cond needs mt_assemble on this grid");
-  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
-  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   const Grid& g = H.g;
   const int nx = g.nx, ny = g.ny, nz = g.nz, nza = H.nzAir;
   const int64_t nN = int64_t(nx + 1) * (ny + 1) * (nz + 1), ncell = int64_t(nx) * ny * nz;
-  // cell conductivities as the host assembly forms them (exp on the host)
-  std::vector<double> cs(ncell);
-  const double airC = std::exp(air_logcond);
-  for (int k = 0; k < nz; ++k)
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i)
-        cs[i + int64_t(nx) * (j + int64_t(ny) * k)] =
-            k < nza ? airC : std::exp(logcond[i + int64_t(nx) * (j + int64_t(ny) * (k - nza))]);
-  CondArgs a;
-  a.nx = nx; a.ny = ny; a.nz = nz;
+  int rc;
+  a.nx = nx; a.ny = ny; a.nz = nz; a.nzAir = nza;
   a.nXe = g.nXe; a.nYe = g.nYe; a.nE = g.nE; a.nInt = g.nInt;
   double *ddx, *ddy, *ddz, *dlx, *dly, *dlz, *dcc, *dcs, *dsE, *dsN, *dm2;
   int* dslot;
@@ -2193,11 +2197,37 @@ int mt_update_cond(mt_ctx* ctx, const double* logcond, double air_logcond) {
     MT_HIP_CHECK(ctx, hipMemcpy(dslot, &slot[0][0][0], 36 * sizeof(int), hipMemcpyHostToDevice));
     ctx->cond_ready = true;
   }
-  MT_HIP_CHECK(ctx, hipMemcpyAsync(dcs, cs.data(), ncell * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-  DeviceOperator& d = ctx->dop;
+  if (logcond) {
+    std::vector<double> cs(ncell);
+    const double airC = std::exp(air_logcond);
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+          cs[i + int64_t(nx) * (j + int64_t(ny) * k)] =
+              k < nza ? airC : std::exp(logcond[i + int64_t(nx) * (j + int64_t(ny) * (k - nza))]);
+    MT_HIP_CHECK(ctx, hipMemcpy(dcs, cs.data(), ncell * sizeof(double), hipMemcpyHostToDevice));
+  }
   a.dx = ddx; a.dy = ddy; a.dz = ddz; a.delX = dlx; a.delY = dly; a.delZ = dlz;
-  a.csig = dcs; a.imap = d.imap; a.coefCC = dcc; a.slot = dslot;
+  a.csig = dcs; a.imap = ctx->dop.imap; a.coefCC = dcc; a.slot = dslot;
   a.sigE = dsE; a.earthE = dearth; a.sigN = dsN; a.m2 = dm2;
+  a.coef = nullptr; a.vms1 = nullptr;
+  return MT_OK;
+}
+
+int mt_update_cond(mt_ctx* ctx, const double* logcond, double air_logcond) {
+  if (!ctx || !logcond) return mt_fail(ctx, MT_E_ARG, "mt_update_cond: null argument");
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  HostOperator& H = ctx->hop;
+  if (!H.have_model || H.coefCC.empty())
+    return mt_fail(ctx, MT_E_STATE, "mt_update_cond needs mt_assemble on this grid");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  const Grid& g = H.g;
+  const int nx = g.nx, ny = g.ny;
+  CondArgs a;
+  if ((rc = cond_args(ctx, logcond, air_logcond, a))) return rc;
+  DeviceOperator& d = ctx->dop;
   a.coef = d.coef64; a.vms1 = d.vms64;
   k_cond_edges<<<2048, 256, 0, ctx->stream>>>(a);
   k_cond_nodes<<<1024, 256, 0, ctx->stream>>>(a);
@@ -2219,5 +2249,205 @@ int mt_update_cond(mt_ctx* ctx, const double* logcond, double air_logcond) {
   H.logcond.assign(logcond, logcond + int64_t(nx) * ny * H.nzEarth);
   H.airLog = air_logcond;
   ctx->host_stale = true;
+  ctx->sens_fields = true;   // sigma_E, sigma_N, m2 of this model are in cond_*
+  return MT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sensitivity path on the device: FWDsolve3D's forward and TRN
+// post-processing, the TRN right-hand side, PmultT and the gradient assembly
+// ---------------------------------------------------------------------------
+
+// The model fields (sigma_E, sigma_N, node weights; from the last
+// mt_update_cond, else computed here for the assembled model) and the
+// sigma-free data of the sensitivity kernels: ghost volumes, the all-edge ->
+// interior-row map and CCib transposed (rows in EDGEb order).
+struct SensData {
+  CondArgs a;
+  double *gE, *gN, *tE, *tN, *bt_val;
+  unsigned char* cntE;
+  int *rowOf, *bmap, *bt_row, *bt_col;
+  int64_t nN, nBdry;
+};
+
+static int ensure_sens(mt_ctx* ctx, SensData& S) {
+  if (!ctx) return MT_E_ARG;
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  HostOperator& H = ctx->hop;
+  if (!H.have_model || H.coefCC.empty())
+    return mt_fail(ctx, MT_E_STATE, "the sensitivity entry points need mt_assemble");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const Grid& g = H.g;
+  const int64_t nN = int64_t(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  const int64_t nB = int64_t(H.bmap.size());
+  S.nN = nN;
+  S.nBdry = nB;
+  if ((rc = cond_args(ctx, ctx->sens_fields ? nullptr : H.logcond.data(), H.airLog, S.a))) return rc;
+  if (!ctx->sens_fields) {
+    double* vtmp;
+    if ((rc = ws(ctx, "sens_vms", g.nE, &vtmp))) return rc;
+    CondArgs b = S.a;
+    b.vms1 = vtmp;
+    k_cond_edges<<<2048, 256, 0, ctx->stream>>>(b);
+    k_cond_nodes<<<1024, 256, 0, ctx->stream>>>(b);
+    MT_HIP_CHECK(ctx, hipGetLastError());
+    ctx->sens_fields = true;
+  }
+  if ((rc = ws(ctx, "sens_ge", g.nE, &S.gE)) || (rc = ws(ctx, "sens_gn", nN, &S.gN)) ||
+      (rc = ws(ctx, "sens_cnt", g.nE, &S.cntE)) || (rc = ws(ctx, "sens_rowof", g.nE, &S.rowOf)) ||
+      (rc = ws(ctx, "sens_bmap", std::max<int64_t>(1, nB), &S.bmap)) ||
+      (rc = ws(ctx, "sens_btrow", nB + 1, &S.bt_row)) || (rc = ws(ctx, "sens_te", g.nE, &S.tE)) ||
+      (rc = ws(ctx, "sens_tn", nN, &S.tN)))
+    return rc;
+  if (!ctx->sens_geom) {
+    k_sens_ghost<<<2048, 256, 0, ctx->stream>>>(S.a, S.gE, S.cntE, S.gN);
+    MT_HIP_CHECK(ctx, hipGetLastError());
+    std::vector<int> rowOf(g.nE, 0);
+    for (int64_t r = 0; r < g.nInt; ++r) rowOf[H.imap[r]] = int(r);
+    // CCib^T: for each boundary edge, its (interior row, value) pairs in
+    // increasing row order
+    std::vector<int> bpos(g.nE, -1);
+    for (int64_t b = 0; b < nB; ++b) bpos[H.bmap[b]] = int(b);
+    std::vector<int> cnt(nB + 1, 0);
+    for (size_t q = 0; q < H.ccib_col.size(); ++q) cnt[bpos[H.ccib_col[q]] + 1]++;
+    for (int64_t b = 0; b < nB; ++b) cnt[b + 1] += cnt[b];
+    std::vector<int> bcol(H.ccib_col.size()), fill(cnt.begin(), cnt.end() - 1);
+    std::vector<double> bval(H.ccib_col.size());
+    for (int64_t r = 0; r < g.nInt; ++r)
+      for (int q = H.ccib_row[r]; q < H.ccib_row[r + 1]; ++q) {
+        const int b = bpos[H.ccib_col[q]];
+        bcol[fill[b]] = int(r);
+        bval[fill[b]] = H.ccib_val[q];
+        ++fill[b];
+      }
+    const size_t nt = std::max<size_t>(1, bcol.size());
+    if ((rc = ws(ctx, "sens_btcol", nt, &S.bt_col)) || (rc = ws(ctx, "sens_btval", nt, &S.bt_val)))
+      return rc;
+    MT_HIP_CHECK(ctx, hipMemcpy(S.rowOf, rowOf.data(), g.nE * sizeof(int), hipMemcpyHostToDevice));
+    if (nB) MT_HIP_CHECK(ctx, hipMemcpy(S.bmap, H.bmap.data(), nB * sizeof(int), hipMemcpyHostToDevice));
+    MT_HIP_CHECK(ctx, hipMemcpy(S.bt_row, cnt.data(), (nB + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if (!bcol.empty()) {
+      MT_HIP_CHECK(ctx, hipMemcpy(S.bt_col, bcol.data(), bcol.size() * sizeof(int), hipMemcpyHostToDevice));
+      MT_HIP_CHECK(ctx, hipMemcpy(S.bt_val, bval.data(), bval.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    ctx->sens_geom = true;
+  } else {
+    const size_t nt = std::max<size_t>(1, H.ccib_col.size());
+    if ((rc = ws(ctx, "sens_btcol", nt, &S.bt_col)) || (rc = ws(ctx, "sens_btval", nt, &S.bt_val)))
+      return rc;
+  }
+  return MT_OK;
+}
+
+static unsigned grid_for(int64_t n) { return unsigned(std::min<int64_t>(8192, (n + 255) / 256)); }
+
+int mt_post_fwd_dev(mt_ctx* ctx, int nsys, const void* x_dev, void* e_dev) {
+  if (!ctx || !x_dev || !e_dev || nsys < 1) return mt_fail(ctx, MT_E_ARG, "mt_post_fwd_dev: bad arguments");
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  const Grid& g = ctx->hop.g;
+  k_post_fwd<<<grid_for(g.nInt * nsys), 256, 0, ctx->stream>>>(nsys, g.nInt, g.nE, ctx->dop.imap,
+                                                              (const Z*)x_dev, (Z*)e_dev);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_rhs_trn_dev(mt_ctx* ctx, int nsys, const double* omegas, const void* s_dev, void* b_dev,
+                   int gd_correct) {
+  if (!ctx || !s_dev || !b_dev || nsys < 1 || (gd_correct && !omegas))
+    return mt_fail(ctx, MT_E_ARG, "mt_rhs_trn_dev: bad arguments");
+  for (int s = 0; gd_correct && s < nsys; ++s)
+    if (!(omegas[s] > 0.0)) return mt_fail(ctx, MT_E_ARG, "mt_rhs_trn_dev: omega must be > 0");
+  SensData S;
+  int rc = ensure_sens(ctx, S);
+  if (rc) return rc;
+  Z* div = nullptr;
+  if (gd_correct) {
+    // iOmegaMuInv = 1 / (i ISIGN omega mu0) per system, divided as the host does
+    std::vector<Z> inv(nsys);
+    for (int s = 0; s < nsys; ++s) {
+      const mt_zdouble v = 1.0 / mt_zdouble(0.0, kIsign * omegas[s] * kMu0);
+      inv[s] = {v.real(), v.imag()};
+    }
+    Z* dinv;
+    if ((rc = ws(ctx, "sens_inv", nsys, &dinv)) || (rc = ws(ctx, "sens_div", size_t(nsys) * S.nN, &div)))
+      return rc;
+    MT_HIP_CHECK(ctx, hipMemcpy(dinv, inv.data(), nsys * sizeof(Z), hipMemcpyHostToDevice));
+    k_trn_div<<<grid_for(S.nN * nsys), 256, 0, ctx->stream>>>(S.a, nsys, dinv, (const Z*)s_dev, div);
+  }
+  k_trn_rhs<<<grid_for(S.a.nInt * nsys), 256, 0, ctx->stream>>>(S.a, nsys, gd_correct ? 1 : 0,
+                                                                 (const Z*)s_dev, div, (Z*)b_dev);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_post_trn_dev(mt_ctx* ctx, int nsys, const void* ei_dev, void* e_dev) {
+  if (!ctx || !ei_dev || !e_dev || nsys < 1) return mt_fail(ctx, MT_E_ARG, "mt_post_trn_dev: bad arguments");
+  SensData S;
+  int rc = ensure_sens(ctx, S);
+  if (rc) return rc;
+  k_post_trn<<<grid_for((S.a.nInt + S.nBdry) * nsys), 256, 0, ctx->stream>>>(
+      S.a, nsys, S.bmap, S.nBdry, S.bt_row, S.bt_col, S.bt_val, (const Z*)ei_dev, (Z*)e_dev);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_pmultT_dev(mt_ctx* ctx, int nfreq, const double* omegas, int npol, const void* e0_dev,
+                  const void* lam_dev, void* dm_dev, int accumulate) {
+  if (!ctx || !omegas || !e0_dev || !lam_dev || !dm_dev || nfreq < 1 || npol < 1)
+    return mt_fail(ctx, MT_E_ARG, "mt_pmultT_dev: bad arguments");
+  SensData S;
+  int rc = ensure_sens(ctx, S);
+  if (rc) return rc;
+  const CondArgs& a = S.a;
+  const int64_t ncell = int64_t(a.nx) * a.ny * (a.nz - a.nzAir);
+  const Z* e0 = (const Z*)e0_dev;
+  const Z* lam = (const Z*)lam_dev;
+  for (int f = 0; f < nfreq; ++f) {
+    const int64_t off = int64_t(f) * npol * a.nE;
+    k_pmult_cells<<<grid_for(ncell), 256, 0, ctx->stream>>>(a, npol, omegas[f], e0 + off, lam + off,
+                                                           S.gE, S.cntE, (double*)dm_dev,
+                                                           (accumulate || f > 0) ? 1 : 0);
+  }
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+int mt_gradient_dev(mt_ctx* ctx, int nfreq, const double* omegas, int npol, const void* e_dev,
+                    const void* lam_dev, void* dm_dev, int mode, int accumulate) {
+  if (!ctx || !omegas || !e_dev || !lam_dev || !dm_dev || nfreq < 1 || npol < 1 || mode < 0 ||
+      mode > 1)
+    return mt_fail(ctx, MT_E_ARG, "mt_gradient_dev: bad arguments");
+  SensData S;
+  int rc = ensure_sens(ctx, S);
+  if (rc) return rc;
+  const CondArgs& a = S.a;
+  const int64_t ncell = int64_t(a.nx) * a.ny * (a.nz - a.nzAir);
+  Z *D, *Sn;
+  if ((rc = ws(ctx, "sens_dnode", S.nN, &D)) || (rc = ws(ctx, "sens_snode", S.nN, &Sn))) return rc;
+  if (!accumulate) MT_HIP_CHECK(ctx, hipMemsetAsync(dm_dev, 0, ncell * sizeof(double), ctx->stream));
+  const Z* e = (const Z*)e_dev;
+  const Z* lam = (const Z*)lam_dev;
+  for (int f = 0; f < nfreq; ++f) {
+    MT_HIP_CHECK(ctx, hipMemsetAsync(S.tE, 0, a.nE * sizeof(double), ctx->stream));
+    MT_HIP_CHECK(ctx, hipMemsetAsync(S.tN, 0, S.nN * sizeof(double), ctx->stream));
+    for (int p = 0; p < npol; ++p) {
+      const Z* es = e + (int64_t(f) * npol + p) * a.nE;
+      const Z* ls = lam + (int64_t(f) * npol + p) * a.nInt;
+      if (mode) k_grad_nodes<<<grid_for(S.nN), 256, 0, ctx->stream>>>(a, S.rowOf, ls, es, D, Sn);
+      k_grad_accum<<<grid_for(a.nInt + S.nN), 256, 0, ctx->stream>>>(a, omegas[f], mode, ls, es, D, Sn,
+                                                                     S.tE, S.tN);
+    }
+    k_grad_cells<<<grid_for(ncell), 256, 0, ctx->stream>>>(a, mode, S.tE, S.tN, S.gE, S.gN,
+                                                          (double*)dm_dev);
+  }
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MT_OK;
 }

@@ -85,6 +85,11 @@ _SIGS = {
     "mt_bc_x0_dev": (_I, [_P, _I, _PD, _PI, _P]),
     "mt_rhs_trn": (_I, [_P, _D, _PZ, _PZ]),
     "mt_post_trn": (_I, [_P, _PZ, _PZ]),
+    "mt_post_fwd_dev": (_I, [_P, _I, _P, _P]),
+    "mt_rhs_trn_dev": (_I, [_P, _I, _PD, _P, _P, _I]),
+    "mt_post_trn_dev": (_I, [_P, _I, _P, _P]),
+    "mt_pmultT_dev": (_I, [_P, _I, _PD, _I, _P, _P, _P, _I]),
+    "mt_gradient_dev": (_I, [_P, _I, _PD, _I, _P, _P, _P, _I, _I]),
     "mt_divcor": (_I, [_P, _D, _PZ, _P, _I, _D, C.POINTER(_I), C.POINTER(_D)]),
     "mt_pmultT": (_I, [_P, _D, _I, _PZ, _PZ, _PD, _I]),
     "mt_gradient": (_I, [_P, _D, _I, _PZ, _PZ, _PD, _I]),
@@ -387,6 +392,31 @@ class Context:
             np.asarray(dm, np.float64).ravel(order="F")).copy()
         self._chk(lib().mt_gradient(self._h, float(omega), e.shape[0], e, la, out, int(mode)))
         return out.reshape(shape, order="F")
+
+    # -- sensitivity path on device buffers ------------------------------------
+    def post_fwd_dev(self, nsys, x_ptr, e_ptr):
+        """e(EDGEi) = x for nsys systems (x [nsys][n_int], e [nsys][n_edge])."""
+        self._chk(lib().mt_post_fwd_dev(self._h, int(nsys), C.c_void_p(x_ptr), C.c_void_p(e_ptr)))
+
+    def rhs_trn_dev(self, omegas, s_ptr, b_ptr, gd_correct=True):
+        om = np.ascontiguousarray(np.atleast_1d(omegas), np.float64)
+        self._chk(lib().mt_rhs_trn_dev(self._h, len(om), om, C.c_void_p(s_ptr), C.c_void_p(b_ptr),
+                                       int(bool(gd_correct))))
+
+    def post_trn_dev(self, nsys, ei_ptr, e_ptr):
+        self._chk(lib().mt_post_trn_dev(self._h, int(nsys), C.c_void_p(ei_ptr), C.c_void_p(e_ptr)))
+
+    def pmultT_dev(self, omegas, npol, e0_ptr, lam_ptr, dm_ptr, accumulate=False):
+        om = np.ascontiguousarray(np.atleast_1d(omegas), np.float64)
+        self._chk(lib().mt_pmultT_dev(self._h, len(om), om, int(npol), C.c_void_p(e0_ptr),
+                                      C.c_void_p(lam_ptr), C.c_void_p(dm_ptr), int(bool(accumulate))))
+
+    def gradient_dev(self, omegas, npol, e_ptr, lam_ptr, dm_ptr, mode=None, accumulate=False):
+        om = np.ascontiguousarray(np.atleast_1d(omegas), np.float64)
+        mode = GRAD_EXACT if mode is None else mode
+        self._chk(lib().mt_gradient_dev(self._h, len(om), om, int(npol), C.c_void_p(e_ptr),
+                                        C.c_void_p(lam_ptr), C.c_void_p(dm_ptr), int(mode),
+                                        int(bool(accumulate))))
 
     def divcor(self, omega, ei, maxit=100, tol=1e-7):
         e = np.array(ei, np.complex128)
