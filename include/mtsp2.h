@@ -153,6 +153,20 @@ int mt_operator_get_csr(mt_ctx *ctx, int *rowptr, int *col, double *val, double 
 int mt_bicg(mt_ctx *ctx, double omega, const mt_zdouble *b, mt_zdouble *x, int maxit,
             double tol, int precond, int *niter, double *rerr, int *failed);
 
+/* QMR (solver.f90:140-370), the reference's default solver_name
+ * (EMsolve3D.f90:70), which its CUDA build runs on the host
+ * (EMsolve3D.f90:375-377): QMR without look-ahead, M1 = L and M2 = U of the
+ * block DILU in the reference ordering (precond MT_PC_DILU; MT_PC_NONE for
+ * none), A^H and the L^-H / U^-H solves for the dual recurrence.  Same
+ * arguments as mt_bicg; as the reference, x returns the last iterate, rerr
+ * (length maxit) the relative residual per iteration stored as a default
+ * real, rerr[niter-1] the smallest one seen, and *niter < maxit means
+ * converged.  Status MT_OK, MT_E_MAXIT, MT_E_BREAKDOWN (*failed = 1: rho,
+ * psi, delta, epsilon or beta = 0, where the reference stops or exits) or
+ * MT_E_NAN_RHS. */
+int mt_qmr(mt_ctx *ctx, double omega, const mt_zdouble *b, mt_zdouble *x, int maxit, double tol,
+           int precond, int *niter, double *rerr, int *failed);
+
 /* ---- batched solve -------------------------------------------------------- */
 /* nsys systems A(omega_s) x_s = b_s sharing AAii, solved together: both
  * polarizations and many frequencies in one launch sequence.  b and x are

@@ -168,13 +168,17 @@ __global__ __launch_bounds__(BS) void k_finalize(SysScal* scs, const int* act, i
   } else if (KIND == FIN_ALPHA) {
     if (sc.status != ST_RUN) return;
     sc.alpha = zdiv(sc.rho, a0);
-    if ((sc.alpha.x == 0.0 && sc.alpha.y == 0.0) || isnan(sc.alpha.x) || isnan(sc.alpha.y))
+    // <rt, v> = 0 (the CPU BICG's check, solver.f90:720-926) or an alpha
+    // that is zero or not finite: breakdown
+    if ((a0.x == 0.0 && a0.y == 0.0) || (sc.alpha.x == 0.0 && sc.alpha.y == 0.0) ||
+        !isfinite(sc.alpha.x) || !isfinite(sc.alpha.y))
       sc.status = ST_BREAK;
   } else if (KIND == FIN_OMEGA) {
     if (sc.status != ST_RUN) return;
     sc.omg = zdiv(a1, a0);
     sc.best_due = 0;  // consumed by k_supdate of this iteration
-    if ((sc.omg.x == 0.0 && sc.omg.y == 0.0) || isnan(sc.omg.x) || isnan(sc.omg.y))
+    if ((a0.x == 0.0 && a0.y == 0.0) || (sc.omg.x == 0.0 && sc.omg.y == 0.0) ||
+        !isfinite(sc.omg.x) || !isfinite(sc.omg.y))
       sc.status = ST_BREAK;
   } else if (KIND == FIN_RUPDATE) {
     if (sc.status != ST_RUN) return;

@@ -21,6 +21,7 @@
 #include "mt_host_ops.hpp"
 #include "mt_internal.h"
 #include "mt_kernels.hpp"
+#include "mt_qmr.hpp"
 #include "mt_sens.hpp"
 #include "mt_sites.hpp"
 #include "mt_spmv.hpp"
@@ -147,6 +148,9 @@ static int ws_get(mt_ctx* ctx, const std::string& name, size_t bytes, void** out
                                       std::to_string(bytes) + " bytes)");
   }
   MT_HIP_CHECK(ctx, hipMemsetAsync(p, 0, bytes ? bytes : 16, ctx->stream));
+  // the stream is non-blocking: callers upload into a fresh workspace with
+  // synchronous copies on the null stream, which must not race the clear
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   ctx->ws[name] = {p, bytes};
   *out = p;
   return MT_OK;
@@ -1623,7 +1627,8 @@ int mt_bc_x0_dev(mt_ctx* ctx, int nsys, const double* omegas, const int* modes, 
 static int stage_up(mt_ctx* ctx, const char* name, const void* h, size_t bytes, void** d) {
   int rc = ws_get(ctx, name, bytes, d);
   if (rc) return rc;
-  MT_HIP_CHECK(ctx, hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+  MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MT_OK;
 }
 
@@ -2449,5 +2454,205 @@ int mt_gradient_dev(mt_ctx* ctx, int nfreq, const double* omegas, int npol, cons
   }
   MT_HIP_CHECK(ctx, hipGetLastError());
   MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// QMR (the reference's default solver, solver.f90:140-370) on the device
+// ---------------------------------------------------------------------------
+// One FP64 system on padded all-edge vectors.  M1 = L and M2 = U of the block
+// DILU in the reference ordering (the level-scheduled tile sweeps); the
+// adjoint solves use the same sweeps on the transposed stencil with the
+// conjugated pivots: with q = conj(p) (the pivots of A^H's blocks) and c the
+// couplings of A^T,
+//   U^-H x = Q^-1 Lsweep_{c,q}(x),    L^-H w = Usweep_{c,q}(Q w),
+// where Lsweep / Usweep are the forward / backward DILU sweeps.  Scalars are
+// formed on the host from per-block partial sums in a fixed order.
+namespace {
+struct QmrEngine {
+  mt_ctx* ctx;
+  int64_t n;          // all-edge length the vector kernels cover
+  int G = 512;        // blocks of the inner-product kernel
+  bool pc;            // block DILU (else M1 = M2 = I)
+  double omega;
+  Z *dinv = nullptr, *dinvH = nullptr, *part = nullptr;
+  SysScal* sc = nullptr;
+  int* act = nullptr;
+  std::vector<Z> hpart;
+  unsigned grid() const { return unsigned(std::min<int64_t>(4096, (n + 255) / 256)); }
+
+  int dots(int nd, const Z* a0, const Z* b0, const Z* a1, const Z* b1, const Z* a2, const Z* b2,
+           mt_zdouble* out) {
+    k_q_dots<<<G, QMR_BS, 0, ctx->stream>>>(n, nd, a0, b0, a1, b1, a2, b2, part);
+    MT_HIP_CHECK(ctx, hipGetLastError());
+    hpart.resize(size_t(3) * G);
+    MT_HIP_CHECK(ctx, hipMemcpyAsync(hpart.data(), part, sizeof(Z) * nd * G, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < nd; ++k) {
+      mt_zdouble s = 0.0;
+      for (int g = 0; g < G; ++g) s += mt_zdouble(hpart[size_t(k) * G + g].x, hpart[size_t(k) * G + g].y);
+      out[k] = s;
+    }
+    return MT_OK;
+  }
+  DiluArgs<double, double> dargs(bool transposed, const Z* in, Z* out, const Z* dv) {
+    const Grid& g = ctx->hop.g;
+    return dilu_args<double, double>(ctx->tiles, view<double>(ctx, transposed).coef, g.nE, g.nE, act, 1,
+                                     sc, in, out, dv, ST_RUN);
+  }
+  void copy(const Z* in, Z* out) { k_q_diag<<<grid(), 256, 0, ctx->stream>>>(n, 0, in, nullptr, out); }
+  // y = L^-1 x
+  void m1(const Z* x, Z* y) {
+    if (!pc) { copy(x, y); return; }
+    dilu_fwd<double, double>(ctx, ctx->tiles, dargs(false, x, y, dinv));
+  }
+  // y = U^-1 x
+  void m2(const Z* x, Z* y) {
+    copy(x, y);
+    if (pc) dilu_bwd<double, double>(ctx, ctx->tiles, dargs(false, y, y, dinv));
+  }
+  // y = L^-H x
+  void m1h(const Z* x, Z* y) {
+    if (!pc) { copy(x, y); return; }
+    k_q_diag<<<grid(), 256, 0, ctx->stream>>>(n, 1, x, dinvH, y);
+    dilu_bwd<double, double>(ctx, ctx->tiles, dargs(true, y, y, dinvH));
+  }
+  // y = U^-H x (t: scratch)
+  void m2h(const Z* x, Z* y, Z* t) {
+    if (!pc) { copy(x, y); return; }
+    dilu_fwd<double, double>(ctx, ctx->tiles, dargs(true, x, t, dinvH));
+    k_q_diag<<<grid(), 256, 0, ctx->stream>>>(n, 2, t, dinvH, y);
+  }
+  void apply(bool adjoint, const Z* x, Z* y) {
+    // A x, or A^H x = AAii^T x - i ISIGN omega vms1 x
+    k_apply<double, double><<<1024, BS, 0, ctx->stream>>>(view<double>(ctx, adjoint), x, y,
+                                                          adjoint ? -omega : omega);
+  }
+};
+}  // namespace
+
+int mt_qmr(mt_ctx* ctx, double omega, const mt_zdouble* b, mt_zdouble* x, int maxit, double tol,
+           int precond, int* niter, double* rerr, int* failed) {
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  if (!b || !x || maxit < 1 || (precond != MT_PC_DILU && precond != MT_PC_NONE))
+    return mt_fail(ctx, MT_E_ARG, "mt_qmr: bad arguments (precond is MT_PC_DILU or MT_PC_NONE)");
+  MT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+  if ((rc = ensure_transpose(ctx))) return rc;
+  const Grid& g = ctx->hop.g;
+  const size_t nv = size_t(g.vlen(1));
+  QmrEngine E;
+  E.ctx = ctx;
+  E.n = g.nE;
+  E.pc = precond == MT_PC_DILU;
+  E.omega = omega;
+  static const char* names[] = {"q_x", "q_b", "q_r", "q_vt", "q_y", "q_z", "q_wt", "q_v", "q_w",
+                                "q_yt", "q_zt", "q_p", "q_q", "q_pt", "q_d", "q_s", "q_t"};
+  Z* v[17];
+  for (int k = 0; k < 17; ++k)
+    if ((rc = ws(ctx, names[k], nv, &v[k]))) return rc;
+  Z *X = v[0], *B = v[1], *R = v[2], *VT = v[3], *Y = v[4], *Zv = v[5], *WT = v[6], *V = v[7], *W = v[8];
+  Z *YT = v[9], *ZT = v[10], *P = v[11], *Q = v[12], *PT = v[13], *D = v[14], *S = v[15], *T = v[16];
+  if ((rc = ws(ctx, "q_part", size_t(3) * E.G, &E.part)) || (rc = ws(ctx, "q_sc", 1, &E.sc)) ||
+      (rc = ws(ctx, "q_act", 1, &E.act)))
+    return rc;
+  SysScal hs;
+  std::memset(&hs, 0, sizeof(hs));
+  hs.status = ST_RUN;
+  hs.fidx = 0;
+  hs.freq_omega = omega;
+  const int zero = 0;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(E.sc, &hs, sizeof(hs), hipMemcpyHostToDevice, ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(E.act, &zero, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  if (E.pc) {
+    C2<double>* dv = nullptr;
+    if ((rc = build_dinv<double>(ctx, PCK_DILU, false, std::vector<double>{omega}, &dv))) return rc;
+    E.dinv = reinterpret_cast<Z*>(dv);
+    if ((rc = ws(ctx, "q_dinvh", nv, &E.dinvH))) return rc;
+    k_q_conj<<<E.grid(), 256, 0, ctx->stream>>>(g.nE, E.dinv, E.dinvH);
+  }
+  if ((rc = upload_padded(ctx, 1, b, B, "io_tmp")) || (rc = upload_padded(ctx, 1, x, X, "io_tmp")))
+    return rc;
+  const unsigned gr = E.grid();
+  for (int i = 0; i < maxit; ++i) rerr[i] = 0.0;
+  // R = b - A x
+  E.apply(false, X, R);
+  k_q_axpy<<<gr, 256, 0, ctx->stream>>>(g.nE, Z{1.0, 0.0}, B, R, R);
+  mt_zdouble dd[3];
+  if ((rc = E.dots(2, B, B, R, R, nullptr, nullptr, dd))) return rc;
+  const double bnorm = std::sqrt(dd[0]).real();
+  double rnorm = std::sqrt(dd[1]).real();
+  double rnormin = rnorm;
+  if (std::isnan(std::abs(bnorm))) {
+    *niter = 0;
+    *failed = 1;
+    return mt_fail(ctx, MT_E_NAN_RHS, "mt_qmr: the right-hand side contains NaN");
+  }
+  int iter = 1;
+  rerr[0] = double(float(rnorm / bnorm));
+  E.copy(R, VT);
+  E.m1(VT, Y);
+  E.copy(R, WT);
+  E.m2h(WT, Zv, T);
+  if ((rc = E.dots(2, Y, Y, Zv, Zv, nullptr, nullptr, dd))) return rc;
+  mt_zdouble RHO = std::sqrt(dd[0]), PSI = std::sqrt(dd[1]);
+  mt_zdouble GAMM = 1.0, ETA = -1.0, EPSIL = 0.0, THET = 0.0, THET1 = 0.0;
+  int fl = 0;
+  auto zc = [](mt_zdouble a) { return Z{a.real(), a.imag()}; };
+  while (rerr[iter - 1] > tol && iter < maxit) {
+    if (RHO == 0.0 || PSI == 0.0) { fl = 1; break; }
+    const mt_zdouble rhoInv = (1.0 / RHO) * mt_zdouble(1.0, 0.0);
+    const mt_zdouble psiInv = (1.0 / PSI) * mt_zdouble(1.0, 0.0);
+    k_q_scale<<<gr, 256, 0, ctx->stream>>>(g.nE, zc(rhoInv), zc(psiInv), VT, WT, V, W, Y, Zv);
+    if ((rc = E.dots(1, Zv, Y, nullptr, nullptr, nullptr, nullptr, dd))) return rc;
+    const mt_zdouble DELTA = dd[0];
+    if (DELTA == 0.0) { fl = 1; break; }
+    E.m2(Y, YT);
+    E.m1h(Zv, ZT);
+    if (iter == 1) {
+      E.copy(YT, P);
+      E.copy(ZT, Q);
+    } else {
+      const mt_zdouble PDE = PSI * DELTA / EPSIL;
+      const mt_zdouble RDE = RHO * std::conj(DELTA / EPSIL);
+      k_q_axpy<<<gr, 256, 0, ctx->stream>>>(g.nE, zc(PDE), YT, P, P);
+      k_q_axpy<<<gr, 256, 0, ctx->stream>>>(g.nE, zc(RDE), ZT, Q, Q);
+    }
+    E.apply(false, P, PT);
+    if ((rc = E.dots(1, Q, PT, nullptr, nullptr, nullptr, nullptr, dd))) return rc;
+    EPSIL = dd[0];
+    if (EPSIL == 0.0) { fl = 1; break; }
+    const mt_zdouble BETA = EPSIL / DELTA;
+    if (BETA == 0.0) { fl = 1; break; }
+    k_q_axpy<<<gr, 256, 0, ctx->stream>>>(g.nE, zc(BETA), PT, V, VT);
+    const mt_zdouble RHO1 = RHO;
+    E.m1(VT, Y);
+    E.apply(true, Q, WT);
+    k_q_axpy<<<gr, 256, 0, ctx->stream>>>(g.nE, zc(std::conj(BETA)), WT, W, WT);
+    E.m2h(WT, Zv, T);
+    if ((rc = E.dots(2, Y, Y, Zv, Zv, nullptr, nullptr, dd))) return rc;
+    RHO = std::sqrt(dd[0]);
+    PSI = std::sqrt(dd[1]);
+    if (iter > 1) THET1 = THET;
+    THET = RHO / (GAMM * std::abs(BETA));
+    const mt_zdouble GAMM1 = GAMM;
+    GAMM = 1.0 / std::sqrt(1.0 + THET * THET);
+    if (GAMM == 0.0) break;
+    ETA = -(ETA * RHO1 * GAMM * GAMM / (BETA * GAMM1 * GAMM1));
+    const mt_zdouble TM2 = THET1 * THET1 * GAMM * GAMM;
+    k_q_update<<<gr, 256, 0, ctx->stream>>>(g.nE, iter == 1 ? 1 : 0, zc(ETA), zc(TM2), P, PT, D, S, X, R);
+    if ((rc = E.dots(1, R, R, nullptr, nullptr, nullptr, nullptr, dd))) return rc;
+    rnorm = std::sqrt(dd[0]).real();
+    if (rnorm < rnormin) rnormin = rnorm;
+    iter = iter + 1;
+    rerr[iter - 1] = double(float(rnorm / bnorm));
+  }
+  *niter = iter;
+  rerr[iter - 1] = double(float(rnormin / bnorm));
+  *failed = fl;
+  if ((rc = download_padded(ctx, 1, X, x, "io_tmp"))) return rc;
+  if (fl) return mt_fail(ctx, MT_E_BREAKDOWN, "mt_qmr: breakdown (rho, psi, delta, epsilon or beta = 0)");
+  if (iter >= maxit) return mt_fail(ctx, MT_E_MAXIT, "mt_qmr: not converged within maxit");
   return MT_OK;
 }

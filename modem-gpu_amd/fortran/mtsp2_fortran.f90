@@ -36,11 +36,11 @@ module mtsp2_fortran
 
    public :: mt_default_opts, mt_ctx_create, mt_ctx_destroy, mt_device_count, mt_set_grid, &
              mt_operator_set_csr, mt_bicg, mt_solve_batch, mt_apply, mt_precond_apply, &
-             mt_divcor, mt_pmultT, mt_last_error_string, mt_receivers_set, mt_data_resp, &
+             mt_divcor, mt_pmultT, mt_last_error_string, mt_receivers_set, mt_data_resp, mt_qmr, &
              mt_lmultT, mt_dev_alloc, mt_dev_free, mt_dev_copy, mt_assemble, mt_update_cond, &
              mt_bc_x0_dev, mt_rhs_fwd_dev
-   public :: mtsp2_bicg, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, mtsp2_data_resp, &
-             mtsp2_bc_x0, mtsp2_operator_reset
+   public :: mtsp2_bicg, mtsp2_qmr, mtsp2_solve_batch, mtsp2_release, mtsp2_lmultT, &
+             mtsp2_data_resp, mtsp2_bc_x0, mtsp2_operator_reset
 
    interface
       subroutine mt_default_opts(opts) bind(C, name='mt_default_opts')
@@ -111,6 +111,17 @@ module mtsp2_fortran
       end function
       integer(c_int) function mt_bicg(ctx, omega, b, x, maxit, tol, precond, niter, rerr, failed) &
             bind(C, name='mt_bicg')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         real(c_double), value :: omega, tol
+         complex(c_double_complex), intent(in) :: b(*)
+         complex(c_double_complex), intent(inout) :: x(*)
+         integer(c_int), value :: maxit, precond
+         integer(c_int), intent(out) :: niter, failed
+         real(c_double), intent(out) :: rerr(*)
+      end function
+      integer(c_int) function mt_qmr(ctx, omega, b, x, maxit, tol, precond, niter, rerr, failed) &
+            bind(C, name='mt_qmr')
          import :: c_int, c_ptr, c_double, c_double_complex
          type(c_ptr), value :: ctx
          real(c_double), value :: omega, tol
@@ -350,6 +361,34 @@ contains
       pc = MT_PC_DILU
       if (present(precond)) pc = int(precond, c_int)
       rc = mt_bicg(ctx, omega, b, x, int(maxIt, c_int), tol, pc, nit, rerr, fl)
+      ierr = rc
+      niter = nit
+      failed = (fl /= 0)
+   end subroutine
+
+   ! Drop-in for QMR (solver.f90:140-370), the reference's default solver,
+   ! with the arguments of mtsp2_bicg: x holds x0 on entry and the last
+   ! iterate on exit, rerr(niter) the smallest relative residual, niter <
+   ! maxIt means converged, failed the reference's breakdown flag.  ierr is
+   ! MT_OK, MT_E_MAXIT or MT_E_BREAKDOWN (with failed set) or a device error.
+   subroutine mtsp2_qmr(device_idx, nx, ny, nz, row, col, val, vms1, omega, b, x, &
+                        maxIt, tol, niter, rerr, failed, ierr, precond)
+      integer, intent(in)                     :: device_idx, nx, ny, nz, maxIt
+      integer(c_int), intent(in)              :: row(:), col(:)
+      real(c_double), intent(in)              :: val(:), vms1(:), omega, tol
+      complex(c_double_complex), intent(in)   :: b(:)
+      complex(c_double_complex), intent(inout):: x(:)
+      integer, intent(out)                    :: niter, ierr
+      real(c_double), intent(out)             :: rerr(:)
+      logical, intent(out)                    :: failed
+      integer, intent(in), optional           :: precond
+      type(c_ptr) :: ctx
+      integer(c_int) :: rc, nit, fl, pc
+      call get_ctx(device_idx, nx, ny, nz, row, col, val, vms1, ctx, ierr)
+      if (ierr /= MT_OK) return
+      pc = MT_PC_DILU
+      if (present(precond)) pc = int(precond, c_int)
+      rc = mt_qmr(ctx, omega, b, x, int(maxIt, c_int), tol, pc, nit, rerr, fl)
       ierr = rc
       niter = nit
       failed = (fl /= 0)

@@ -73,6 +73,7 @@ _SIGS = {
     "mt_operator_size": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
     "mt_operator_get_csr": (_I, [_P, _PI, _PI, _PD, _PD]),
     "mt_bicg": (_I, [_P, _D, _PZ, _PZ, _I, _D, _I, C.POINTER(_I), _PD, C.POINTER(_I)]),
+    "mt_qmr": (_I, [_P, _D, _PZ, _PZ, _I, _D, _I, C.POINTER(_I), _PD, C.POINTER(_I)]),
     "mt_solve_batch": (_I, [_P, _I, _PD, _PZ, _PZ, C.POINTER(SolveOpts), C.POINTER(SysStats)]),
     "mt_solve_batch_dev": (_I, [_P, _I, _PD, _P, _P, C.POINTER(SolveOpts), C.POINTER(SysStats)]),
     "mt_dev_alloc": (_I, [_P, C.c_size_t, C.POINTER(_P)]),
@@ -278,6 +279,19 @@ class Context:
                                 int(maxit), float(tol), int(precond), C.byref(nit), rerr,
                                 C.byref(fl)))
         return x, nit.value, rerr, bool(fl.value)
+
+    def qmr(self, omega, b, x0=None, maxit=1200, tol=1e-7, precond=PC_DILU):
+        """QMR (the reference's solver.f90:140-370 semantics): returns
+        (x, niter, rerr, failed, status); status is MT_OK, MT_E_MAXIT or
+        MT_E_BREAKDOWN (other codes raise)."""
+        x = np.zeros(self.n_int, np.complex128) if x0 is None else np.array(x0, np.complex128)
+        rerr = np.zeros(maxit)
+        nit, fl = _I(), _I()
+        rc = lib().mt_qmr(self._h, float(omega), np.ascontiguousarray(b, np.complex128), x,
+                          int(maxit), float(tol), int(precond), C.byref(nit), rerr, C.byref(fl))
+        if rc not in (MT_OK, MT_E_MAXIT, MT_E_BREAKDOWN):
+            self._chk(rc)
+        return x, nit.value, rerr, bool(fl.value), rc
 
     def solve_batch(self, omegas, b, x0=None, **opts):
         omegas = np.ascontiguousarray(omegas, np.float64)

@@ -14,6 +14,7 @@
 !                          :155-164 UpdateFreq (-> updateOmegaMuSig, PC_setup :512-611)
 !   operator apply         modelOperator3D.f90:415-436 Mult_Aii, :622-644 PC_Lsolve/PC_Usolve
 !   CPU BiCGStab           3D_MT/FWD_SP2/solver.f90:720-926 BICG
+!   CPU QMR                3D_MT/FWD_SP2/solver.f90:140-370 QMR
 !   divergence correction  modelOperator3D.f90:648-730 DivCorInit/DivCorSetup,
 !                          solver.f90:57-136 PCG
 !   edge -> model adjoint  ModelMap.inc:281-372 dEdgeToModelParam
@@ -29,6 +30,9 @@
 !   int32 napply ; napply x { int32 iomega ; complex128 x(nInt) }
 !   int32 divcor ; if 1: { int32 maxit ; real64 tol ; complex128 rhs(nNodeInt) }
 !   int32 e2m    ; if 1: { real64 e(nEdgeAll) }
+!   int32 nqmr   ; nqmr x { int32 iomega, maxit ; real64 tol ;
+!                           complex128 b(nInt), x0(nInt) }          (QMR solves)
+!   int32 nadj   ; nadj x { int32 iomega ; complex128 x(nInt) }   (A^H x, L^-H x, U^-H x)
 program ref_driver
    use math_constants
    use griddef
@@ -47,6 +51,7 @@ program ref_driver
    character(len=512)            :: infile, outdir
    integer                       :: fin, nx, ny, nzAir, nzEarth, nz, i, k, s
    integer                       :: nomega, nsolve, napply, idiv, ie2m, iom, maxit
+   integer                       :: nqmr, nadj, ios
    real(kind=prec), allocatable  :: dx(:), dy(:), dz(:), logc(:), omegas(:)
    real(kind=prec)               :: airLog, tol
    type(grid_t), target          :: grid
@@ -229,6 +234,48 @@ program ref_driver
       call getValue_cells(dm, s)
       close(s)
    endif
+
+   ! QMR solves (CPU reference path, solver.f90:140-370) and the adjoint
+   ! applications QMR uses (Mult_Aii adjt, PC_Lsolve/PC_Usolve adjt,
+   ! modelOperator3D.f90:415-436, 622-644)
+   read(fin, iostat=ios) nqmr
+   if (ios .ne. 0) nqmr = 0
+   do k = 1, nqmr
+      read(fin) iom, maxit
+      read(fin) tol
+      read(fin) b, x
+      call UpdateFreq(omegas(iom))
+      ctl%maxIt = maxit
+      ctl%tol = tol
+      ctl%niter = 0
+      ctl%failed = .false.
+      allocate(ctl%rerr(maxit))
+      ctl%rerr = 0.0
+      call QMR(b, x, ctl)
+      write(tag, '(I0)') k
+      call wr_open(trim(outdir)//'/qmr_'//trim(tag)//'.bin', s)
+      i = 0
+      if (ctl%failed) i = 1
+      write(s) ctl%niter, i, maxit
+      write(s) ctl%rerr
+      write(s) x
+      close(s)
+      deallocate(ctl%rerr)
+   enddo
+   read(fin, iostat=ios) nadj
+   if (ios .ne. 0) nadj = 0
+   do k = 1, nadj
+      read(fin) iom
+      read(fin) x
+      call UpdateFreq(omegas(iom))
+      call Mult_Aii(x, .true., y)
+      call PC_Lsolve(x, .true., t1)
+      call PC_Usolve(x, .true., t2)
+      write(tag, '(I0)') k
+      call wr_open(trim(outdir)//'/adj_'//trim(tag)//'.bin', s)
+      write(s) y, t1, t2
+      close(s)
+   enddo
    close(fin)
 
 contains

@@ -36,11 +36,13 @@ def _rd_csr(path, cplx=False):
 
 
 def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
-        e2m=None, workdir=None, timeout=600, dump=True):
+        e2m=None, workdir=None, timeout=600, dump=True, qmr=(), adj_applies=()):
     """grid: dict(dx, dy, dz, nzAir); logcond: (nx, ny, nzEarth) natural-log
-    conductivities.  solves: list of (iomega0, maxit, tol, b, x0).  applies:
-    list of (iomega0, x).  divcor: (maxit, tol, rhs) or None.  e2m: real
-    all-edge vector or None.  Returns a dict of numpy arrays."""
+    conductivities.  solves: list of (iomega0, maxit, tol, b, x0) for BICG,
+    qmr: the same for QMR.  applies: list of (iomega0, x) (A x, L^-1 x,
+    U^-1 L^-1 x); adj_applies: the same for the adjoint forms (A^H x, L^-H x,
+    U^-H x).  divcor: (maxit, tol, rhs) or None.  e2m: real all-edge vector or
+    None.  Returns a dict of numpy arrays."""
     dx = np.asarray(grid["dx"], np.float64)
     dy = np.asarray(grid["dy"], np.float64)
     dz = np.asarray(grid["dz"], np.float64)
@@ -82,6 +84,16 @@ def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
         else:
             np.array([1], np.int32).tofile(f)
             np.asarray(e2m, np.float64).tofile(f)
+        np.array([len(qmr)], np.int32).tofile(f)
+        for (iom, maxit, tol, b, x0) in qmr:
+            np.array([iom + 1, maxit], np.int32).tofile(f)
+            np.array([tol], np.float64).tofile(f)
+            np.asarray(b, np.complex128).tofile(f)
+            np.asarray(x0, np.complex128).tofile(f)
+        np.array([len(adj_applies)], np.int32).tofile(f)
+        for (iom, x) in adj_applies:
+            np.array([iom + 1], np.int32).tofile(f)
+            np.asarray(x, np.complex128).tofile(f)
     subprocess.run([REF_BIN, inp, wd], check=True, timeout=timeout,
                    stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
     out = {}
@@ -97,6 +109,15 @@ def run(grid, logcond, air_log, omegas, solves=(), applies=(), divcor=None,
             x = np.fromfile(f, np.complex128, nint)
         out[f"bicg_{k}"] = {"niter": int(niter), "failed": bool(failed), "rerr": rerr, "x": x,
                             "seconds": secs}
+    for k in range(len(qmr)):
+        with open(os.path.join(wd, f"qmr_{k+1}.bin"), "rb") as f:
+            niter, failed, maxit = np.fromfile(f, np.int32, 3)
+            rerr = np.fromfile(f, np.float64, maxit)
+            x = np.fromfile(f, np.complex128, nint)
+        out[f"qmr_{k}"] = {"niter": int(niter), "failed": bool(failed), "rerr": rerr, "x": x}
+    for k in range(len(adj_applies)):
+        a = np.fromfile(os.path.join(wd, f"adj_{k+1}.bin"), np.complex128)
+        out[f"adj_{k}"] = {"AHx": a[:nint], "LHsolve": a[nint:2 * nint], "UHsolve": a[2 * nint:]}
     if not dump:
         _cleanup(wd, own)
         return out

@@ -1223,6 +1223,115 @@ done:
   return status;
 }
 
+/* -------------------------------------------------------------------- QMR */
+/* PC_Lsolve / PC_Usolve with adjt = .true. (modelOperator3D.f90:622-644):
+ * UTsolve_Cmplx(LH) and LTsolve_Cmplx(UH), LH = L^H and UH = U^H as PC_setup
+ * forms them with CMATtrans (conjugate). */
+static void pc_lsolve_h(const sp2o_problem *p, const zdouble *x, zdouble *y) {
+  o_csrz LH = cmat_trans(&p->L);
+  utsolve_z(&LH, x, y);
+  csrz_free(&LH);
+}
+static void pc_usolve_h(const sp2o_problem *p, const zdouble *x, zdouble *y) {
+  o_csrz UH = cmat_trans(&p->U);
+  ltsolve_z(&UH, x, y);
+  csrz_free(&UH);
+}
+void sp2o_pc_lsolve_h(const sp2o_problem *p, const zdouble *x, zdouble *y) { pc_lsolve_h(p, x, y); }
+void sp2o_pc_usolve_h(const sp2o_problem *p, const zdouble *x, zdouble *y) { pc_usolve_h(p, x, y); }
+
+/* QMR solver.f90:140-370: QMR without look-ahead (Barrett et al., Templates,
+ * p. 24) with M1 = L, M2 = U of the block DILU; returns the last iterate,
+ * rerr(niter) = the smallest relative residual seen.  Status: 0 converged
+ * (niter < maxit), 1 not converged, 2 breakdown (the reference's `failed`;
+ * rho or psi = 0, where the reference stops the program, also returns 2). */
+int sp2o_qmr(const sp2o_problem *p, const zdouble *b, zdouble *x, int maxit, double tol,
+             int *niter, double *rerr) {
+  int n = p->nInt, i, iter, failed = 0;
+  for (i = 0; i < maxit; i++) rerr[i] = 0.0;
+  zdouble *R = XALLOC(zdouble, n), *VT = XALLOC(zdouble, n), *Y = XALLOC(zdouble, n);
+  zdouble *Z = XALLOC(zdouble, n), *WT = XALLOC(zdouble, n), *V = XALLOC(zdouble, n);
+  zdouble *W = XALLOC(zdouble, n), *YT = XALLOC(zdouble, n), *ZT = XALLOC(zdouble, n);
+  zdouble *P = XALLOC(zdouble, n), *Q = XALLOC(zdouble, n), *PT = XALLOC(zdouble, n);
+  zdouble *D = XALLOC(zdouble, n), *S = XALLOC(zdouble, n);
+  sp2o_mult_aii(p, 0, x, R);
+  for (i = 0; i < n; i++) R[i] = b[i] - R[i];
+  double bnorm = creal(csqrt(zdot(n, b, b))), rnorm = creal(csqrt(zdot(n, R, R)));
+  double rnormin = rnorm;
+  if (isnan(fabs(bnorm))) { *niter = 0; failed = 3; goto done; }
+  iter = 1;
+  rerr[iter - 1] = (float)(rnorm / bnorm);
+  memcpy(VT, R, sizeof(zdouble) * n);
+  ltsolve_z(&p->L, VT, Y);
+  zdouble RHO = csqrt(zdot(n, Y, Y));
+  memcpy(WT, R, sizeof(zdouble) * n);
+  pc_usolve_h(p, WT, Z);
+  zdouble PSI = csqrt(zdot(n, Z, Z));
+  zdouble GAMM = 1.0, ETA = -1.0, EPSIL = 0.0, THET = 0.0, THET1 = 0.0;
+  while (rerr[iter - 1] > tol && iter < maxit) {
+    if (RHO == 0.0 || PSI == 0.0) { failed = 2; break; }
+    zdouble rhoInv = zdiv(1.0, RHO) * CMPLX(1.0, 0.0);
+    zdouble psiInv = zdiv(1.0, PSI) * CMPLX(1.0, 0.0);
+    for (i = 0; i < n; i++) V[i] = VT[i] * rhoInv;
+    for (i = 0; i < n; i++) W[i] = WT[i] * psiInv;
+    for (i = 0; i < n; i++) Y[i] = Y[i] * rhoInv;
+    for (i = 0; i < n; i++) Z[i] = Z[i] * psiInv;
+    zdouble DELTA = zdot(n, Z, Y);
+    if (DELTA == 0.0) { failed = 2; break; }
+    utsolve_z(&p->U, Y, YT);
+    pc_lsolve_h(p, Z, ZT);
+    if (iter == 1) {
+      memcpy(P, YT, sizeof(zdouble) * n);
+      memcpy(Q, ZT, sizeof(zdouble) * n);
+    } else {
+      zdouble PDE = zdiv(PSI * DELTA, EPSIL);
+      zdouble RDE = RHO * conj(zdiv(DELTA, EPSIL));
+      for (i = 0; i < n; i++) P[i] = YT[i] - PDE * P[i];
+      for (i = 0; i < n; i++) Q[i] = ZT[i] - RDE * Q[i];
+    }
+    sp2o_mult_aii(p, 0, P, PT);
+    EPSIL = zdot(n, Q, PT);
+    if (EPSIL == 0.0) { failed = 2; break; }
+    zdouble BETA = zdiv(EPSIL, DELTA);
+    if (BETA == 0.0) { failed = 2; break; }
+    for (i = 0; i < n; i++) VT[i] = PT[i] - BETA * V[i];
+    zdouble RHO1 = RHO;
+    ltsolve_z(&p->L, VT, Y);
+    RHO = csqrt(zdot(n, Y, Y));
+    sp2o_mult_aii(p, 1, Q, WT);
+    for (i = 0; i < n; i++) WT[i] = WT[i] - conj(BETA) * W[i];
+    pc_usolve_h(p, WT, Z);
+    PSI = csqrt(zdot(n, Z, Z));
+    if (iter > 1) THET1 = THET;
+    THET = zdiv(RHO, GAMM * cabs(BETA));
+    zdouble GAMM1 = GAMM;
+    GAMM = zdiv(1.0, csqrt(1.0 + THET * THET));
+    if (GAMM == 0.0) break;
+    ETA = -zdiv(ETA * RHO1 * GAMM * GAMM, BETA * GAMM1 * GAMM1);
+    if (iter == 1) {
+      for (i = 0; i < n; i++) D[i] = ETA * P[i];
+      for (i = 0; i < n; i++) S[i] = ETA * PT[i];
+    } else {
+      zdouble TM2 = THET1 * THET1 * GAMM * GAMM;
+      for (i = 0; i < n; i++) D[i] = ETA * P[i] + TM2 * D[i];
+      for (i = 0; i < n; i++) S[i] = ETA * PT[i] + TM2 * S[i];
+    }
+    for (i = 0; i < n; i++) x[i] = x[i] + D[i];
+    for (i = 0; i < n; i++) R[i] = R[i] - S[i];
+    rnorm = creal(csqrt(zdot(n, R, R)));
+    if (rnorm < rnormin) rnormin = rnorm;
+    iter = iter + 1;
+    rerr[iter - 1] = (float)(rnorm / bnorm);
+  }
+  *niter = iter;
+  rerr[iter - 1] = (float)(rnormin / bnorm);
+done:
+  free(R); free(VT); free(Y); free(Z); free(WT); free(V); free(W); free(YT); free(ZT);
+  free(P); free(Q); free(PT); free(D); free(S);
+  if (failed) return failed;
+  return *niter < maxit ? 0 : 1;
+}
+
 /* ------------------------------------------------- divergence correction */
 /* lowerTri_Real/upperTri_Real + Dilu_Real spOpTools.f90:1092-1186, 1903-1948 */
 static o_csr tri_real(const o_csr *A, int lower) {

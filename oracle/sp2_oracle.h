@@ -72,6 +72,16 @@ void sp2o_pc_usolve(const sp2o_problem *p, const zdouble *x, zdouble *y);
 int sp2o_bicg(const sp2o_problem *p, const zdouble *b, zdouble *x, int maxit, double tol,
               int gpu_semantics, int precond, int *niter, double *rerr);
 
+/* PC_Lsolve / PC_Usolve with adjt = .true.: y = L^-H x, y = U^-H x */
+void sp2o_pc_lsolve_h(const sp2o_problem *p, const zdouble *x, zdouble *y);
+void sp2o_pc_usolve_h(const sp2o_problem *p, const zdouble *x, zdouble *y);
+/* QMR (solver.f90:140-370), preconditioned by the block DILU's L and U.
+ * Returns 0 converged (niter < maxit), 1 not converged, 2 breakdown, 3 NaN
+ * right-hand side; x holds the last iterate, rerr[niter-1] the smallest
+ * relative residual. */
+int sp2o_qmr(const sp2o_problem *p, const zdouble *b, zdouble *x, int maxit, double tol,
+             int *niter, double *rerr);
+
 /* Divergence correction: DivCorInit + DivCorSetup, PCG (solver.f90:57-136) */
 void sp2o_divcor_setup(sp2o_problem *p);
 int sp2o_pcg(const sp2o_problem *p, const zdouble *b, zdouble *x, int maxit, double tol,

@@ -50,6 +50,9 @@ def lib():
             "sp2o_pc_lsolve": (None, [_P, _PZ, _PZ]),
             "sp2o_pc_usolve": (None, [_P, _PZ, _PZ]),
             "sp2o_bicg": (_I, [_P, _PZ, _PZ, _I, _D, _I, _I, C.POINTER(_I), _PD]),
+            "sp2o_qmr": (_I, [_P, _PZ, _PZ, _I, _D, C.POINTER(_I), _PD]),
+            "sp2o_pc_lsolve_h": (None, [_P, _PZ, _PZ]),
+            "sp2o_pc_usolve_h": (None, [_P, _PZ, _PZ]),
             "sp2o_divcor_setup": (None, [_P]),
             "sp2o_pcg": (_I, [_P, _PZ, _PZ, _I, _D, C.POINTER(_I), _PD]),
             "sp2o_divc": (None, [_P, _PZ, _PZ]),
@@ -174,6 +177,25 @@ class Problem:
         nit = _I()
         st = lib().sp2o_bicg(self._h, np.ascontiguousarray(b, np.complex128), x, maxit,
                              float(tol), int(gpu_semantics), int(precond), C.byref(nit), rerr)
+        return x, {"status": st, "niter": nit.value, "rerr": rerr}
+
+    def pc_lsolve_h(self, x):
+        y = np.zeros(self.nInt, np.complex128)
+        lib().sp2o_pc_lsolve_h(self._h, np.ascontiguousarray(x, np.complex128), y)
+        return y
+
+    def pc_usolve_h(self, x):
+        y = np.zeros(self.nInt, np.complex128)
+        lib().sp2o_pc_usolve_h(self._h, np.ascontiguousarray(x, np.complex128), y)
+        return y
+
+    def qmr(self, b, x0=None, maxit=500, tol=1e-7):
+        """QMR (solver.f90:140-370); returns (x, info) like bicg."""
+        x = np.zeros(self.nInt, np.complex128) if x0 is None else np.array(x0, np.complex128)
+        rerr = np.zeros(maxit)
+        nit = _I()
+        st = lib().sp2o_qmr(self._h, np.ascontiguousarray(b, np.complex128), x, maxit, float(tol),
+                            C.byref(nit), rerr)
         return x, {"status": st, "niter": nit.value, "rerr": rerr}
 
     def divcor_setup(self):

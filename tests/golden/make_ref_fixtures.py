@@ -9,10 +9,12 @@ from them (ref_io.npz), and a solution file written by the reference's
 writer with its inputs, and tests/golden/ref_jmultT.npz: the reference's
 sensitivity chain for one MT transmitter at two frequencies (forward fields,
 LmultT sources, the TRN solves of sensSolve, PmultT and JmultT_TX) on a small
-seeded grid.  Inputs and outputs only -- data, no reference code.
+seeded grid, and tests/golden/ref_qmr.npz: its CPU QMR solves and the
+adjoint applications (A^H, L^-H, U^-H) they use.  Inputs and outputs only --
+data, no reference code.
 
 Run where the reference is mounted:
-    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0] [io] [jmultT]
+    python tests/golden/make_ref_fixtures.py [small] [datafunc] [bcx0] [io] [jmultT] [qmr]
 """
 import os
 import sys
@@ -277,6 +279,30 @@ def io_main():
     print("wrote", d)
 
 
+def qmr_main():
+    """The reference's CPU QMR (solver.f90:140-370) and the adjoint
+    applications it uses, on the small grid of case(), two frequencies."""
+    rng, dx, dy, dz, nza, lc = case()
+    grid = dict(dx=dx, dy=dy, dz=dz, nzAir=nza)
+    om = [2 * np.pi * 1.0, 2 * np.pi * 0.01]
+    n = len(refdriver.run(grid, lc, np.log(1e-10), om[:1], dump=True)["EDGEi"])
+    rng = np.random.default_rng(99)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x0 = 0.1 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    out = refdriver.run(grid, lc, np.log(1e-10), om, qmr=[(0, 300, 1e-10, b, x0), (1, 300, 1e-10, b, x0)],
+                        adj_applies=[(0, b), (1, x0)], dump=False)
+    d = dict(dx=dx, dy=dy, dz=dz, nzAir=nza, logcond=lc, omegas=np.array(om), b=b, x0=x0, maxit=300,
+             tol=1e-10)
+    for k in range(2):
+        q = out[f"qmr_{k}"]
+        d.update({f"qmr_x_{k}": q["x"], f"qmr_niter_{k}": q["niter"], f"qmr_rerr_{k}": q["rerr"],
+                  f"qmr_failed_{k}": q["failed"]})
+        a = out[f"adj_{k}"]
+        d.update({f"adj_AHx_{k}": a["AHx"], f"adj_LH_{k}": a["LHsolve"], f"adj_UH_{k}": a["UHsolve"]})
+    np.savez_compressed(os.path.join(HERE, "ref_qmr.npz"), **d)
+    print("wrote", os.path.join(HERE, "ref_qmr.npz"))
+
+
 def jmult_case():
     """Grid, model, sites and full-impedance data of the sensitivity fixture."""
     sys.path.insert(0, os.path.join(ROOT, "modem-gpu_amd"))
@@ -305,7 +331,9 @@ def jmult_main():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["small", "datafunc", "bcx0", "io", "jmultT"]
+    which = sys.argv[1:] or ["small", "datafunc", "bcx0", "io", "jmultT", "qmr"]
+    if "qmr" in which:
+        qmr_main()
     if "jmultT" in which:
         jmult_main()
     if "small" in which:

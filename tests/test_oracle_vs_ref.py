@@ -99,3 +99,71 @@ def test_live_reference_bit_exact(seed):
     p.update_freq(om[0])
     assert np.array_equal(p.mult_aii(b), ref["apply_0"]["Ax"])
     assert np.array_equal(p.pc_usolve(p.pc_lsolve(b)), ref["apply_0"]["LUsolve"])
+
+
+QMR_GOLD = os.path.join(os.path.dirname(__file__), "golden", "ref_qmr.npz")
+
+
+def _qmr_check(p, om, b, x0, maxit, tol, ref):
+    """Replays the driver's UpdateFreq sequence (every frequency once, then
+    one per QMR record, then one per adjoint record) and compares."""
+    for w in om:
+        p.update_freq(w)
+    for k, w in enumerate(om):
+        p.update_freq(w)
+        x, info = p.qmr(b, x0, maxit=maxit, tol=tol)
+        assert info["niter"] == ref[f"qmr_niter_{k}"], k
+        assert np.array_equal(x, ref[f"qmr_x_{k}"]), k
+        assert np.array_equal(info["rerr"], ref[f"qmr_rerr_{k}"]), k
+    for k, (w, v) in enumerate(zip(om, (b, x0))):
+        p.update_freq(w)
+        assert np.array_equal(p.mult_aii(v, adjt=True), ref[f"adj_AHx_{k}"]), k
+        assert np.array_equal(p.pc_lsolve_h(v), ref[f"adj_LH_{k}"]), k
+        assert np.array_equal(p.pc_usolve_h(v), ref[f"adj_UH_{k}"]), k
+
+
+def test_fixture_qmr_bit_exact():
+    """QMR (solver.f90:140-370) and the adjoint applications it uses
+    (Mult_Aii adjt, PC_Lsolve/PC_Usolve adjt) against the reference's."""
+    g = np.load(QMR_GOLD)
+    p = sp2oracle.Problem(g["dx"], g["dy"], g["dz"], int(g["nzAir"]), g["logcond"])
+    _qmr_check(p, g["omegas"], g["b"], g["x0"], int(g["maxit"]), float(g["tol"]), g)
+    # one of the two systems converges, the other is cut at maxit or converges
+    assert int(g["qmr_niter_0"]) < int(g["maxit"])
+
+
+@pytest.mark.skipif(not refdriver.available(), reason="reference build not present")
+def test_live_qmr_bit_exact():
+    rng = np.random.default_rng(17)
+    nx, ny, nza, nze = 9, 8, 3, 6
+    dx = rng.uniform(100, 400, nx)
+    dy = rng.uniform(100, 400, ny)
+    dz = np.concatenate([[8000.0, 1500.0, 300.0], rng.uniform(40, 250, nze)])
+    lc = np.log(rng.uniform(0.002, 0.5, (nx, ny, nze)))
+    grid = dict(dx=dx, dy=dy, dz=dz, nzAir=nza)
+    p = sp2oracle.Problem(dx, dy, dz, nza, lc)
+    n = p.nInt
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x0 = np.zeros(n, complex)
+    om = [2 * np.pi * 3.0, 2 * np.pi * 0.05]
+    maxit, tol = 40, 1e-9
+    out = refdriver.run(grid, lc, np.log(1e-10), om, qmr=[(0, maxit, tol, b, x0), (1, maxit, tol, b, x0)],
+                        adj_applies=[(0, b), (1, x0 + b)], dump=False)
+    ref = {}
+    for k in range(2):
+        ref.update({f"qmr_x_{k}": out[f"qmr_{k}"]["x"], f"qmr_niter_{k}": out[f"qmr_{k}"]["niter"],
+                    f"qmr_rerr_{k}": out[f"qmr_{k}"]["rerr"],
+                    f"adj_AHx_{k}": out[f"adj_{k}"]["AHx"], f"adj_LH_{k}": out[f"adj_{k}"]["LHsolve"],
+                    f"adj_UH_{k}": out[f"adj_{k}"]["UHsolve"]})
+    for w in om:
+        p.update_freq(w)
+    for k, w in enumerate(om):
+        p.update_freq(w)
+        x, info = p.qmr(b, x0, maxit=maxit, tol=tol)
+        assert info["niter"] == ref[f"qmr_niter_{k}"] and np.array_equal(x, ref[f"qmr_x_{k}"])
+        assert np.array_equal(info["rerr"], ref[f"qmr_rerr_{k}"])
+    for k, (w, v) in enumerate(zip(om, (b, x0 + b))):
+        p.update_freq(w)
+        assert np.array_equal(p.mult_aii(v, adjt=True), ref[f"adj_AHx_{k}"])
+        assert np.array_equal(p.pc_lsolve_h(v), ref[f"adj_LH_{k}"])
+        assert np.array_equal(p.pc_usolve_h(v), ref[f"adj_UH_{k}"])
