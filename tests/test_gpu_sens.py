@@ -14,9 +14,9 @@ Checks and tolerances:
       the flang build of SensComp.f90 JmultT_TX): every solve to 1e-10, so
       fields rel L2 <= 1e-7 and the cell gradient rel max-norm <= 1e-6
   exact gradient at 80x80x50+10, 12 frequencies, 144 sites (forward and A^T
-      solves batched on the GPU, mixed precision to 1e-12) vs central
-      differences of the misfit (step 1e-4 in ln sigma) on 20 seeded cells:
-      rel <= 1e-3
+      solves batched on the GPU, mixed precision to 3e-14) vs central
+      differences of the misfit (step 1e-4 in ln sigma) on 20 seeded cells
+      among those whose gradient is at least 5 % of the largest: rel <= 1e-3
 """
 import os
 
@@ -205,11 +205,14 @@ class Misfit:
             self.ctx.dev_free(p)
 
     def forward(self, lc):
+        """Z(m), each solve started from the boundary fields' interior values
+        (as FWDsolve3D starts)."""
         ctx = self.ctx
         ctx.update_cond(lc)
         ctx.rhs_fwd_dev(self.ns, self.pbc, self.pb, self.px)
         st = ctx.solve_batch_dev(self.om, self.pb, self.px, tol=self.tol, maxit=20000,
                                  precision=mtsp2.PREC_MIXED, precond=mtsp2.PC_DILU_RB)
+        self.last = st
         assert all(s["status"] == 0 for s in st), st
         ctx.dev_copy(self.pe, self.pbc, self.ns * self.ne * 16)
         ctx.post_fwd_dev(self.ns, self.px, self.pe)
@@ -252,7 +255,10 @@ def test_gradient_vs_central_differences_80x80x50(gpu_ctx, capsys):
     ctx = gpu_ctx
     ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc0)
     sites = P.config_sites("cfg4", grid)
-    M = Misfit(ctx, grid, lc0, P.config_freqs("cfg4"), sites, tol=1e-12)
+    # every solve to 3e-14, near FP64's floor for this operator (about 1.1e-14):
+    # the misfit's change under a step of 1e-4 in one cell's ln sigma must
+    # stand above the solves' error in the fields (at 1e-12 it does not)
+    M = Misfit(ctx, grid, lc0, P.config_freqs("cfg4"), sites, tol=3e-14)
     try:
         rng = np.random.default_rng(4)
         # synthetic data: a perturbed "true" model, 5 % Gaussian noise on the
@@ -264,10 +270,16 @@ def test_gradient_vs_central_differences_80x80x50(gpu_ctx, capsys):
         M.d = Zt + floor * (rng.standard_normal(Zt.shape) + 1j * rng.standard_normal(Zt.shape)) / np.sqrt(2)
         M.err = np.broadcast_to(floor, Zt.shape)
         g = M.gradient(lc0)
-        say(f"adjoint gradient done, max |g| {np.max(np.abs(g)):.3e}")
-        nx, ny, nze = lc0.shape
-        cells = [(int(rng.integers(10, nx - 10)), int(rng.integers(10, ny - 10)),
-                  int(rng.integers(0, 12))) for _ in range(20)]
+        say(f"adjoint gradient done, max |g| {np.max(np.abs(g)):.3e}; forward iters "
+            f"{[(q['niter'], q['nouter']) for q in M.last]}, relres {max(q['relres'] for q in M.last):.1e}")
+        say(f"phi(m0) = {M.phi(lc0):.10e}")
+        # 20 seeded cells among those the data resolve: gradient at least 5 %
+        # of the largest (a step of 1e-4 in ln sigma of a cell the data do not
+        # see changes the misfit by less than the solves' rounding)
+        cand = np.argwhere(np.abs(g) >= 0.05 * np.max(np.abs(g)))
+        say(f"{len(cand)} cells with |g| >= 5 % of the largest")
+        pick = rng.choice(len(cand), size=min(20, len(cand)), replace=False)
+        cells = [tuple(int(v) for v in cand[q]) for q in pick]
         h = 1e-4
         rows = []
         for c in cells:
@@ -279,6 +291,5 @@ def test_gradient_vs_central_differences_80x80x50(gpu_ctx, capsys):
             say(f"cell {c}: FD {fd:+.6e}  adjoint {g[c]:+.6e}  rel {abs(fd - g[c]) / abs(fd):.1e}")
     finally:
         M.close()
-    scale = max(abs(fd) for _, fd, _ in rows)
     for c, fd, ga in rows:
-        assert abs(fd - ga) <= 1e-3 * max(abs(fd), 1e-2 * scale), (c, fd, ga)
+        assert abs(fd - ga) <= 1e-3 * abs(fd), (c, fd, ga)
