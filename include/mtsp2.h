@@ -346,6 +346,19 @@ int mt_data_resp(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const 
  * each functional's first component. */
 int mt_lmultT(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const void *e_dev,
               const double *d, const unsigned char *exist, void *comb_dev);
+/* mt_lmultT without any device allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops such as an inversion's
+ * gradient loop).  No counterpart in the reference.  Same arguments and
+ * results as mt_lmultT; its scratch (the per-(frequency, site) functional
+ * coefficients, the data, the flags and the frequencies) is a file-scope
+ * device array of 4.8 MB held by every GPU the library runs on until the
+ * process ends: nfreq * nsite <= 16384 and nfreq <= 1024, else MT_E_ARG.
+ * Every call on one GPU shares that array (which keeps the last call's
+ * contents), so calls that overlap on one GPU race; comb_dev is cleared by a
+ * kernel, not a memset.  Receivers come from mt_receivers_set as for
+ * mt_lmultT (set-up, which allocates). */
+int mt_lmultT_noalloc(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const void *e_dev,
+                      const double *d, const unsigned char *exist, void *comb_dev);
 
 /* Runs the SpMV autotuner for nsys systems: times the solver's SpMV kernel for
  * several row-tile group counts per system and keeps the fastest; *variant

@@ -29,6 +29,21 @@
 using namespace mt;
 
 // ============================================================== context
+// Scratch of the allocation-free LmultT (mt_lmultT_noalloc): one array per
+// GPU, alive until the process ends, carved into the per-(frequency, site)
+// functional coefficients (14 complex each), the data and presence flags (up
+// to 8 components each) and the frequencies.  Host code never uses this name
+// as a device address: mt_ctx_create resolves the device's copy with
+// hipGetSymbolAddress and every use goes through that pointer.
+constexpr int kNoAllocMaxSiteFreq = 16384;   // nfreq * nsite at most
+constexpr int kNoAllocMaxFreq = 1024;
+constexpr size_t kNoAllocCoefOff = 0;
+constexpr size_t kNoAllocDataOff = kNoAllocCoefOff + size_t(kNoAllocMaxSiteFreq) * 14 * sizeof(Cd);
+constexpr size_t kNoAllocExistOff = kNoAllocDataOff + size_t(kNoAllocMaxSiteFreq) * 8 * sizeof(double);
+constexpr size_t kNoAllocOmegaOff = kNoAllocExistOff + size_t(kNoAllocMaxSiteFreq) * 8;
+constexpr size_t kNoAllocBytes = kNoAllocOmegaOff + size_t(kNoAllocMaxFreq) * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 struct DeviceOperator {
   bool ready = false;
   double* coef64 = nullptr;  // [15][nE]
@@ -123,6 +138,7 @@ struct mt_ctx {
   struct KTime { long long launches = 0; double ms = 0.0, bytes = 0.0; } kt[2];
   std::vector<hipEvent_t> tev;  // event pool
   int64_t nnz_stencil = -1;     // stored AAii couplings (cached)
+  unsigned char* scratch_dev = nullptr;  // this device's copy of `scratch`
 };
 
 static int mt_fail(mt_ctx* ctx, int code, const std::string& msg) {
@@ -1149,6 +1165,16 @@ int mt_ctx_create(int device, mt_ctx** out) {
     return MT_E_DEVICE;
   }
   for (auto& e : ctx->ev) hipEventCreate(&e);
+  // the device address of this GPU's scratch array (the allocation-free
+  // entry points use it; the array's host-side name is no device address)
+  if (hipGetSymbolAddress(reinterpret_cast<void**>(&ctx->scratch_dev), HIP_SYMBOL(scratch)) !=
+          hipSuccess ||
+      !ctx->scratch_dev) {
+    (void)hipGetLastError();
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MT_E_DEVICE;
+  }
   int ncu = 0;
   if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
       ncu > 0)
@@ -2082,9 +2108,15 @@ int mt_receivers_set(mt_ctx* ctx, int nx, int ny, int nzAir, int nzEarth, const 
   return MT_OK;
 }
 
+// comb = 0 without hipMemsetAsync (a fill kernel, safe inside captured work)
+__global__ void k_zero_cd(int64_t n, Cd* __restrict__ out) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    out[q] = {0.0, 0.0};
+}
+
 static int site_launch(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const void* e_dev,
                        double* resp_host, const double* d, const unsigned char* exist,
-                       void* comb_dev) {
+                       void* comb_dev, bool noalloc = false) {
   const SiteData& S = ctx->sites;
   const int ncomp = dtype_ncomp(dtype);
   if (!ncomp) return mt_fail(ctx, MT_E_ARG, "data type must be 1..11 (dataTypes.f90 numbering)");
@@ -2097,19 +2129,36 @@ static int site_launch(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, 
   const size_t nsd = size_t(nfreq) * S.nsite;
   void *w_om, *w_out, *w_d = nullptr, *w_ex = nullptr;
   int rc;
-  if ((rc = ws_get(ctx, "df_omega", nfreq * sizeof(double), &w_om))) return rc;
-  if ((rc = ws_get(ctx, "df_out", adj ? nsd * 14 * sizeof(Cd) : nsd * ncomp * sizeof(double), &w_out)))
-    return rc;
+  if (noalloc) {
+    // every buffer in this device's scratch array, nothing allocated
+    if (!adj) return mt_fail(ctx, MT_E_ARG, "the allocation-free path is LmultT's");
+    if (nfreq > kNoAllocMaxFreq || nsd > size_t(kNoAllocMaxSiteFreq))
+      return mt_fail(ctx, MT_E_ARG, "mt_lmultT_noalloc: nfreq * nsite = " + std::to_string(nsd) +
+                                        " (nfreq " + std::to_string(nfreq) + ") exceeds its scratch: at most " +
+                                        std::to_string(kNoAllocMaxSiteFreq) + " (nfreq at most " +
+                                        std::to_string(kNoAllocMaxFreq) + ")");
+    unsigned char* base = ctx->scratch_dev;
+    if (!base) return mt_fail(ctx, MT_E_STATE, "scratch array address not resolved");
+    w_out = base + kNoAllocCoefOff;
+    w_d = base + kNoAllocDataOff;
+    w_om = base + kNoAllocOmegaOff;
+    if (exist) w_ex = base + kNoAllocExistOff;
+  } else {
+    if ((rc = ws_get(ctx, "df_omega", nfreq * sizeof(double), &w_om))) return rc;
+    if ((rc = ws_get(ctx, "df_out", adj ? nsd * 14 * sizeof(Cd) : nsd * ncomp * sizeof(double), &w_out)))
+      return rc;
+    if (adj) {
+      if ((rc = ws_get(ctx, "df_d", nsd * ncomp * sizeof(double), &w_d))) return rc;
+      if (exist && (rc = ws_get(ctx, "df_exist", nsd * ncomp, &w_ex))) return rc;
+    }
+  }
   MT_HIP_CHECK(ctx, hipMemcpyAsync(w_om, omegas, nfreq * sizeof(double), hipMemcpyHostToDevice,
                                    ctx->stream));
   if (adj) {
-    if ((rc = ws_get(ctx, "df_d", nsd * ncomp * sizeof(double), &w_d))) return rc;
     MT_HIP_CHECK(ctx, hipMemcpyAsync(w_d, d, nsd * ncomp * sizeof(double), hipMemcpyHostToDevice,
                                      ctx->stream));
-    if (exist) {
-      if ((rc = ws_get(ctx, "df_exist", nsd * ncomp, &w_ex))) return rc;
+    if (exist)
       MT_HIP_CHECK(ctx, hipMemcpyAsync(w_ex, exist, nsd * ncomp, hipMemcpyHostToDevice, ctx->stream));
-    }
   }
   SiteArgs a;
   a.nsite = S.nsite; a.nfreq = nfreq; a.dtype = dtype; a.ncomp = ncomp; a.want_coef = adj;
@@ -2125,7 +2174,12 @@ static int site_launch(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, 
   hipLaunchKernelGGL(k_site_eval, dim3((nt + 63) / 64), dim3(64), 0, ctx->stream, a);
   MT_HIP_CHECK(ctx, hipGetLastError());
   if (adj) {
-    MT_HIP_CHECK(ctx, hipMemsetAsync(comb_dev, 0, size_t(nfreq) * 2 * S.nE * sizeof(Cd), ctx->stream));
+    const int64_t ncomb = int64_t(nfreq) * 2 * S.nE;
+    if (noalloc)
+      k_zero_cd<<<unsigned(std::min<int64_t>(4096, (ncomb + 255) / 256)), 256, 0, ctx->stream>>>(
+          ncomb, (Cd*)comb_dev);
+    else
+      MT_HIP_CHECK(ctx, hipMemsetAsync(comb_dev, 0, size_t(ncomb) * sizeof(Cd), ctx->stream));
     const int64_t ng = int64_t(nfreq) * S.nu;
     if (ng > 0) {
       hipLaunchKernelGGL(k_lmultT_gather, dim3(unsigned((ng + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -2151,6 +2205,12 @@ int mt_lmultT(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const voi
               const double* d, const unsigned char* exist, void* comb_dev) {
   if (!ctx || !d || !comb_dev) return mt_fail(ctx, MT_E_ARG, "mt_lmultT: null argument");
   return site_launch(ctx, dtype, nfreq, omegas, e_dev, nullptr, d, exist, comb_dev);
+}
+
+int mt_lmultT_noalloc(mt_ctx* ctx, int dtype, int nfreq, const double* omegas, const void* e_dev,
+                      const double* d, const unsigned char* exist, void* comb_dev) {
+  if (!ctx || !d || !comb_dev) return mt_fail(ctx, MT_E_ARG, "mt_lmultT_noalloc: null argument");
+  return site_launch(ctx, dtype, nfreq, omegas, e_dev, nullptr, d, exist, comb_dev, true);
 }
 
 // ---------------------------------------------------------------------------

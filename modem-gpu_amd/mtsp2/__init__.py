@@ -106,6 +106,7 @@ _SIGS = {
     "mt_receivers_set": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _P, _I, _PD, _P, _P]),
     "mt_data_resp": (_I, [_P, _I, _I, _PD, _P, _PD]),
     "mt_lmultT": (_I, [_P, _I, _I, _PD, _P, _PD, _P, _P]),
+    "mt_lmultT_noalloc": (_I, [_P, _I, _I, _PD, _P, _PD, _P, _P]),
 }
 
 
@@ -512,6 +513,16 @@ class Context:
         ex = None if exist is None else np.ascontiguousarray(exist, np.uint8)
         self._chk(lib().mt_lmultT(self._h, int(dtype), len(om), om, C.c_void_p(e_ptr), d.ravel(),
                                   None if ex is None else ex.ctypes.data, C.c_void_p(comb_ptr)))
+
+    def lmultT_noalloc(self, dtype, omegas, e_ptr, d, comb_ptr, exist=None):
+        """lmultT without device allocations (scratch in a static device
+        array; nfreq * nsite <= 16384)."""
+        om = np.ascontiguousarray(np.atleast_1d(omegas), np.float64)
+        d = np.ascontiguousarray(d, np.float64)
+        ex = None if exist is None else np.ascontiguousarray(exist, np.uint8)
+        self._chk(lib().mt_lmultT_noalloc(self._h, int(dtype), len(om), om, C.c_void_p(e_ptr),
+                                          d.ravel(), None if ex is None else ex.ctypes.data,
+                                          C.c_void_p(comb_ptr)))
 
     def bench_spmv(self, nsys, precision=PREC_FP64, reps=20):
         ms, by = _D(), _D()
