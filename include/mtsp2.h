@@ -388,6 +388,18 @@ int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
  * workgroups per CU).  The same operator; 5 rounds the black tiles' backward
  * sweep differently.  No counterpart in the reference. */
 int mt_set_dilu_variant(mt_ctx *ctx, int variant);
+/* Layout of the mixed solver's inner FP32 vectors when they are
+ * preconditioned by the red-black DILU with the default SpMV: 1 (default) =
+ * bricks (each 8x8x8 tile of the red-black plan contiguous), 0 = the all-edge
+ * layout of the outer solve.  The same operator and preconditioner; the
+ * inner sums differ in rounding.  No counterpart in the reference. */
+int mt_set_inner_layout(mt_ctx *ctx, int bricks);
+/* The inner solves' brick kernels on one vector, for checking them: y = A x
+ * (what = 0) or y = M^-1 x (what = 1, the red-black DILU of A(omega)), of A
+ * (transpose = 0) or A^T (1), computed in FP32 on bricks; x, y: n_int
+ * complex in EDGEi order.  No counterpart in the reference. */
+int mt_inner_apply(mt_ctx *ctx, double omega, int what, int transpose, const mt_zdouble *x,
+                   mt_zdouble *y);
 /* Fixes the row-tile group count per system (0: automatic). */
 int mt_set_tuning(mt_ctx *ctx, int groups_per_system);
 /* Times the solver's fused SpMV+dot kernel (v = A x, <rt,v>) for nsys systems

@@ -214,6 +214,30 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// Sum of a double over the wave's 64 lanes, valid in lane 63, by DPP lane
+// moves inside the VALU (no LDS crossbar round trips): quad [1,0,3,2],
+// quad [2,3,0,1], half-row mirror and row mirror leave each 16-lane row's
+// sum in all of its lanes; row broadcast 15 adds row 0 into row 1 and row 2
+// into row 3, row broadcast 31 adds lane 31 into rows 2 and 3.  A fixed
+// order, so the result is deterministic.  Every lane of the wave must be
+// active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_d(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+__device__ __forceinline__ double wave_sum63(double v) {
+  v += dpp_d<0xb1, 0xf>(v);    // quad_perm [1,0,3,2]
+  v += dpp_d<0x4e, 0xf>(v);    // quad_perm [2,3,0,1]
+  v += dpp_d<0x141, 0xf>(v);   // row_half_mirror
+  v += dpp_d<0x140, 0xf>(v);   // row_mirror
+  v += dpp_d<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_d<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 // Sums NV complex values over the block; result valid in thread 0.
 template <int NV, int BS>
 __device__ __forceinline__ void block_sum(Z (&v)[NV], Z* lds /* >= NV*BS/64 */) {

@@ -37,12 +37,34 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 constexpr int kMaxTilePts = 1024;
+// LDS box of the lean red-black sweeps (k_dilu_rbl): the 8x8x8 tile and one
+// pad layer on the low side, i fastest, row pitch 9 and plane pitch 82.  With
+// a plane pitch of 81 the points of a hyperplane i+j+k = h sit 8*dj + 80*dk
+// apart, which is 0 or 8 modulo 16: every 8-byte access of a hyperplane step
+// fell into two of the sixteen bank pairs.  Pitch 82 spreads them over all of
+// them (one spare slot per plane).  An upper neighbour past the end of a row,
+// plane or box lands in a low pad slot (index mod kRbNB), disjoint from the
+// tile and from the other high faces.
+constexpr int kRbBX = 9, kRbPK = 82, kRbNB = 9 * kRbPK, kRbNP = 512;
+// packed faces per tile and system in DiluArgs::xf: six faces of 64 (k_dilu_bk;
+// the all-edge-layout red-black kernels use the first two, the i-faces)
+constexpr int kRbFaces = 384;
+__host__ __device__ __forceinline__ int rb_box(int li, int lj, int lk) {
+  return (li + 1) + kRbBX * (lj + 1) + kRbPK * (lk + 1);
+}
 struct TileShape {
   int tx, ty, tz;
   int npts;
   int nlev;
   int lvl[64];                  // level start offsets into pts, nlev+1 entries
   unsigned int pts[kMaxTilePts];  // packed (li | lj<<8 | lk<<16) sorted by li+lj+lk
+  // 8x8x8 tiles only (k_dilu_rbl): the same levels with the points of a level
+  // ordered so that lanes of a 16-lane group hit distinct box banks, packed
+  // li | lj<<3 | lk<<6 | slot<<9 where slot is the position in this list, and
+  // slot of tile point li + 8*(lj + 8*lk).  The per-point LDS data (pivots,
+  // couplings) are stored by slot, so a level's lanes read consecutive words.
+  unsigned int rpts[kRbNP];
+  unsigned short rslot[kRbNP];
 };
 
 struct LatticeInfo {
@@ -71,12 +93,15 @@ struct DiluArgs {
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed
-  // red-black sweeps: the i-faces (li = 0 and li = 7) of every tile, packed
-  // [half][system][tile][face][64], entry lj + 8*lk.  Half 0 holds dinv*y of
-  // the red tiles (written by the red forward sweep, read by the black one),
-  // half 1 holds z of the black tiles (written by the black sweep, read by
-  // the red backward one).  Packed, a neighbour face is one coalesced read
-  // instead of 64 reads of 8 bytes, one per cache line.
+  // red-black sweeps: the faces of every tile, packed
+  // [half][system][tile][face][64]: face 2d is the low and 2d+1 the high face
+  // in direction d (i, j, k), entry = the other two tile coordinates a + 8*b
+  // in (i, j, k) order; kRbFaces entries per tile (k_dilu_bk uses all six,
+  // the all-edge-layout red-black kernels the i-faces).  Half 0 holds dinv*y of the red
+  // tiles (written by the red forward sweep, read by the black one), half 1
+  // holds z of the black tiles (written by the black sweep, read by the red
+  // backward one).  Packed, a neighbour face is one coalesced read instead of
+  // 64 reads of 8 bytes spread over 8 or 64 cache lines.
   C2<T>* xf = nullptr;
   int64_t xfsys = 0, xfhalf = 0;
 };
@@ -857,10 +882,12 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbm(DiluArgs<T, V> a, const Ti
 //  * The right-hand side lives in the box in place: a point's forward step
 //    reads its own box entry, so the per-point data shrink to the pivots
 //    (red forward then keeps y in the pivot's slot, which it no longer needs).
-//  * The box is 9 x 9 x 9: the tile and one pad layer on the low side.  An
+//  * The box (rb_box) is the tile and one pad layer on the low side, with a
+//    plane pitch that spreads a hyperplane over the LDS banks.  An
 //    upper-neighbour index past the end wraps to the start, which puts the
 //    high faces in low-side pad slots, disjoint from one another and from the
-//    tile.  A mode reads one side's faces from the box (red forward none,
+//    tile.  Pivots and couplings are stored by level slot, so the lanes of a
+//    sweep step read consecutive entries.  A mode reads one side's faces from the box (red forward none,
 //    black the low ones, red backward the high ones); the other side's terms
 //    are folded into the right-hand side before the sweeps by the lanes that
 //    loaded the face values, one direction after another, which is the
@@ -871,12 +898,13 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbm(DiluArgs<T, V> a, const Ti
 //  * System tables per workgroup.
 template <class T, class V, int MODE, int NW>
 __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const TileShape* shp) {
-  constexpr int TX = 8, NP = 512, NU = 8, NLEV = 22, BX = 9, NB = 729;
+  constexpr int TX = 8, NP = kRbNP, NU = 8, NLEV = 22, NB = kRbNB;
   constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
   constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
   constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
   struct Cpl { V c[3]; };
   __shared__ C2<T> Wall[NW][NB];                 // per system: the box
+  // per-point data by level slot (TileShape::rslot)
   __shared__ C2<T> DVall[NW][NP];                // per system: pivots (red forward: then y)
   __shared__ Cpl CL[FWDP ? NP : 1];              // raw lower couplings (slots 1, 3, 5)
   __shared__ Cpl CH[BWDP ? NP : 1];              // raw upper couplings (slots 2, 4, 6)
@@ -912,10 +940,11 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
   const int ex = min(TX, L.A - gi0), ey = min(TX, L.B - gj0), ez = min(TX, L.C - gk0);
   const V* __restrict__ cfs = a.coef;
   const int64_t cst = a.cstride;
-  const int st[3] = {1, BX, BX * BX};
+  const int st[3] = {1, kRbBX, kRbPK};
   // ---- the tile's couplings, once for the NW systems
   for (int p = tid; p < NP; p += 64 * NW) {
     const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+    const int sl = shp->rslot[p];
     Cpl c, h;
 #pragma unroll
     for (int s = 0; s < 3; ++s) c.c[s] = h.c[s] = V(0);
@@ -927,8 +956,8 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
         if (BWDP) h.c[s] = cfs[(int64_t)(2 + 2 * s) * cst + e];
       }
     }
-    if (FWDP) CL[p] = c;
-    if (BWDP) CH[p] = h;
+    if (FWDP) CL[sl] = c;
+    if (BWDP) CH[sl] = h;
   }
   __syncthreads();   // s_ tables
   // this lane's point on every hyperplane of the tile (at most 48 per plane),
@@ -943,7 +972,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
 #pragma unroll
     for (int h = 0; h < NLEV; ++h) {
       const int idx = lv[h] + lane;
-      const unsigned int v = shp->pts[min(idx, kMaxTilePts - 1)];
+      const unsigned int v = shp->rpts[min(idx, NP - 1)];
       pk[h] = idx < lv[h + 1] ? v : 0xffffffffu;
     }
   }
@@ -969,10 +998,12 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
     const C2<T>* __restrict__ dinv = a.dinv + (int64_t)s_fidx[wv] * cst;
     const C2<T> al = s_al[wv];
     C2<T> rv[NU], v2[NU], dd[NU];
+    int rs[NU];
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
       const int p = lane + 64 * u;
       const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+      rs[u] = shp->rslot[p];
       rv[u] = v2[u] = dd[u] = {T(0), T(0)};
       if (li < ex && lj < ey && lk < ez) {
         const int64_t e = lat_e(L, gi0 + li, gj0 + lj, gk0 + lk);
@@ -1020,11 +1051,10 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
     for (int u = 0; u < NU; ++u) {
       const int p = lane + 64 * u;
       const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
-      const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
       C2<T> r = rv[u];
       if (FWDP && pin2) r = csub(r, cmul(al, v2[u]));
-      W[q] = r;
-      DV[p] = dd[u];
+      W[rb_box(li, lj, lk)] = r;
+      DV[rs[u]] = dd[u];
     }
     if (HALO) {
       const int fa = lane & 7, fb = lane >> 3;
@@ -1044,7 +1074,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
           continue;
         }
         if (!(ti < ex && tj < ey && tk < ez)) continue;
-        const int qp = (ti + 1) + BX * ((tj + 1) + BX * (tk + 1));
+        const int qp = rb_box(ti, tj, tk);
         int qs = hi ? qp + st[dir] : qp - st[dir];
         if (qs >= NB) qs -= NB;
         W[qs] = v;
@@ -1066,9 +1096,8 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
       else if (s == 1) { ti = fa; tj = t0; tk = fb; }
       else { ti = fa; tj = fb; tk = t0; }
       if (ti < ex && tj < ey && tk < ez) {
-        const int p = ti + TX * (tj + TX * tk);
-        const int q = (ti + 1) + BX * ((tj + 1) + BX * (tk + 1));
-        const T c = (MODE == RB_BLACK_FB) ? T(CH[p].c[s]) : fold_c[s];
+        const int q = rb_box(ti, tj, tk);
+        const T c = (MODE == RB_BLACK_FB) ? T(CH[shp->rslot[ti + TX * (tj + TX * tk)]].c[s]) : fold_c[s];
         const C2<T> v = fold_v[s];
         C2<T> r = W[q];
         r.x -= c * v.x;
@@ -1082,10 +1111,10 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
 #pragma unroll
     for (int h = 0; h < NLEV; ++h) {
       const unsigned int pt = pk[h];
-      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+      const int li = pt & 7, lj = (pt >> 3) & 7, lk = (pt >> 6) & 7;
       if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
-        const int p = li + TX * (lj + TX * lk);
-        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        const int p = pt >> 9;   // level slot
+        const int q = rb_box(li, lj, lk);
         const C2<T> dvp = DV[p];
         const Cpl c4 = CL[p];
         // red: face neighbours are upper, so only in-tile lower couplings;
@@ -1109,10 +1138,10 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
 #pragma unroll
     for (int h = NLEV - 1; h >= 0; --h) {
       const unsigned int pt = pk[h];
-      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = (pt >> 16) & 0xff;
+      const int li = pt & 7, lj = (pt >> 3) & 7, lk = (pt >> 6) & 7;
       if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
-        const int p = li + TX * (lj + TX * lk);
-        const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
+        const int p = pt >> 9;   // level slot
+        const int q = rb_box(li, lj, lk);
         const C2<T> dvp = DV[p];
         const Cpl h4 = CH[p];
         // black: upper neighbours in the tile only; red: in tile or across
@@ -1148,8 +1177,8 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_rbl(DiluArgs<T, V> a, const Ti
   for (int p = lane; p < NP; p += 64) {
     const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
     if (!(li < ex && lj < ey && lk < ez)) continue;
-    const int q = (li + 1) + BX * ((lj + 1) + BX * (lk + 1));
-    const C2<T> zv = (MODE == RB_RED_FWD) ? DV[p] : W[q];
+    const int q = rb_box(li, lj, lk);
+    const C2<T> zv = (MODE == RB_RED_FWD) ? DV[shp->rslot[p]] : W[q];
     dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = zv;
     if (MODE != RB_RED_BWD && (li == 0 || li == TX - 1))
       xfo[(li == 0 ? 0 : 64) + lj + 8 * lk] = W[q];   // red forward: dinv*y; black: z

@@ -189,6 +189,13 @@ __global__ __launch_bounds__(BS) void k_finalize(SysScal* scs, const int* act, i
   }
 }
 
+// y = d .* x over n complex entries (the Jacobi preconditioner apply)
+__global__ __launch_bounds__(BS) void k_diag_mul(const Z* __restrict__ d, const Z* __restrict__ x,
+                                                 Z* __restrict__ y, int64_t n) {
+  for (int64_t q = blockIdx.x * int64_t(BS) + threadIdx.x; q < n; q += int64_t(gridDim.x) * BS)
+    y[q] = zmul(d[q], x[q]);
+}
+
 // ------------------------------------------------------------------ kernels
 // r = b - A x, partials <r,r>, <b,b> (then FIN_INIT); xbest = x0.
 template <class T, class V>

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "mt_bc2d.hpp"
+#include "mt_brick.hpp"
 #include "mt_cond.hpp"
 #include "mt_datafunc.hpp"
 #include "mt_dilu.hpp"
@@ -98,6 +99,26 @@ static void free_sites(SiteData& s) {
   s = SiteData();
 }
 
+// Bricks of the mixed solver's inner solves (red-black DILU): the tile
+// geometry, the brick -> all-edge map and the brick-ordered FP32 couplings of
+// A and A^T (row-major for the SpMV, slots 1..6 for the sweeps).
+struct BrickData {
+  bool ready = false;
+  BrickGeom g{};
+  int* bmap = nullptr;                   // [nB] all-edge index, -1 past a partial tile
+  float* rows[2] = {nullptr, nullptr};   // [nB][16] of A, A^T
+  float* slots[2] = {nullptr, nullptr};  // [6][nB] of A, A^T
+};
+
+static void free_brick(BrickData& b) {
+  hipFree(b.bmap);
+  for (int q = 0; q < 2; ++q) {
+    hipFree(b.rows[q]);
+    hipFree(b.slots[q]);
+  }
+  b = BrickData();
+}
+
 struct mt_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -106,6 +127,8 @@ struct mt_ctx {
   DeviceOperator dop;
   TilePlan tiles;
   TilePlan tiles_rb;  // red-black tile ordering of the same lattices
+  BrickData bk;       // bricks of the mixed solver's inner solves
+  bool brick = true;  // inner solves on bricks (mixed, red-black DILU, 4-system SpMV)
   DivcorData dc;
   SiteData sites;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
@@ -274,6 +297,38 @@ static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInf
           if (li + lj + lk == h) sh.pts[w++] = unsigned(li) | (unsigned(lj) << 8) | (unsigned(lk) << 16);
   }
   sh.lvl[sh.nlev] = w;
+  if (tp.tx == 8 && tp.ty == 8 && tp.tz == 8) {
+    // lane order of the lean red-black sweeps: fill each 16-lane group of a
+    // level with points whose box slots differ modulo 16 (8-byte bank pairs)
+    // while such points remain, then with the rest in level order
+    for (int h = 0; h < sh.nlev; ++h) {
+      std::vector<unsigned int> rem(sh.pts + sh.lvl[h], sh.pts + sh.lvl[h + 1]);
+      int slot = sh.lvl[h];
+      auto put = [&](unsigned int v) {
+        const int li = v & 0xff, lj = (v >> 8) & 0xff, lk = (v >> 16) & 0xff;
+        sh.rpts[slot] = unsigned(li) | (unsigned(lj) << 3) | (unsigned(lk) << 6) | (unsigned(slot) << 9);
+        sh.rslot[li + 8 * (lj + 8 * lk)] = (unsigned short)slot;
+        ++slot;
+      };
+      while (!rem.empty()) {
+        bool used[16] = {};
+        int n = 0;
+        for (size_t r = 0; r < rem.size() && n < 16;) {
+          const unsigned int v = rem[r];
+          const int b = rb_box(v & 0xff, (v >> 8) & 0xff, (v >> 16) & 0xff) & 15;
+          if (used[b]) { ++r; continue; }
+          used[b] = true;
+          put(v);
+          rem.erase(rem.begin() + r);
+          ++n;
+        }
+        for (; n < 16 && !rem.empty(); ++n) {
+          put(rem.front());
+          rem.erase(rem.begin());
+        }
+      }
+    }
+  }
   // the red-black sweeps give each lane one point per hyperplane
   for (int h = 0; h < sh.nlev; ++h)
     if (sh.lvl[h + 1] - sh.lvl[h] > 64)
@@ -313,6 +368,7 @@ static int upload_operator(mt_ctx* ctx) {
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   ctx->dc = DivcorData();
+  free_brick(ctx->bk);
   DeviceOperator& d = ctx->dop;
   const size_t nc = size_t(kSlots) * g.nE;
   std::vector<float> tmp(nc);
@@ -388,6 +444,60 @@ static int ensure_transpose(mt_ctx* ctx) {
                                                           ctx->dop.rowT32);
   MT_HIP_CHECK(ctx, hipGetLastError());
   MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MT_OK;
+}
+
+// The bricks of the inner solves (geometry and map once per grid, the
+// couplings of A or A^T once per operator).
+static int ensure_brick(mt_ctx* ctx, bool transpose) {
+  BrickData& B = ctx->bk;
+  if (!B.ready) {
+    const TilePlan& tp = ctx->tiles_rb;
+    if (tp.tx != 8 || tp.ty != 8 || tp.tz != 8 || tp.nlat != 3)
+      return mt_fail(ctx, MT_E_STATE, "bricks need the 8x8x8 red-black tile plan");
+    if (int64_t(tp.ntall) * kBkPts * 16 >= (int64_t(1) << 31))
+      return mt_fail(ctx, MT_E_ARG, "grid too large for 32-bit brick indices");
+    BrickGeom& G = B.g;
+    for (int c = 0; c < 3; ++c) {
+      const LatticeInfo& L = tp.lat[c];
+      G.A[c] = L.A; G.B[c] = L.B; G.C[c] = L.C;
+      G.oi[c] = L.oi; G.oj[c] = L.oj; G.ok[c] = L.ok;
+      G.ntx[c] = L.ntx; G.nty[c] = L.nty; G.ntz[c] = (L.C + 7) / 8;
+      G.tbase[c] = L.tbase;
+    }
+    G.tbase[3] = tp.ntall;
+    G.nB = int64_t(tp.ntall) * kBkPts;
+    std::vector<int> bm(size_t(G.nB), -1);
+    for (int c = 0; c < 3; ++c) {
+      const LatticeInfo& L = tp.lat[c];
+      for (int tk = 0; tk < G.ntz[c]; ++tk)
+        for (int tj = 0; tj < G.nty[c]; ++tj)
+          for (int ti = 0; ti < G.ntx[c]; ++ti) {
+            const int64_t t = L.tbase + ti + int64_t(L.ntx) * (tj + int64_t(L.nty) * tk);
+            for (int p = 0; p < kBkPts; ++p) {
+              const int i = ti * 8 + (p & 7), j = tj * 8 + ((p >> 3) & 7), k = tk * 8 + (p >> 6);
+              if (i >= L.A || j >= L.B || k >= L.C) continue;
+              bm[size_t(t * kBkPts + p)] =
+                  int(L.base + (i + L.oi) + int64_t(L.a) * ((j + L.oj) + int64_t(L.b) * (k + L.ok)));
+            }
+          }
+    }
+    MT_HIP_CHECK(ctx, hipMalloc(&B.bmap, bm.size() * sizeof(int)));
+    MT_HIP_CHECK(ctx, hipMemcpy(B.bmap, bm.data(), bm.size() * sizeof(int), hipMemcpyHostToDevice));
+    B.ready = true;
+  }
+  const int q = transpose ? 1 : 0;
+  if (!B.rows[q]) {
+    if (transpose) { int rc = ensure_transpose(ctx); if (rc) return rc; }
+    const StencilView<float> A = view<float>(ctx, transpose);
+    const int64_t nB = B.g.nB;
+    MT_HIP_CHECK(ctx, hipMalloc(&B.rows[q], size_t(nB) * 16 * sizeof(float)));
+    MT_HIP_CHECK(ctx, hipMalloc(&B.slots[q], size_t(nB) * 6 * sizeof(float)));
+    k_bk_rows<float><<<4096, BS, 0, ctx->stream>>>(B.bmap, nB, A.rowc, B.rows[q]);
+    k_bk_slots<float><<<2048, BS, 0, ctx->stream>>>(B.bmap, nB, A.nE, A.coef, B.slots[q]);
+    MT_HIP_CHECK(ctx, hipGetLastError());
+    MT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  }
   return MT_OK;
 }
 
@@ -538,10 +648,33 @@ static void dilu_rb_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
 
 // Packed i-face buffer of the red-black sweeps for systems 0..nsys-1
 // (DiluArgs::xf); sets a.xf / a.xfsys / a.xfhalf.
+// The red-black sweeps on bricks (k_dilu_bk): the three launches of
+// dilu_rb_apply, four systems of one tile per workgroup.
+template <class T, class V>
+static void dilu_bk_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
+  if (!a.xf) {
+    ctx->err = "red-black DILU launched without its face buffer";
+    return;
+  }
+  constexpr int NW = 4;
+  for (int step = 0; step < 3; ++step) {
+    const int L = (step == 1) ? 1 : 0;
+    if (L >= tp.nlev) continue;
+    a.tiles = tp.tiles + tp.lvl_off[L];
+    a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
+    if (a.ntiles == 0) continue;
+    const int64_t nwork = int64_t(a.ntiles) * ((a.nact + NW - 1) / NW);
+    const int g = int(8 * ((nwork + 7) / 8));
+    if (step == 0) k_dilu_bk<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+    else if (step == 1) k_dilu_bk<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+    else k_dilu_bk<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+  }
+}
+
 template <class T, class V>
 static int rb_faces(mt_ctx* ctx, int nsys, DiluArgs<T, V>& a) {
   const TilePlan& tp = ctx->tiles_rb;
-  a.xfsys = int64_t(tp.ntall) * 128;
+  a.xfsys = int64_t(tp.ntall) * kRbFaces;
   a.xfhalf = int64_t(nsys) * a.xfsys;
   return ws(ctx, sizeof(T) == 8 ? "rb_faces64" : "rb_faces32", size_t(2 * a.xfhalf), &a.xf);
 }
@@ -612,6 +745,12 @@ struct Engine {
   int ms = 0;          // >0: the multi-system row SpMV with this many systems per block
   C2<T>* xf = nullptr;  // red-black DILU packed faces (rb_faces)
   int64_t xfsys = 0, xfhalf = 0;
+  // inner solves on bricks: the geometry, the brick-ordered couplings (rows
+  // for the SpMV, slots 1..6 for the sweeps); a.nE is then the brick length
+  bool brick = false;
+  BrickGeom bg{};
+  const V* bk_rows = nullptr;
+  const V* bk_slots = nullptr;
   // timing: event pairs (family, first, second) recorded by one chunk; with a
   // graph they are recorded by the graph's event nodes on every replay
   struct TPair { int fam, e0, e1; };
@@ -636,26 +775,32 @@ struct Engine {
     if (a.pc != PCK_DILU && a.pc != PCK_DILU_RB) return;
     const Grid& g = ctx->hop.g;
     const TilePlan& tp = a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles;
-    DiluArgs<T, V> d = dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE,
-                                       a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN);
+    DiluArgs<T, V> d =
+        brick ? dilu_args<T, V>(tp, bk_slots, a.nE, a.nE, a.act, a.nact, a.sc, in, out, a.dinv, ST_RUN)
+              : dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE, a.act, a.nact, a.sc,
+                                in, out, a.dinv, ST_RUN);
     d.in2 = in2;
     d.xf = xf;
     d.xfsys = xfsys;
     d.xfhalf = xfhalf;
-    if (!ctx->timing) {
-      apply_pc(d);
-      return;
-    }
-    const int e0 = trec();
-    apply_pc(d);
-    tpairs.push_back({MT_KT_PRECOND, e0, trec()});
+    const int e0 = ctx->timing ? trec() : -1;
+    if (brick) dilu_bk_apply<T, V>(ctx, tp, d);
+    else apply_pc(d);
+    if (ctx->timing) tpairs.push_back({MT_KT_PRECOND, e0, trec()});
   }
   // the SpMV and its dot partials; returns the partial count per system
   template <int WHICH>
   int spmv(int nblk) {
     const int e0 = ctx->timing ? trec() : -1;
     int G;
-    if (ms > 0) {
+    if (brick) {
+      // one block per chunk of tiles for all systems: at most 2048 chunks
+      // (the partials buffer), a multiple of 8 (XCDs)
+      IterArgs<T, V> b = a;
+      b.G = std::min(2048, (bg.tbase[3] + 7) / 8 * 8);
+      k_spmv_bt<T, V, WHICH><<<b.G, kBtThreads, 0, ctx->stream>>>(b, bg, bk_rows);
+      G = b.G;
+    } else if (ms > 0) {
       const int nb = ((a.nact + ms - 1) / ms) * a.G;
       if (ms == 4) k_spmv_ms<T, V, WHICH, 4><<<nb, BS, 0, ctx->stream>>>(a);
       else k_spmv_ms<T, V, WHICH, 8><<<nb, BS, 0, ctx->stream>>>(a);
@@ -955,14 +1100,34 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   IterArgs<float, float>& a = E.a;
   a.A = view<float>(ctx, tr);
   a.pc = o.precond;
-  a.nE = nE;
   int rc = 0;
+  // inner solves on bricks: the red-black DILU with the 4-system SpMV
+  const bool bricks = ctx->brick && o.precond == PCK_DILU_RB && ctx->spmv_variant == 3 &&
+                      ctx->ms_S == 4;
+  if (bricks && (rc = ensure_brick(ctx, tr))) return rc;
+  const int64_t nI = bricks ? ctx->bk.g.nB : nE;   // inner vector stride
+  const size_t nvi = bricks ? size_t(nI) * nsys : nv;
+  a.nE = nI;
+  E.brick = bricks;
+  if (bricks) {
+    E.bg = ctx->bk.g;
+    E.bk_rows = ctx->bk.rows[tr ? 1 : 0];
+    E.bk_slots = ctx->bk.slots[tr ? 1 : 0];
+  }
+  const char* nm[9] = {"f32_x", "f32_r", "f32_rt", "f32_p", "f32_ph", "f32_v", "f32_s", "f32_sh", "f32_t"};
+  const char* nmb[9] = {"bk_x", "bk_r", "bk_rt", "bk_p", "bk_ph", "bk_v", "bk_s", "bk_sh", "bk_t"};
+  const char* const* n9 = bricks ? nmb : nm;
   F* d32;
-  if ((rc = ws(ctx, "f32_x", nv, &d32)) || (rc = ws(ctx, "f32_r", nv, &a.r)) ||
-      (rc = ws(ctx, "f32_rt", nv, &a.rt)) || (rc = ws(ctx, "f32_p", nv, &a.p)) ||
-      (rc = ws(ctx, "f32_ph", nv, &a.ph)) || (rc = ws(ctx, "f32_v", nv, &a.v)) ||
-      (rc = ws(ctx, "f32_s", nv, &a.s)) || (rc = ws(ctx, "f32_sh", nv, &a.sh)) ||
-      (rc = ws(ctx, "f32_t", nv, &a.t)))
+  if ((rc = ws(ctx, n9[0], nvi, &d32)) || (rc = ws(ctx, n9[1], nvi, &a.r)) ||
+      (rc = ws(ctx, n9[2], nvi, &a.rt)) || (rc = ws(ctx, n9[3], nvi, &a.p)) ||
+      (rc = ws(ctx, n9[4], nvi, &a.ph)) || (rc = ws(ctx, n9[5], nvi, &a.v)) ||
+      (rc = ws(ctx, n9[6], nvi, &a.s)) || (rc = ws(ctx, n9[7], nvi, &a.sh)) ||
+      (rc = ws(ctx, n9[8], nvi, &a.t)))
+    return rc;
+  // the outer correction's FP32 residual and correction in the all-edge
+  // layout (permuted to and from the bricks around each inner solve)
+  F *r_nat = a.r, *d_nat = d32;
+  if (bricks && ((rc = ws(ctx, "f32_r", nv, &r_nat)) || (rc = ws(ctx, "f32_x", nv, &d_nat))))
     return rc;
   a.x = d32;
   a.xbest = nullptr;
@@ -970,6 +1135,16 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   a.fused = (o.precond == PCK_DILU || o.precond == PCK_DILU_RB) ? 1 : 0;
   C2<float>* dinv = nullptr;
   if ((rc = build_dinv<float>(ctx, o.precond, tr, freqs, &dinv))) return rc;
+  if (bricks) {
+    // the pivots of every frequency in brick order
+    const int64_t nB = ctx->bk.g.nB;
+    C2<float>* db;
+    if ((rc = ws(ctx, "bk_dinv", size_t(nB) * freqs.size(), &db))) return rc;
+    k_bk_gather<float><<<4096, BS, 0, ctx->stream>>>(ctx->bk.bmap, nB, int(freqs.size()), nE, nB,
+                                                    dinv, db);
+    MT_HIP_CHECK(ctx, hipGetLastError());
+    dinv = db;
+  }
   a.dinv = dinv;
   if (o.precond == PCK_DILU_RB) {
     DiluArgs<float, float> fa;
@@ -999,8 +1174,8 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   oa.nE = nE;
   oa.b = b;
   oa.x = x;
-  oa.r32 = a.r;
-  oa.d32 = d32;
+  oa.r32 = r_nat;
+  oa.d32 = d_nat;
   oa.partials = a.partials;
   oa.tol = o.tol;
   oa.inner_tol = o.inner_tol > 0 ? o.inner_tol : 3e-4;
@@ -1056,6 +1231,10 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, iact.data(), iact.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     a.nact = int(iact.size());
     a.G = pick_G(ctx, a.nact);
+    if (bricks) {
+      k_bk_to_brick<<<4096, BS, 0, ctx->stream>>>(d_act, a.nact, ctx->bk.bmap, nI, nE, r_nat, a.r, a.x);
+      MT_HIP_CHECK(ctx, hipGetLastError());
+    }
     while (true) {
       if ((rc = E.run_chunk(chunk))) return rc;
       if ((rc = read_scal(ctx, d_si, nsys, hi))) return rc;
@@ -1069,6 +1248,10 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     act = outer_act;
     oa.nact = int(outer_act.size());
     oa.G = pick_G(ctx, oa.nact);
+    if (bricks) {
+      k_bk_from_brick<<<4096, BS, 0, ctx->stream>>>(d_act, oa.nact, ctx->bk.bmap, nI, nE, a.x, d_nat);
+      MT_HIP_CHECK(ctx, hipGetLastError());
+    }
     k_outer_update<<<oa.nact * oa.G, BS, 0, ctx->stream>>>(oa);
     MT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = read_scal(ctx, d_so, nsys, ho))) return rc;
@@ -1197,6 +1380,7 @@ int mt_ctx_destroy(mt_ctx* ctx) {
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   free_sites(ctx->sites);
+  free_brick(ctx->bk);
   for (auto& e : ctx->ev) if (e) hipEventDestroy(e);
   for (auto& e : ctx->tev) hipEventDestroy(e);
   hipStreamDestroy(ctx->stream);
@@ -1475,12 +1659,7 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
     if ((rc = rb_faces(ctx, 1, d))) return rc;
     dilu_rb_apply<double, double>(ctx, ctx->tiles_rb, d);
   } else if (precond == MT_PC_JACOBI) {
-    // y = dinv .* x via the p-update kernel path is overkill; do it on the host view
-    std::vector<Z> hd(g.nE), hx(g.nE), hy(g.nE);
-    MT_HIP_CHECK(ctx, hipMemcpy(hd.data(), dinv, sizeof(Z) * g.nE, hipMemcpyDeviceToHost));
-    MT_HIP_CHECK(ctx, hipMemcpy(hx.data(), dx, sizeof(Z) * g.nE, hipMemcpyDeviceToHost));
-    for (int64_t e = 0; e < g.nE; ++e) hy[e] = zmul(hd[e], hx[e]);
-    MT_HIP_CHECK(ctx, hipMemcpy(dy, hy.data(), sizeof(Z) * g.nE, hipMemcpyHostToDevice));
+    k_diag_mul<<<2048, BS, 0, ctx->stream>>>(dinv, dx, dy, g.nE);
   } else {
     MT_HIP_CHECK(ctx, hipMemcpyAsync(dy, dx, sizeof(Z) * g.nE, hipMemcpyDeviceToDevice, ctx->stream));
   }
@@ -1488,6 +1667,78 @@ int mt_precond_apply(mt_ctx* ctx, double omega, int precond, const mt_zdouble* x
   return download_padded(ctx, 1, dy, y, "io_tmp");
 }
 
+
+// The inner solves' brick kernels on one vector: y = A x (what 0, k_spmv_bt)
+// or y = M^-1 x (what 1, k_dilu_bk with the red-black DILU of A(omega)), of A
+// or of A^T, in FP32 on bricks; x and y in EDGEi order, FP64.
+int mt_inner_apply(mt_ctx* ctx, double omega, int what, int transpose, const mt_zdouble* x,
+                   mt_zdouble* y) {
+  int rc = need_op(ctx);
+  if (rc) return rc;
+  if (!x || !y || (what != 0 && what != 1) || (transpose != 0 && transpose != 1))
+    return mt_fail(ctx, MT_E_ARG, "mt_inner_apply: bad arguments");
+  hipSetDevice(ctx->device);
+  const bool tr = transpose != 0;
+  if ((rc = ensure_brick(ctx, tr))) return rc;
+  const Grid& g = ctx->hop.g;
+  const int64_t nB = ctx->bk.g.nB, nE = g.nE;
+  Z *dx, *dy;
+  C2<float> *xn, *yn, *xb, *yb, *zb;
+  if ((rc = ws(ctx, "ap_x", g.vlen(1), &dx)) || (rc = ws(ctx, "ap_y", g.vlen(1), &dy)) ||
+      (rc = ws(ctx, "ia_xn", nE, &xn)) || (rc = ws(ctx, "ia_yn", nE, &yn)) ||
+      (rc = ws(ctx, "ia_xb", nB, &xb)) || (rc = ws(ctx, "ia_yb", nB, &yb)) ||
+      (rc = ws(ctx, "ia_zb", nB, &zb)))
+    return rc;
+  if ((rc = upload_padded(ctx, 1, x, dx, "io_tmp"))) return rc;
+  SysScal h;
+  std::memset(&h, 0, sizeof(h));
+  h.status = ST_RUN;
+  h.freq_omega = omega;
+  SysScal* d_sc;
+  int* d_act;
+  if ((rc = ws(ctx, "pc_sc", 1, &d_sc)) || (rc = ws(ctx, "pc_act", 1, &d_act))) return rc;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(d_sc, &h, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemsetAsync(d_act, 0, sizeof(int), ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemsetAsync(yb, 0, sizeof(C2<float>) * nB, ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemsetAsync(yn, 0, sizeof(C2<float>) * nE, ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemsetAsync(zb, 0, sizeof(C2<float>) * nB, ctx->stream));
+  k_bk_narrow<<<2048, BS, 0, ctx->stream>>>(dx, xn, nE);
+  k_bk_gather<float><<<2048, BS, 0, ctx->stream>>>(ctx->bk.bmap, nB, 1, nE, nB, xn, xb);
+  if (what == 0) {
+    IterArgs<float, float> a{};
+    a.A = view<float>(ctx, tr);
+    a.sc = d_sc;
+    a.act = d_act;
+    a.nact = 1;
+    a.nE = nB;
+    a.ph = xb;
+    a.v = yb;
+    a.rt = zb;
+    Z* part;
+    a.G = std::min(2048, (ctx->bk.g.tbase[3] + 7) / 8 * 8);
+    if ((rc = ws(ctx, "ia_part", size_t(a.G) * 2, &part))) return rc;
+    a.partials = part;
+    k_spmv_bt<float, float, 1><<<a.G, kBtThreads, 0, ctx->stream>>>(a, ctx->bk.g, ctx->bk.rows[tr ? 1 : 0]);
+  } else {
+    std::vector<double> freqs(1, omega);
+    C2<float>* dinv;
+    if ((rc = build_dinv<float>(ctx, PCK_DILU_RB, tr, freqs, &dinv))) return rc;
+    C2<float>* db;
+    if ((rc = ws(ctx, "ia_dinv", nB, &db))) return rc;
+    k_bk_gather<float><<<2048, BS, 0, ctx->stream>>>(ctx->bk.bmap, nB, 1, nE, nB, dinv, db);
+    DiluArgs<float, float> d = dilu_args<float, float>(ctx->tiles_rb, ctx->bk.slots[tr ? 1 : 0], nB, nB,
+                                                       d_act, 1, d_sc, xb, yb, db, ST_RUN);
+    if ((rc = rb_faces(ctx, 1, d))) return rc;
+    dilu_bk_apply<float, float>(ctx, ctx->tiles_rb, d);
+  }
+  k_bk_from_brick<<<2048, BS, 0, ctx->stream>>>(d_act, 1, ctx->bk.bmap, nB, nE, yb, yn);
+  MT_HIP_CHECK(ctx, hipMemsetAsync(dy, 0, sizeof(Z) * g.nE, ctx->stream));
+  k_bk_widen<<<2048, BS, 0, ctx->stream>>>(yn, dy, nE);
+  MT_HIP_CHECK(ctx, hipGetLastError());
+  if (!ctx->err.empty() && ctx->err.find("face buffer") != std::string::npos)
+    return mt_fail(ctx, MT_E_STATE, ctx->err);
+  return download_padded(ctx, 1, dy, y, "io_tmp");
+}
 
 int mt_rhs_fwd(mt_ctx* ctx, const mt_zdouble* e_all, mt_zdouble* b) {
   if (!ctx || !e_all || !b) return mt_fail(ctx, MT_E_ARG, "mt_rhs_fwd: null argument");
@@ -1995,6 +2246,13 @@ int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
   return MT_OK;
 }
 
+int mt_set_inner_layout(mt_ctx* ctx, int bricks) {
+  if (!ctx || (bricks != 0 && bricks != 1))
+    return mt_fail(ctx, MT_E_ARG, "mt_set_inner_layout: bricks is 0 or 1");
+  ctx->brick = bricks != 0;
+  return MT_OK;
+}
+
 int mt_set_spmv_variant(mt_ctx* ctx, int variant, int layers_per_segment) {
   if (!ctx || variant < 0 || variant > 3 || layers_per_segment < 0 ||
       (variant == 3 && layers_per_segment != 0 && layers_per_segment != 4 && layers_per_segment != 8))
@@ -2307,6 +2565,7 @@ int mt_update_cond(mt_ctx* ctx, const double* logcond, double air_logcond) {
   // what was derived from the previous model
   hipFree(d.coefT64); hipFree(d.coefT32); hipFree(d.rowT64); hipFree(d.rowT32);
   d.coefT64 = nullptr; d.coefT32 = nullptr; d.rowT64 = nullptr; d.rowT32 = nullptr;
+  free_brick(ctx->bk);
   hipFree(ctx->dc.coefN); hipFree(ctx->dc.wE); hipFree(ctx->dc.invL); hipFree(ctx->dc.dinv);
   hipFree(ctx->dc.tiles.tiles); hipFree(ctx->dc.tiles.shape);
   ctx->dc = DivcorData();

@@ -101,6 +101,8 @@ _SIGS = {
     "mt_timing_enable": (_I, [_P, _I]),
     "mt_set_spmv_variant": (_I, [_P, _I, _I]),
     "mt_set_dilu_variant": (_I, [_P, _I]),
+    "mt_set_inner_layout": (_I, [_P, _I]),
+    "mt_inner_apply": (_I, [_P, _D, _I, _I, _PZ, _PZ]),
     "mt_bench_precond": (_I, [_P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
     "mt_receivers_set": (_I, [_P, _I, _I, _I, _I, _PD, _PD, _PD, _P, _I, _PD, _P, _P]),
@@ -349,6 +351,14 @@ class Context:
                                          np.ascontiguousarray(x, np.complex128), y))
         return y
 
+    def inner_apply(self, omega, x, what=0, transpose=False):
+        """The mixed solver's brick kernels on one vector, FP32: y = A x
+        (what 0) or y = M^-1 x (what 1, red-black DILU), of A or A^T."""
+        y = np.zeros(self.n_int, np.complex128)
+        self._chk(lib().mt_inner_apply(self._h, float(omega), int(what), int(bool(transpose)),
+                                       np.ascontiguousarray(x, np.complex128), y))
+        return y
+
     def rhs_fwd(self, e_all):
         b = np.zeros(self.n_int, np.complex128)
         self._chk(lib().mt_rhs_fwd(self._h, np.ascontiguousarray(e_all, np.complex128), b))
@@ -458,6 +468,11 @@ class Context:
         single-wave workgroup, 4 four systems per workgroup, 5 (default) four
         systems per workgroup in the lean LDS layout."""
         self._chk(lib().mt_set_dilu_variant(self._h, int(variant)))
+
+    def set_inner_layout(self, bricks):
+        """Inner FP32 vectors of the mixed solver (red-black DILU, default
+        SpMV) on bricks (True, default) or in the all-edge layout (False)."""
+        self._chk(lib().mt_set_inner_layout(self._h, int(bool(bricks))))
 
     def set_tuning(self, groups):
         self._chk(lib().mt_set_tuning(self._h, int(groups)))
