@@ -332,12 +332,17 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
 }
 
 // ------------------------------------------------- red-black DILU on bricks
-// k_dilu_rbl's sweeps (the same LDS layout, level order and sums) on bricked
-// data: a tile's right-hand sides, pivots and the six couplings it uses are
-// contiguous runs, and the faces a tile reads from its neighbours of the
-// other colour come packed (six faces of 64 per tile, DiluArgs::xf), as the
-// colour that produced them wrote them: dinv*y of the red tiles for the black
-// sweep, z of the black tiles for the red backward sweep.
+// k_dilu_rbl's sweeps (the same LDS layout and level order) on bricked data:
+// a tile's right-hand sides, pivots and the six couplings it uses are
+// contiguous runs.  The red forward sweep leaves w = dinv*y of the red tiles
+// in the output (k_dilu_rbl leaves y): that is what the black tiles take from
+// their red neighbours, and the red backward sweep forms z = w - dinv*(sum of
+// the upper terms), as the black backward sweep does (the same operator as
+// dinv*(y - sum), rounded differently).  A tile reads its neighbours' k- and
+// j-faces from their bricks (a k-face is 512 contiguous bytes, a j-face eight
+// runs of 64) and their i-faces, one value per 64 bytes there, from the
+// packed i-faces (DiluArgs::xf) the other colour wrote: w of the red tiles,
+// z of the black tiles.
 // DiluArgs on bricks: coef = the [6][nB] couplings (slots 1..6), cstride =
 // nB, vstride = nB, dinv = [nfreq][nB].
 template <class T, class V, int MODE, int NW>
@@ -419,6 +424,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
   C2<T>* __restrict__ W = Wall[wv];
   C2<T>* __restrict__ DV = DVall[wv];
   const int64_t o = (int64_t)sys * a.vstride;
+  const C2<T>* __restrict__ pout = a.out + o;   // the other colour's results
   const C2<T>* __restrict__ xfi =
       a.xf + (MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys;   // faces read
   C2<T>* __restrict__ xfw =
@@ -465,11 +471,18 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
         const bool pin_ok = ti < ex && tj < ey && tk < ez;
         const bool ok = pin_ok && ni >= 0 && nj >= 0 && nk >= 0 && ni < L.A && nj < L.B && nk < L.C;
         if (ok) {
-          // the neighbour tile's facing face, packed (entry = lane)
+          // the neighbour tile's facing face: the i-faces packed (entry
+          // tj + 8*tk = lane), the j- and k-faces from the neighbour's brick
           const int d = hi ? 1 : -1;
           const int tn = L.tbase + (tl.y + (dir == 0 ? d : 0)) +
                          L.ntx * ((tl.z + (dir == 1 ? d : 0)) + L.nty * (tl.w + (dir == 2 ? d : 0)));
-          fv[f] = xfi[(int64_t)tn * kRbFaces + (2 * dir + (hi ? 0 : 1)) * 64 + lane];
+          if (dir == 0) {
+            fv[f] = xfi[(int64_t)tn * kRbFaces + (hi ? 0 : 1) * 64 + lane];
+          } else {
+            const int fl = hi ? 0 : TX - 1;   // the neighbour's facing layer
+            const int pn = dir == 1 ? ti + TX * (fl + TX * tk) : ti + TX * (tj + TX * fl);
+            fv[f] = pout[(int64_t)tn * kBkPts + pn];
+          }
         }
         if (MODE == RB_RED_BWD && !hi && pin_ok)
           fold_c[dir] = T(cfs[(int64_t)(2 * dir) * cst + tb + ti + TX * (tj + TX * tk)]);
@@ -525,12 +538,19 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
       else { ti = fa; tj = fb; tk = t0; }
       if (ti < ex && tj < ey && tk < ez) {
         const int q = rb_box(ti, tj, tk);
-        const T c = (MODE == RB_BLACK_FB) ? T(CH[shp->rslot[ti + TX * (tj + TX * tk)]].c[s])
-                                          : fold_c[s];
+        const int sl = shp->rslot[ti + TX * (tj + TX * tk)];
+        const T c = (MODE == RB_BLACK_FB) ? T(CH[sl].c[s]) : fold_c[s];
         const C2<T> v = fold_v[s];
         C2<T> r = W[q];
-        r.x -= c * v.x;
-        r.y -= c * v.y;
+        if (MODE == RB_BLACK_FB) {
+          r.x -= c * v.x;
+          r.y -= c * v.y;
+        } else {
+          // the box holds w = dinv*y: subtract dinv*(c v)
+          const C2<T> d = cmul(DV[sl], C2<T>{c * v.x, c * v.y});
+          r.x -= d.x;
+          r.y -= d.y;
+        }
         W[q] = r;
       }
       wave_sync();
@@ -558,7 +578,6 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
           y.y -= c * v.y;
         }
         W[q] = cmul(dvp, y);
-        if (MODE == RB_RED_FWD) DV[p] = y;
       }
       wave_sync();
     }
@@ -575,7 +594,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
         const Cpl h4 = CH[p];
         // black: upper neighbours in the tile only; red: in tile or across
         const bool up[3] = {li + 1 < ex, lj + 1 < ey, lk + 1 < ez};
-        C2<T> y = (MODE == RB_RED_BWD) ? W[q] : C2<T>{T(0), T(0)};
+        C2<T> y = {T(0), T(0)};
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
           const T c = (MODE == RB_RED_BWD || up[s]) ? T(h4.c[s]) : T(0);
@@ -585,14 +604,10 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
           y.x -= c * z.x;
           y.y -= c * z.y;
         }
-        if constexpr (MODE == RB_RED_BWD) {
-          W[q] = cmul(dvp, y);
-        } else {
-          // y holds -(sum of the upper terms); the box holds dv*y_forward
-          const C2<T> d = cmul(dvp, y);
-          const C2<T> b = W[q];
-          W[q] = {b.x + d.x, b.y + d.y};
-        }
+        // y holds -(sum of the upper terms); the box holds w = dv*y_forward
+        const C2<T> d = cmul(dvp, y);
+        const C2<T> b = W[q];
+        W[q] = {b.x + d.x, b.y + d.y};
       }
       wave_sync();
     }
@@ -605,17 +620,9 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
     const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
     if (!(li < ex && lj < ey && lk < ez)) continue;
     const int q = rb_box(li, lj, lk);
-    const C2<T> zv = (MODE == RB_RED_FWD) ? DV[shp->rslot[p]] : W[q];
+    const C2<T> zv = W[q];
     dst[p] = zv;
-    if (MODE != RB_RED_BWD) {
-      const C2<T> fz = W[q];
-      if (li == 0) xfo[0 * 64 + lj + 8 * lk] = fz;
-      if (li == TX - 1) xfo[1 * 64 + lj + 8 * lk] = fz;
-      if (lj == 0) xfo[2 * 64 + li + 8 * lk] = fz;
-      if (lj == TX - 1) xfo[3 * 64 + li + 8 * lk] = fz;
-      if (lk == 0) xfo[4 * 64 + li + 8 * lj] = fz;
-      if (lk == TX - 1) xfo[5 * 64 + li + 8 * lj] = fz;
-    }
+    if (MODE != RB_RED_BWD && (li == 0 || li == TX - 1)) xfo[(li == 0 ? 0 : 64) + lj + 8 * lk] = zv;
   }
 }
 

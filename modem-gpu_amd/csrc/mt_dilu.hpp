@@ -46,9 +46,8 @@ constexpr int kMaxTilePts = 1024;
 // plane or box lands in a low pad slot (index mod kRbNB), disjoint from the
 // tile and from the other high faces.
 constexpr int kRbBX = 9, kRbPK = 82, kRbNB = 9 * kRbPK, kRbNP = 512;
-// packed faces per tile and system in DiluArgs::xf: six faces of 64 (k_dilu_bk;
-// the all-edge-layout red-black kernels use the first two, the i-faces)
-constexpr int kRbFaces = 384;
+// packed i-faces per tile and system in DiluArgs::xf: two faces of 64
+constexpr int kRbFaces = 128;
 __host__ __device__ __forceinline__ int rb_box(int li, int lj, int lk) {
   return (li + 1) + kRbBX * (lj + 1) + kRbPK * (lk + 1);
 }
@@ -93,15 +92,12 @@ struct DiluArgs {
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed
-  // red-black sweeps: the faces of every tile, packed
-  // [half][system][tile][face][64]: face 2d is the low and 2d+1 the high face
-  // in direction d (i, j, k), entry = the other two tile coordinates a + 8*b
-  // in (i, j, k) order; kRbFaces entries per tile (k_dilu_bk uses all six,
-  // the all-edge-layout red-black kernels the i-faces).  Half 0 holds dinv*y of the red
-  // tiles (written by the red forward sweep, read by the black one), half 1
-  // holds z of the black tiles (written by the black sweep, read by the red
-  // backward one).  Packed, a neighbour face is one coalesced read instead of
-  // 64 reads of 8 bytes spread over 8 or 64 cache lines.
+  // red-black sweeps: the i-faces (li = 0 and li = 7) of every tile, packed
+  // [half][system][tile][face][64], entry lj + 8*lk (kRbFaces per tile).
+  // Half 0 holds dinv*y of the red tiles (written by the red forward sweep,
+  // read by the black one), half 1 holds z of the black tiles (written by the
+  // black sweep, read by the red backward one).  Packed, a neighbour face is
+  // one coalesced read instead of 64 reads of 8 bytes, one per cache line.
   C2<T>* xf = nullptr;
   int64_t xfsys = 0, xfhalf = 0;
 };

@@ -2122,6 +2122,9 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   const KmGeom km = km_geom(ctx);
   const bool use_km = ctx->spmv_variant == 2 || (ctx->spmv_variant == 1 && precision == MT_PREC_MIXED);
   const int msS = (ctx->spmv_variant == 3 && precision == MT_PREC_MIXED) ? ctx->ms_S : 0;
+  // what the mixed solver's inner solves run: the brick SpMV
+  const bool bricks = ctx->brick && msS == 4;
+  if (bricks && (rc = ensure_brick(ctx, false))) return rc;
   if ((rc = ws(ctx, "bn_sc", nsys, &d_sc)) || (rc = ws(ctx, "bn_act", nsys, &d_act)) ||
       (rc = ws(ctx, "partials", size_t(nsys) * std::max(pick_G(ctx, 1), km.Gpad) * 2 + 64,
                &partials)))
@@ -2144,13 +2147,21 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
     a.partials = partials;
     C2<T>* vin; C2<T>* vout; C2<T>* vrt;
     int r2;
-    if ((r2 = ws(ctx, sizeof(T) == 8 ? "bn_in64" : "bn_in32", nv, &vin)) ||
-        (r2 = ws(ctx, sizeof(T) == 8 ? "bn_out64" : "bn_out32", nv, &vout)) ||
-        (r2 = ws(ctx, sizeof(T) == 8 ? "bn_rt64" : "bn_rt32", nv, &vrt)))
+    const size_t nvb = bricks ? size_t(ctx->bk.g.nB) * nsys : nv;
+    if ((r2 = ws(ctx, sizeof(T) == 8 ? "bn_in64" : (bricks ? "bn_inb" : "bn_in32"), nvb, &vin)) ||
+        (r2 = ws(ctx, sizeof(T) == 8 ? "bn_out64" : (bricks ? "bn_outb" : "bn_out32"), nvb, &vout)) ||
+        (r2 = ws(ctx, sizeof(T) == 8 ? "bn_rt64" : (bricks ? "bn_rtb" : "bn_rt32"), nvb, &vrt)))
       return r2;
     a.ph = vin; a.v = vout; a.rt = vrt;
+    if (bricks) {
+      a.nE = ctx->bk.g.nB;
+      a.G = std::min(2048, (ctx->bk.g.tbase[3] + 7) / 8 * 8);
+    }
     auto launch = [&]() {
-      if (msS == 4) k_spmv_ms<T, T, 1, 4><<<((nsys + 3) / 4) * G, BS, 0, ctx->stream>>>(a);
+      if (bricks)
+        k_spmv_bt<T, T, 1><<<a.G, kBtThreads, 0, ctx->stream>>>(a, ctx->bk.g,
+                                                                reinterpret_cast<const T*>(ctx->bk.rows[0]));
+      else if (msS == 4) k_spmv_ms<T, T, 1, 4><<<((nsys + 3) / 4) * G, BS, 0, ctx->stream>>>(a);
       else if (msS == 8) k_spmv_ms<T, T, 1, 8><<<((nsys + 7) / 8) * G, BS, 0, ctx->stream>>>(a);
       else if (use_km) k_spmv_km<T, T, 1><<<nsys * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
       else k_spmv_dot<T, T, 1><<<nsys * G, BS, 0, ctx->stream>>>(a);
@@ -2214,9 +2225,26 @@ int mt_bench_precond(mt_ctx* ctx, int nsys, int precision, int precond, int reps
     const TilePlan& tp = precond == MT_PC_DILU_RB ? ctx->tiles_rb : ctx->tiles;
     DiluArgs<T, T> d = dilu_args<T, T>(tp, view<T>(ctx, false).coef, g.nE, g.nE, d_act,
                                        nsys, d_sc, vin, vout, dinv, ST_RUN);
+    // what the mixed solver's inner solves run: the sweeps on bricks
+    const bool bricks = sizeof(T) == 4 && precond == MT_PC_DILU_RB && ctx->brick &&
+                        ctx->spmv_variant == 3 && ctx->ms_S == 4;
+    if (bricks) {
+      if ((r2 = ensure_brick(ctx, false))) return r2;
+      const int64_t nB = ctx->bk.g.nB;
+      C2<T> *bin, *bout, *bdv;
+      if ((r2 = ws(ctx, "bp_inb", size_t(nB) * nsys, &bin)) ||
+          (r2 = ws(ctx, "bp_outb", size_t(nB) * nsys, &bout)) ||
+          (r2 = ws(ctx, "bp_dinvb", size_t(nB) * nf, &bdv)))
+        return r2;
+      k_bk_gather<T><<<4096, BS, 0, ctx->stream>>>(ctx->bk.bmap, nB, nf, g.nE, nB, dinv, bdv);
+      d = dilu_args<T, T>(tp, reinterpret_cast<const T*>(ctx->bk.slots[0]), nB, nB, d_act, nsys,
+                          d_sc, bin, bout, bdv, ST_RUN);
+    }
     if (precond == MT_PC_DILU_RB && (r2 = rb_faces(ctx, nsys, d))) return r2;
     auto apply = [&]() {
-      if (precond == MT_PC_DILU_RB) {
+      if (bricks) {
+        dilu_bk_apply<T, T>(ctx, tp, d);
+      } else if (precond == MT_PC_DILU_RB) {
         dilu_rb_apply<T, T>(ctx, tp, d);
       } else {
         dilu_fwd<T, T>(ctx, tp, d);
