@@ -180,6 +180,15 @@ def main():
                          "per workgroup, 5 the same with the lean LDS layout")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
+    ap.add_argument("--inner-layout", default="bricks", choices=("bricks", "all-edge"),
+                    help="inner FP32 vectors of the mixed solves: 8x8x8 tiles contiguous "
+                         "(bricks) or the all-edge layout of the outer solve")
+    ap.add_argument("--brick-systems", type=int, default=8,
+                    help="systems per workgroup of the brick SpMV (0: all)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time and pick the brick SpMV's systems per workgroup and the vector "
+                         "updates' row chunks before the run (the default keeps fixed choices, "
+                         "so that every run takes the same iterations)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -200,6 +209,8 @@ def main():
     adjoint = not args.forward_only and cfg != "cfg3"
     ctx = mtsp2.Context(0)
     ctx.set_dilu_variant(args.dilu_variant)
+    bricks = args.inner_layout == "bricks"
+    ctx.set_inner_layout(bricks, args.brick_systems)
     t0 = time.time()
     ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
     log(f"assembled n_int={ctx.n_int} nnz={ctx.nnz} in {time.time()-t0:.1f}s")
@@ -243,8 +254,10 @@ def main():
         ctx.close()
         return
 
-    tuned = ctx.autotune(nsys)
-    log(f"autotune: {tuned}")
+    tuned = None
+    if args.autotune:
+        tuned = ctx.autotune(nsys)
+        log(f"autotune: row chunks {tuned} over the active systems")
     if args.spmv_variant is not None:
         ctx.set_spmv_variant(args.spmv_variant,
                              args.spmv_systems if args.spmv_variant == 3 else 0)
@@ -338,8 +351,15 @@ def main():
     # solver in both orderings
     ablation = {}
     fp64_iters = None
+    if not args.no_fp64 and bricks:
+        # the same mixed solves with the inner vectors in the all-edge layout
+        ctx.set_inner_layout(False)
+        t0 = time.time()
+        step()
+        ablation["mixed_batched_red_black_all_edge_layout"] = per_step / (time.time() - t0)
+        ctx.set_inner_layout(True, args.brick_systems)
+        log(f"mixed, all-edge inner layout: {ablation['mixed_batched_red_black_all_edge_layout']:.2f} solves/s")
     if not args.no_fp64:
-        ctx.set_tuning(0)      # the one-system runs pick their own row-chunk count
         sets = [(d_b, d_x0, 0)] + ([(d_ba, d_zero, 1)] if adjoint else [])
         t0 = time.time()
         its = []
@@ -355,7 +375,6 @@ def main():
         fp64_iters = its
         log(f"plain FP64 GPU (one system at a time): {plain:.2f} solves/s, iters {its}")
         ablation["fp64_one_at_a_time_reference_order"] = plain
-        ctx.set_tuning(tuned)
         for name, pc in (("fp64_batched_reference_order", mtsp2.PC_DILU),
                          ("fp64_batched_red_black", mtsp2.PC_DILU_RB)):
             t0 = time.time()
@@ -390,6 +409,9 @@ def main():
     rb_kernel = {4: "k_dilu_rbm<float,float,RED_FWD|BLACK_FB|RED_BWD,4>",
                  5: "k_dilu_rbl<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (lean LDS)",
                  1: "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD>"}[args.dilu_variant]
+    on_bricks = bricks and args.precond == 3 and args.spmv_variant in (None, 3) and args.spmv_systems == 4
+    if on_bricks:
+        rb_kernel = "k_dilu_bk<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (on bricks)"
     pc_kernels = {0: "k_dilu_tile<float,float,8,8,8,true|false> (block-DILU level sweeps)",
                   3: rb_kernel + " (red-black tile block-DILU, 3 launches per application)"}
     spv = 3 if args.spmv_variant is None else args.spmv_variant
@@ -398,6 +420,9 @@ def main():
                     2: "k_spmv_km<float,float,1|2> (k-marching v = A x with its dot products)",
                     3: f"k_spmv_ms<float,float,1|2,{args.spmv_systems}> (v = A x for "
                        f"{args.spmv_systems} systems per block, with the dot products)"}
+    if on_bricks:
+        spmv_kernels[3] = (f"k_spmv_bt<float,float,1|2> (v = A x on bricks, "
+                           f"{args.brick_systems or 'all'} systems per block, with the dot products)")
     kernels = {"spmv": spmv_kernels[spv], "precond": pc_kernels.get(args.precond, "preconditioner")}
     achieved = kby / (kms * 1e-3) / 1e9
     traffic = counted_traffic(dom, args)
@@ -418,9 +443,7 @@ def main():
     roofline["spmv_isolated_fp32"] = {"systems": nsys, "us_per_launch": spmv_ms * 1e3,
                                       "bytes_per_launch": spmv_bytes,
                                       "GBps": spmv_bytes / (spmv_ms * 1e-3) / 1e9}
-    ctx.set_tuning(0)
     spmv64_ms, spmv64_bytes = ctx.bench_spmv(1, mtsp2.PREC_FP64, reps=20)
-    ctx.set_tuning(tuned)
     roofline["spmv_isolated_fp64_1sys"] = {"us_per_launch": spmv64_ms * 1e3,
                                            "bytes_per_launch": spmv64_bytes,
                                            "GBps": spmv64_bytes / (spmv64_ms * 1e-3) / 1e9}
@@ -457,6 +480,10 @@ def main():
                                f"{fwd_adj} systems ({per_step} solves per step), tol {args.tol:g} "
                                "FP64 true residual",
                    "nsys": nsys, "seed": args.seed, "inner_tol": args.inner_tol,
+                   "inner_layout": args.inner_layout,
+                   "tuning": ({"autotune": True, "rows_total": tuned} if tuned else
+                              {"autotune": False, "rows_total": 2048,
+                               "brick_systems_per_block": args.brick_systems}),
                    "restart": args.restart, "sites": (len(sites) if adjoint else None),
                    "precond": {0: "block-DILU (reference ordering)", 1: "Jacobi", 2: "none",
                                3: "block-DILU, red-black 8x8x8 tile ordering"}[args.precond]},

@@ -360,9 +360,13 @@ int mt_lmultT(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const voi
 int mt_lmultT_noalloc(mt_ctx *ctx, int dtype, int nfreq, const double *omegas, const void *e_dev,
                       const double *d, const unsigned char *exist, void *comb_dev);
 
-/* Runs the SpMV autotuner for nsys systems: times the solver's SpMV kernel for
- * several row-tile group counts per system and keeps the fastest; *variant
- * receives the chosen group count. */
+/* Tunes, for nsys systems, what the mixed solver's inner iterations run, by
+ * timing and keeping the fastest: the systems per workgroup of the brick SpMV
+ * (all, 8 or 4) and the row chunks of the vector updates, as a total over the
+ * active systems (1024 ... 32768); *variant receives the chosen total.  The
+ * choices follow measured times, so they can differ between runs (and with
+ * them the reduction order of the dot products).  mt_set_tuning's fixed
+ * count per system, when set, takes precedence over the total. */
 int mt_autotune(mt_ctx *ctx, int nsys, int *variant);
 /* Times one block-DILU application (MT_PC_DILU or MT_PC_DILU_RB) on nsys
  * systems, two per frequency, at MT_PREC_FP64 or MT_PREC_MIXED (FP32), with
@@ -391,9 +395,11 @@ int mt_set_dilu_variant(mt_ctx *ctx, int variant);
 /* Layout of the mixed solver's inner FP32 vectors when they are
  * preconditioned by the red-black DILU with the default SpMV: 1 (default) =
  * bricks (each 8x8x8 tile of the red-black plan contiguous), 0 = the all-edge
- * layout of the outer solve.  The same operator and preconditioner; the
- * inner sums differ in rounding.  No counterpart in the reference. */
-int mt_set_inner_layout(mt_ctx *ctx, int bricks);
+ * layout of the outer solve.  systems_per_block: systems per workgroup of the
+ * brick SpMV (0 = all active systems; default 8).  The same operator and
+ * preconditioner; the inner sums differ in rounding.  No counterpart in the
+ * reference. */
+int mt_set_inner_layout(mt_ctx *ctx, int bricks, int systems_per_block);
 /* The inner solves' brick kernels on one vector, for checking them: y = A x
  * (what = 0) or y = M^-1 x (what = 1, the red-black DILU of A(omega)), of A
  * (transpose = 0) or A^T (1), computed in FP32 on bricks; x, y: n_int

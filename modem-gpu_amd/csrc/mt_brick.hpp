@@ -23,6 +23,8 @@
 namespace mt {
 
 constexpr int kBkPts = 512;
+// chunks of tiles of the brick SpMV (at most; a multiple of the 8 XCDs)
+constexpr int kBtChunks = 2048;
 
 // Tile geometry of the bricks (the red-black tile plan's lattices).
 struct BrickGeom {
@@ -195,11 +197,11 @@ __global__ __launch_bounds__(BS) void k_bk_widen(const C2<float>* __restrict__ x
 // ------------------------------------------------------------------ SpMV
 // y_s = A x_s on bricks with the dot products of k_spmv_ms (WHICH=1:
 // v = A ph, <rt,v>; WHICH=2: t = A sh, <t,t>, <t,s>), one block per chunk of
-// tiles for all active systems (four systems' sums per thread, as k_spmv_ms
-// keeps them, cap a brick kernel at two or three waves per SIMD by
+// tiles and group of sg active systems (four systems' sums per thread, as
+// k_spmv_ms keeps them, cap a brick kernel at two or three waves per SIMD by
 // registers): a thread owns one point of the tile, forms its 14 neighbours'
 // brick indices and loads its 16 couplings (the brick-ordered row-major copy)
-// once, then runs the systems one after another, two at a time.  A
+// once, then runs the group's systems one after another, two at a time.  A
 // neighbour that is not an interior edge has a zero coupling and the row's
 // own index.  The row sums run in k_spmv_ms's order.  Each wave reduces its
 // lanes' dot contributions per system (wave_sum63) into LDS; the block then
@@ -207,12 +209,16 @@ __global__ __launch_bounds__(BS) void k_bk_widen(const C2<float>* __restrict__ x
 constexpr int kBtThreads = 512, kBtWaves = kBtThreads / 64, kBtMaxSys = 64;
 template <class T, class V, int WHICH>
 __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickGeom bg,
-                                                        const V* __restrict__ rowc) {
+                                                        const V* __restrict__ rowc, int sg) {
   constexpr int NQ = WHICH == 1 ? 2 : 3;   // doubles per system: <rt,v>; <t,t>, <t,s>
   __shared__ double red[kBtWaves][kBtMaxSys][NQ];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-  const int g = (blockIdx.x & 7) * (a.G >> 3) + (blockIdx.x >> 3);
-  const int nact = a.nact;
+  // blocks (chunk g, group of sg active systems), the groups of a chunk
+  // consecutive inside an XCD (they read the same couplings)
+  const int ngrp = (a.nact + sg - 1) / sg;
+  const int w = blockIdx.x >> 3, grp = w % ngrp;
+  const int g = (blockIdx.x & 7) * (a.G >> 3) + w / ngrp;
+  const int s0 = grp * sg, nact = min(a.nact, s0 + sg);
   for (int q = tid; q < kBtWaves * kBtMaxSys * NQ; q += kBtThreads) (&red[0][0][0])[q] = 0.0;
   __syncthreads();
   const bool fz = WHICH == 2 && a.fused;
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
       }
       return true;
     };
-    for (int si = 0; si < nact; si += 2) {
+    for (int si = s0; si < nact; si += 2) {
       double d0[NQ], d1[NQ];
       const bool r0 = one(si, d0);
       const bool r1 = si + 1 < nact ? one(si + 1, d1) : false;
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
     }
   }
   __syncthreads();
-  for (int si = tid; si < nact; si += kBtThreads) {
+  for (int si = s0 + tid; si < nact; si += kBtThreads) {
     double v[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {

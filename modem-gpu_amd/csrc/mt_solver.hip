@@ -129,11 +129,13 @@ struct mt_ctx {
   TilePlan tiles_rb;  // red-black tile ordering of the same lattices
   BrickData bk;       // bricks of the mixed solver's inner solves
   bool brick = true;  // inner solves on bricks (mixed, red-black DILU, 4-system SpMV)
+  int bk_sg = 8;      // brick SpMV: systems per block (0: all active systems)
   DivcorData dc;
   SiteData sites;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  int G_override = 0;
+  int G_override = 0;           // row chunks per system (mt_set_tuning; 0: rows_total)
+  int rows_total = 0;           // row chunks over the active systems (mt_autotune; 0: 2048)
   int num_cu = 256;             // compute units of the device
   int spmv_variant = 3;         // 0 row-parallel SpMV; 1 k-marching for FP32
                                 // solves, row-parallel for FP64; 2 k-marching;
@@ -710,11 +712,24 @@ static int64_t stencil_nnz(mt_ctx* ctx) {
 }
 
 // ======================================================= BiCGStab engine
+// Row chunks per system of the row-parallel kernels (vector updates, row
+// SpMV): a fixed count (mt_set_tuning), else a total over the active systems
+// (mt_autotune's choice, default 2048).
 static int pick_G(const mt_ctx* ctx, int nact) {
   if (ctx->G_override > 0) return ctx->G_override;
-  int G = 2048 / std::max(1, nact);
+  const int total = ctx->rows_total > 0 ? ctx->rows_total : 2048;
+  int G = total / std::max(1, nact);
   G = std::max(8, (G / 8) * 8);
   return G;
+}
+
+// Chunks of the brick SpMV: at most kBtChunks, a multiple of 8.
+static int brick_chunks(const BrickGeom& g) { return std::min(kBtChunks, (g.tbase[3] + 7) / 8 * 8); }
+
+// Per-system partial sums any reduction kernel may store: the row kernels'
+// chunks, the k-marching SpMV's work items, the brick SpMV's chunks.
+static int partials_G(const mt_ctx* ctx, const KmGeom& km) {
+  return std::max(std::max(pick_G(ctx, 1), km.Gpad), kBtChunks);
 }
 
 // Work items of the k-marching SpMV for the grid of ctx.
@@ -797,8 +812,10 @@ struct Engine {
       // one block per chunk of tiles for all systems: at most 2048 chunks
       // (the partials buffer), a multiple of 8 (XCDs)
       IterArgs<T, V> b = a;
-      b.G = std::min(2048, (bg.tbase[3] + 7) / 8 * 8);
-      k_spmv_bt<T, V, WHICH><<<b.G, kBtThreads, 0, ctx->stream>>>(b, bg, bk_rows);
+      b.G = brick_chunks(bg);
+      const int sg = ctx->bk_sg > 0 ? ctx->bk_sg : a.nact;
+      const int ngrp = (a.nact + sg - 1) / sg;
+      k_spmv_bt<T, V, WHICH><<<b.G * ngrp, kBtThreads, 0, ctx->stream>>>(b, bg, bk_rows, sg);
       G = b.G;
     } else if (ms > 0) {
       const int nb = ((a.nact + ms - 1) / ms) * a.G;
@@ -1013,7 +1030,7 @@ static int solve_fp64(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   }
   E.use_km = ctx->spmv_variant == 2;
   E.km = km_geom(ctx);
-  const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
+  const int Gmax = partials_G(ctx, E.km);
   SysScal* d_sc;
   int* d_act;
   if ((rc = ws(ctx, "f64_sc", nsys, &d_sc)) || (rc = ws(ctx, "act", nsys, &d_act)) ||
@@ -1154,7 +1171,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   E.use_km = ctx->spmv_variant == 1 || ctx->spmv_variant == 2;
   E.ms = ctx->spmv_variant == 3 ? ctx->ms_S : 0;
   E.km = km_geom(ctx);
-  const int Gmax = std::max(pick_G(ctx, 1), E.km.Gpad);
+  const int Gmax = partials_G(ctx, E.km);
   SysScal *d_so, *d_si;
   int* d_act;
   if ((rc = ws(ctx, "mx_so", nsys, &d_so)) || (rc = ws(ctx, "mx_si", nsys, &d_si)) ||
@@ -1715,10 +1732,10 @@ int mt_inner_apply(mt_ctx* ctx, double omega, int what, int transpose, const mt_
     a.v = yb;
     a.rt = zb;
     Z* part;
-    a.G = std::min(2048, (ctx->bk.g.tbase[3] + 7) / 8 * 8);
+    a.G = brick_chunks(ctx->bk.g);
     if ((rc = ws(ctx, "ia_part", size_t(a.G) * 2, &part))) return rc;
     a.partials = part;
-    k_spmv_bt<float, float, 1><<<a.G, kBtThreads, 0, ctx->stream>>>(a, ctx->bk.g, ctx->bk.rows[tr ? 1 : 0]);
+    k_spmv_bt<float, float, 1><<<a.G, kBtThreads, 0, ctx->stream>>>(a, ctx->bk.g, ctx->bk.rows[tr ? 1 : 0], 1);
   } else {
     std::vector<double> freqs(1, omega);
     C2<float>* dinv;
@@ -2126,7 +2143,7 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
   const bool bricks = ctx->brick && msS == 4;
   if (bricks && (rc = ensure_brick(ctx, false))) return rc;
   if ((rc = ws(ctx, "bn_sc", nsys, &d_sc)) || (rc = ws(ctx, "bn_act", nsys, &d_act)) ||
-      (rc = ws(ctx, "partials", size_t(nsys) * std::max(pick_G(ctx, 1), km.Gpad) * 2 + 64,
+      (rc = ws(ctx, "partials", size_t(nsys) * partials_G(ctx, km) * 2 + 64,
                &partials)))
     return rc;
   MT_HIP_CHECK(ctx, hipMemcpyAsync(d_sc, h.data(), nsys * sizeof(SysScal), hipMemcpyHostToDevice, ctx->stream));
@@ -2155,12 +2172,14 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
     a.ph = vin; a.v = vout; a.rt = vrt;
     if (bricks) {
       a.nE = ctx->bk.g.nB;
-      a.G = std::min(2048, (ctx->bk.g.tbase[3] + 7) / 8 * 8);
+      a.G = brick_chunks(ctx->bk.g);
     }
     auto launch = [&]() {
-      if (bricks)
-        k_spmv_bt<T, T, 1><<<a.G, kBtThreads, 0, ctx->stream>>>(a, ctx->bk.g,
-                                                                reinterpret_cast<const T*>(ctx->bk.rows[0]));
+      if (bricks) {
+        const int sg = ctx->bk_sg > 0 ? ctx->bk_sg : nsys;
+        k_spmv_bt<T, T, 1><<<a.G * ((nsys + sg - 1) / sg), kBtThreads, 0, ctx->stream>>>(
+            a, ctx->bk.g, reinterpret_cast<const T*>(ctx->bk.rows[0]), sg);
+      }
       else if (msS == 4) k_spmv_ms<T, T, 1, 4><<<((nsys + 3) / 4) * G, BS, 0, ctx->stream>>>(a);
       else if (msS == 8) k_spmv_ms<T, T, 1, 8><<<((nsys + 7) / 8) * G, BS, 0, ctx->stream>>>(a);
       else if (use_km) k_spmv_km<T, T, 1><<<nsys * km.Gpad, KM_BS, 0, ctx->stream>>>(a, km);
@@ -2274,10 +2293,11 @@ int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
   return MT_OK;
 }
 
-int mt_set_inner_layout(mt_ctx* ctx, int bricks) {
-  if (!ctx || (bricks != 0 && bricks != 1))
-    return mt_fail(ctx, MT_E_ARG, "mt_set_inner_layout: bricks is 0 or 1");
+int mt_set_inner_layout(mt_ctx* ctx, int bricks, int systems_per_block) {
+  if (!ctx || (bricks != 0 && bricks != 1) || systems_per_block < 0 || systems_per_block > 64)
+    return mt_fail(ctx, MT_E_ARG, "mt_set_inner_layout: bricks is 0 or 1, systems_per_block 0..64");
   ctx->brick = bricks != 0;
+  ctx->bk_sg = systems_per_block;
   return MT_OK;
 }
 
@@ -2300,29 +2320,93 @@ int mt_set_tuning(mt_ctx* ctx, int groups_per_system) {
   return MT_OK;
 }
 
+// One inner iteration's vector updates (k_pupdate + k_rupdate_fused) on
+// nsys systems of n entries (the inner layout), ms per pair of launches.
+static int time_updates(mt_ctx* ctx, int nsys, int64_t n, int reps, double* ms_out) {
+  std::vector<SysScal> h(nsys);
+  std::vector<int> act(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    std::memset(&h[s], 0, sizeof(SysScal));
+    h[s].status = ST_RUN;
+    h[s].alpha = {0.5, 0.0};
+    h[s].omg = {0.25, 0.0};
+    h[s].beta = {0.125, 0.0};
+    h[s].rho = {1.0, 0.0};
+    act[s] = s;
+  }
+  SysScal* d_sc;
+  int* d_act;
+  C2<float>* v;
+  Z* part;
+  IterArgs<float, float> a{};
+  a.G = pick_G(ctx, nsys);
+  const size_t nv = size_t(n) * nsys;
+  int rc;
+  if ((rc = ws(ctx, "tu_sc", nsys, &d_sc)) || (rc = ws(ctx, "tu_act", nsys, &d_act)) ||
+      (rc = ws(ctx, "tu_vec", nv * 8, &v)) || (rc = ws(ctx, "tu_part", size_t(nsys) * a.G * 2 + 64, &part)))
+    return rc;
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(d_sc, h.data(), nsys * sizeof(SysScal), hipMemcpyHostToDevice, ctx->stream));
+  MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, act.data(), nsys * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+  a.sc = d_sc; a.act = d_act; a.nact = nsys; a.nE = n; a.pc = PCK_DILU_RB; a.fused = 1;
+  a.partials = part;
+  a.x = v; a.r = v + nv; a.rt = v + 2 * nv; a.p = v + 3 * nv; a.ph = v + 4 * nv;
+  a.v = v + 5 * nv; a.sh = v + 6 * nv; a.t = v + 7 * nv;
+  const int nblk = nsys * a.G;
+  auto run = [&]() {
+    k_pupdate<float, float><<<nblk, BS, 0, ctx->stream>>>(a);
+    k_rupdate_fused<float, float><<<nblk, BS, 0, ctx->stream>>>(a);
+  };
+  run();  // warm-up
+  MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+  for (int q = 0; q < reps; ++q) run();
+  MT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  MT_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[1]));
+  float ms = 0.f;
+  MT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+  *ms_out = double(ms) / reps;
+  return MT_OK;
+}
+
 int mt_autotune(mt_ctx* ctx, int nsys, int* variant) {
   int rc = need_op(ctx);
   if (rc) return rc;
-  const int cands[] = {64, 128, 256, 512, 1024, 2048};
+  if (nsys < 1 || nsys > 64) return mt_fail(ctx, MT_E_ARG, "mt_autotune: nsys is 1..64");
+  hipSetDevice(ctx->device);
+  const bool bricks = ctx->brick && ctx->spmv_variant == 3 && ctx->ms_S == 4;
+  // 1. the brick SpMV's systems per workgroup
+  if (bricks) {
+    double best = 1e30;
+    int bestS = ctx->bk_sg;
+    for (int sg : {0, 8, 4}) {
+      ctx->bk_sg = sg;
+      double ms = 0.0;
+      if ((rc = mt_bench_spmv(ctx, nsys, MT_PREC_MIXED, 5, &ms, nullptr))) return rc;
+      if (ms < best) { best = ms; bestS = sg; }
+    }
+    ctx->bk_sg = bestS;
+  }
+  // 2. the row chunks of the vector updates, as a total over the systems
+  int64_t n = ctx->hop.g.nE;
+  if (bricks) {
+    if ((rc = ensure_brick(ctx, false))) return rc;
+    n = ctx->bk.g.nB;
+  }
+  const int keepG = ctx->G_override;
+  ctx->G_override = 0;
   double best = 1e30;
-  int bestG = 0;
-  // G is the row-chunk count of the row-parallel kernels (the row SpMV and
-  // the vector updates): time it on the row SpMV whatever variant is chosen
-  const int keep = ctx->spmv_variant;
-  ctx->spmv_variant = 0;
-  for (int G : cands) {
-    if (G * nsys > 65536) continue;
-    ctx->G_override = G;
+  int bestT = 2048;
+  for (int T : {1024, 2048, 4096, 8192, 16384, 32768}) {
+    if (T / nsys < 8) continue;
+    ctx->rows_total = T;
     double ms = 0.0;
-    if ((rc = mt_bench_spmv(ctx, nsys, MT_PREC_MIXED, 5, &ms, nullptr))) {
-      ctx->spmv_variant = keep;
+    if ((rc = time_updates(ctx, nsys, n, 5, &ms))) {
+      ctx->G_override = keepG;
       return rc;
     }
-    if (ms < best) { best = ms; bestG = G; }
+    if (ms < best) { best = ms; bestT = T; }
   }
-  ctx->spmv_variant = keep;
-  ctx->G_override = bestG;
-  if (variant) *variant = bestG;
+  ctx->rows_total = bestT;
+  if (variant) *variant = bestT;
   return MT_OK;
 }
 

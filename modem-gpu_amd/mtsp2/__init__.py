@@ -101,7 +101,7 @@ _SIGS = {
     "mt_timing_enable": (_I, [_P, _I]),
     "mt_set_spmv_variant": (_I, [_P, _I, _I]),
     "mt_set_dilu_variant": (_I, [_P, _I]),
-    "mt_set_inner_layout": (_I, [_P, _I]),
+    "mt_set_inner_layout": (_I, [_P, _I, _I]),
     "mt_inner_apply": (_I, [_P, _D, _I, _I, _PZ, _PZ]),
     "mt_bench_precond": (_I, [_P, _I, _I, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "mt_kernel_timing": (_I, [_P, _I, C.POINTER(C.c_longlong), C.POINTER(_D), C.POINTER(_D)]),
@@ -469,10 +469,12 @@ class Context:
         systems per workgroup in the lean LDS layout."""
         self._chk(lib().mt_set_dilu_variant(self._h, int(variant)))
 
-    def set_inner_layout(self, bricks):
+    def set_inner_layout(self, bricks, systems_per_block=8):
         """Inner FP32 vectors of the mixed solver (red-black DILU, default
-        SpMV) on bricks (True, default) or in the all-edge layout (False)."""
-        self._chk(lib().mt_set_inner_layout(self._h, int(bool(bricks))))
+        SpMV) on bricks (True, default) or in the all-edge layout (False);
+        systems_per_block: workgroup size in systems of the brick SpMV (0:
+        all active systems; default 8)."""
+        self._chk(lib().mt_set_inner_layout(self._h, int(bool(bricks)), int(systems_per_block)))
 
     def set_tuning(self, groups):
         self._chk(lib().mt_set_tuning(self._h, int(groups)))
