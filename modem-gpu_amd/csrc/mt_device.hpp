@@ -8,11 +8,14 @@
 //    (system, row tile) XCD-aware: the 8 XCDs work on different row tiles and
 //    consecutive blocks on one XCD cover the systems of one tile, so the
 //    shared stencil coefficients are fetched once per XCD L2;
-//  * dot products use wave64 shuffles, then LDS, then a deterministic
-//    last-block reduction over per-block partials (fixed order, no float
-//    atomics), and the same last block updates the solver scalars on the
-//    device: no host round trip per dot (the reference reads every dot back
-//    to the host, solver.f90:1098-1104).
+//  * dot products: each block reduces its lanes (wave64 shuffles or DPP
+//    moves, then LDS) and stores a per-block partial; a finalize kernel
+//    with one block per system, next on the stream, sums the partials in a
+//    fixed order (no float atomics, deterministic) and updates the solver
+//    scalars on the device: no host round trip per dot (the reference reads
+//    every dot back to the host, solver.f90:1098-1104);
+//  * the inner solves of the mixed solver keep their vectors on bricks
+//    (mt_brick.hpp).
 #pragma once
 
 #include <hip/hip_runtime.h>
