@@ -178,6 +178,9 @@ def main():
     ap.add_argument("--dilu-variant", type=int, default=5, choices=(1, 4, 5),
                     help="red-black DILU launch shape: 1 single-system persistent, 4 four systems "
                          "per workgroup, 5 the same with the lean LDS layout")
+    ap.add_argument("--drop-in-only", type=int, default=0, metavar="NSYS",
+                    help="only the plain FP64 solves (one system at a time, reference-ordered "
+                         "DILU: the cuBiCG drop-in) of the first NSYS forward systems, for profiling")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
     ap.add_argument("--inner-layout", default="bricks", choices=("bricks", "all-edge"),
@@ -251,6 +254,31 @@ def main():
         for _ in range(3):
             ctx.dev_copy(dst, src, gib)
         log("device copy of 1 GiB x 3 done")
+        ctx.close()
+        return
+
+    if args.drop_in_only:
+        use_graph = 0 if args.no_graph else 1
+        ns = min(args.drop_in_only, nsys)
+        ctx.timing_enable(True)
+        t0 = time.time()
+        its = []
+        for s in range(ns):
+            xs = d_x + s * n * 16
+            ctx.dev_copy(xs, d_x0 + s * n * 16, n * 16)
+            r = ctx.solve_batch_dev(om[s:s + 1], d_b + s * n * 16, xs, tol=args.tol, maxit=4000,
+                                    precision=mtsp2.PREC_FP64, precond=mtsp2.PC_DILU, chunk=10,
+                                    use_graph=use_graph)
+            its.append(r[0]["niter"])
+        el = time.time() - t0
+        kt = {name: ctx.kernel_timing(fam) for name, fam in
+              (("spmv", mtsp2.KT_SPMV), ("precond", mtsp2.KT_PRECOND))}
+        ctx.timing_enable(False)
+        log(f"plain FP64 (drop-in): {ns} systems in {el:.3f} s, {ns / el:.2f} solves/s, "
+            f"iters {its}, {1e3 * el / max(1, sum(its)):.3f} ms/iteration")
+        for name, (nl, kms, kby) in kt.items():
+            if nl:
+                log(f"  {name}: {nl} launches, {kms:.1f} ms, {1e3 * kms / nl:.1f} us/launch")
         ctx.close()
         return
 

@@ -64,6 +64,8 @@ struct TileShape {
   // couplings) are stored by slot, so a level's lanes read consecutive words.
   unsigned int rpts[kRbNP];
   unsigned short rslot[kRbNP];
+  // rpts by (hyperplane, lane): lane l's point of hyperplane h, or ~0u
+  unsigned int lpts[22][64];
 };
 
 struct LatticeInfo {
@@ -106,153 +108,188 @@ __device__ __forceinline__ int64_t lat_e(const LatticeInfo& L, int i, int j, int
   return L.base + (i + L.oi) + int64_t(L.a) * ((j + L.oj) + int64_t(L.b) * (k + L.ok));
 }
 
-// FWD=true: lower solve with halo on the low side; false: upper solve.
-// The tile's inputs (box values, pivots, three coefficients per point) are
-// all requested from memory before the first is used, so a tile pays one
-// memory latency, not one per box row; then the TX+TY+TZ-2 hyperplanes are
-// swept out of LDS and the tile is written back in box order.
-//   forward LDS: wb = dinv*y on the whole box (halo from memory, tile points as
-//                they are solved), rb = r -> y on the tile, dv = dinv on the tile
-//   backward LDS: yb = y -> z on the tile and z on the halo, dv on the tile
-template <class T, class V, int TX, int TY, int TZ, bool FWD>
-__global__ __launch_bounds__(64) void k_dilu_tile(DiluArgs<T, V> a, const TileShape* shp) {
-  constexpr int BX = TX + 1, BY = TY + 1, BZ = TZ + 1, NB = BX * BY * BZ, NP = TX * TY * TZ;
-  constexpr int NQ = (NB + 63) / 64;
-  constexpr int NLEV = TX + TY + TZ - 2;
-  __shared__ C2<T> yb[NB];   // FWD: w = dinv*y; BWD: y -> z
-  __shared__ C2<T> rb[FWD ? NP : 1];
-  __shared__ C2<T> dv[NP];
-  __shared__ V cf[3][NP];
-  __shared__ unsigned int spts[NP];
-  __shared__ int slvl[NLEV + 1];
+// ------------------------------------------ reference ordering, 8x8x8 tiles
+// One tile level per launch (grid = tiles of the level x active systems), one
+// wave per (tile, system): the tile's inputs and the upstream neighbours'
+// facing layer (final after the previous launch) are staged into LDS, the 22
+// hyperplanes swept, the tile written back.  A launch holds a few dozen to a
+// few hundred tiles of one system, so a tile's time is the launch's time, and
+// it is latency: the chain of dependent loads before the data arrive, the
+// hyperplane steps, the stores.
+//  * Loads: the system-independent tables (tile, lane's point of every
+//    hyperplane, level slots) and the system lookups are requested together,
+//    then every data load at once; the system's status is checked after (a
+//    skipped system's loads are wasted, never out of bounds).
+//  * A hyperplane step is one round of LDS reads: the lane's points are in
+//    registers (box index and level slot, extent folded in), the pivots and
+//    couplings sit by level slot (consecutive lanes, consecutive entries), and
+//    the box is rb_box's pitch-82 layout, which spreads a hyperplane over the
+//    LDS banks.
+//   forward: the box holds r on the tile and w = dinv*y of the low faces;
+//            a step forms y = r - sum a w, keeps y in the pivot's slot and
+//            leaves w = dinv*y in the box; y is written back.
+//   backward: the box holds y on the tile and z of the high faces (past the
+//             box's last plane, kRbNB + kRbPK entries in all); a step forms
+//             z = dinv*(y - sum a z) in place; z is written back.
+// Per point the three terms are taken in the order i, j, k.
+template <class T, class V, bool FWD>
+__global__ __launch_bounds__(64) void k_dilu_ro(DiluArgs<T, V> a, const TileShape* __restrict__ shp) {
+  constexpr int TX = 8, NP = kRbNP, NU = 8, NLEV = 22;
+  constexpr int NB = FWD ? kRbNB : kRbNB + kRbPK;
+  struct Cpl { V c[3]; };
+  __shared__ C2<T> W[NB];
+  __shared__ C2<T> DV[NP];   // pivots by level slot (forward: then y)
+  __shared__ Cpl CF[NP];     // the sweep's couplings by level slot
   const int lane = threadIdx.x;
   const int tix = blockIdx.x / a.nact, si = blockIdx.x - tix * a.nact;
-  const int sys = a.act[si];
-  if (a.sc[sys].status != a.status_req) return;
+  // ---- system-independent tables and the system lookups, together
   const int4 tl = a.tiles[tix];
+  const int sys = a.act[si];
+  int sl[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) sl[u] = shp->rslot[lane + 64 * u];
+  unsigned int pv[NLEV];
+#pragma unroll
+  for (int h = 0; h < NLEV; ++h) pv[h] = shp->lpts[h][lane];
+  const SysScal& sc = a.sc[sys];
+  const int status = sc.status, fidx = sc.fidx;
   const LatticeInfo L = a.lat[tl.x];
-  const int gi0 = tl.y * TX, gj0 = tl.z * TY, gk0 = tl.w * TZ;
-  const int ex = min(TX, L.A - gi0), ey = min(TY, L.B - gj0), ez = min(TZ, L.C - gk0);
+  const int gi0 = tl.y * TX, gj0 = tl.z * TX, gk0 = tl.w * TX;
+  const int ex = min(TX, L.A - gi0), ey = min(TX, L.B - gj0), ez = min(TX, L.C - gk0);
   const int64_t o = (int64_t)sys * a.vstride;
-  // uniform base pointers, held in registers (a per-lane choice between two
-  // kernel-argument fields would be re-read from the argument segment)
-  const C2<T>* __restrict__ pin = a.in + o;
-  const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o : nullptr;
-  const C2<T> al = ccast<T>(a.sc[sys].alpha);
+  const int64_t cst = a.cstride;
   const C2<T>* __restrict__ pout = a.out + o;
-  const C2<T>* __restrict__ dinv = a.dinv + (int64_t)a.sc[sys].fidx * a.cstride;
-  const int s0 = FWD ? 1 : 2;  // lower slots 1,3,5 / upper slots 2,4,6
-  const V* __restrict__ cfp[3] = {a.coef + (int64_t)s0 * a.cstride,
-                                  a.coef + (int64_t)(s0 + 2) * a.cstride,
-                                  a.coef + (int64_t)(s0 + 4) * a.cstride};
-  // ---- issue every load of the tile
-  C2<T> yv[NQ], dd[NQ];
-  V c3[NQ][3];
+  const C2<T>* __restrict__ dinv = a.dinv + (int64_t)fidx * cst;
+  const V* __restrict__ cf0 = a.coef + (FWD ? 1 : 2) * cst;   // slots 1,3,5 / 2,4,6
+  const int st[3] = {1, kRbBX, kRbPK};
+  // ---- the tile's own data (p = lane + 64u: li = lane&7, lj = lane>>3,
+  // lk = u): forward r (- alpha*in2), backward y; pivots; couplings
+  const int li0 = lane & 7, lj0 = lane >> 3;
+  const bool col_ok = li0 < ex && lj0 < ey;
+  const int64_t e0 = lat_e(L, gi0 + li0, gj0 + lj0, gk0);
+  const int64_t kst = int64_t(L.a) * L.b;
+  C2<T> rv[NU], dd[NU], r2[NU];
+  V cc[NU][3];
 #pragma unroll
-  for (int u = 0; u < NQ; ++u) {
-    const int q = lane + 64 * u;
-    const int bi = q % BX, bj = (q / BX) % BY, bk = q / (BX * BY);
-    // box coordinate b <-> local tile coordinate l: FWD: l = b - 1, BWD: l = b
-    const int li = FWD ? bi - 1 : bi, lj = FWD ? bj - 1 : bj, lk = FWD ? bk - 1 : bk;
-    const bool inTile = li >= 0 && lj >= 0 && lk >= 0 && li < ex && lj < ey && lk < ez;
-    const int gi = gi0 + li, gj = gj0 + lj, gk = gk0 + lk;
-    const bool inLat = q < NB && gi >= 0 && gj >= 0 && gk >= 0 && gi < L.A && gj < L.B && gk < L.C;
-    yv[u] = {T(0), T(0)};
-    dd[u] = {T(0), T(0)};
-    c3[u][0] = c3[u][1] = c3[u][2] = V(0);
-    if (inLat) {
-      const int64_t e = lat_e(L, gi, gj, gk);
-      const C2<T>* src = (FWD && inTile) ? pin : pout;
-      yv[u] = src[e];
-      if (FWD && inTile && pin2) yv[u] = csub(yv[u], cmul(al, pin2[e]));
-      if (FWD || inTile) dd[u] = dinv[e];
-      if (inTile) {
-#pragma unroll
-        for (int s = 0; s < 3; ++s) c3[u][s] = cfp[s][e];
-      }
-    }
-  }
-  unsigned int pk[(NP + 63) / 64];
-#pragma unroll
-  for (int u = 0; u < (NP + 63) / 64; ++u) {
-    const int p = lane + 64 * u;
-    pk[u] = p < NP ? shp->pts[p] : 0u;
-  }
-  const int lv = lane <= NLEV ? shp->lvl[lane] : 0;
-  // ---- stage into LDS
-#pragma unroll
-  for (int u = 0; u < NQ; ++u) {
-    const int q = lane + 64 * u;
-    if (q >= NB) break;
-    const int bi = q % BX, bj = (q / BX) % BY, bk = q / (BX * BY);
-    const int li = FWD ? bi - 1 : bi, lj = FWD ? bj - 1 : bj, lk = FWD ? bk - 1 : bk;
-    const bool inBoxTile = li >= 0 && lj >= 0 && lk >= 0 && li < TX && lj < TY && lk < TZ;
-    if (inBoxTile) {
-      const int p = li + TX * (lj + TY * lk);
-#pragma unroll
-      for (int s = 0; s < 3; ++s) cf[s][p] = c3[u][s];
-      dv[p] = dd[u];
+  for (int u = 0; u < NU; ++u) {
+    rv[u] = dd[u] = r2[u] = {T(0), T(0)};
+    cc[u][0] = cc[u][1] = cc[u][2] = V(0);
+    if (col_ok && u < ez) {
+      const int64_t e = e0 + u * kst;
       if (FWD) {
-        rb[p] = yv[u];
-        yb[q] = {T(0), T(0)};
+        rv[u] = a.in[o + e];
+        if (a.in2) r2[u] = a.in2[o + e];
       } else {
-        yb[q] = yv[u];
+        rv[u] = pout[e];   // in place
       }
-    } else {
-      // halo: FWD keeps w = dinv*y of the (final) upstream point, BWD its z
-      yb[q] = FWD ? cmul(dd[u], yv[u]) : yv[u];
+      dd[u] = dinv[e];
+#pragma unroll
+      for (int s = 0; s < 3; ++s) cc[u][s] = cf0[(int64_t)(2 * s) * cst + e];
     }
   }
+  // the facing layer of the upstream neighbours (forward: low faces, y and
+  // pivot; backward: high faces, z)
+  int fq[3];
+  C2<T> fv[3], fd[3];
 #pragma unroll
-  for (int u = 0; u < (NP + 63) / 64; ++u) {
-    const int p = lane + 64 * u;
-    if (p < NP) spts[p] = pk[u];
+  for (int dir = 0; dir < 3; ++dir) {
+    const int t0 = FWD ? 0 : TX - 1;
+    const int ti = dir == 0 ? t0 : li0;
+    const int tj = dir == 0 ? li0 : (dir == 1 ? t0 : lj0);
+    const int tk = dir == 2 ? t0 : lj0;
+    const int d = FWD ? -1 : 1;
+    const int ni = gi0 + ti + (dir == 0 ? d : 0);
+    const int nj = gj0 + tj + (dir == 1 ? d : 0);
+    const int nk = gk0 + tk + (dir == 2 ? d : 0);
+    fv[dir] = fd[dir] = {T(0), T(0)};
+    fq[dir] = -1;
+    const bool ok = ti < ex && tj < ey && tk < ez && ni >= 0 && nj >= 0 && nk >= 0 &&
+                    ni < L.A && nj < L.B && nk < L.C;
+    if (ok) {
+      const int64_t l = lat_e(L, ni, nj, nk);
+      fv[dir] = pout[l];
+      if (FWD) fd[dir] = dinv[l];
+      fq[dir] = FWD ? rb_box(ti, tj, tk) - st[dir] : rb_box(ti, tj, tk) + st[dir];
+    }
   }
-  if (lane <= NLEV) slvl[lane] = lv;
+  if (status != a.status_req) return;   // uniform
+  // this lane's point on every hyperplane: box index | level slot << 10
+  unsigned int pk[NLEV];
+#pragma unroll
+  for (int h = 0; h < NLEV; ++h) {
+    const unsigned int v = pv[h];
+    const int li = v & 7, lj = (v >> 3) & 7, lk = (v >> 6) & 7;
+    const bool ok = v != 0xffffffffu && li < ex && lj < ey && lk < ez;
+    pk[h] = ok ? (unsigned(rb_box(li, lj, lk)) | ((v >> 9) << 10)) : 0xffffffffu;
+  }
+  // ---- stage: zero box (partial tiles and absent faces read zeros), then
+  // the tile, the faces, the per-slot data
+  for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
   wave_sync();
-  // ---- hyperplane sweep
-  for (int hh = 0; hh < NLEV; ++hh) {
-    const int h = FWD ? hh : NLEV - 1 - hh;
-    for (int pp = slvl[h] + lane; pp < slvl[h + 1]; pp += 64) {
-      const unsigned int pt = spts[pp];
-      const int li = pt & 0xff, lj = (pt >> 8) & 0xff, lk = pt >> 16;
-      if (li >= ex || lj >= ey || lk >= ez) continue;
-      const int p = li + TX * (lj + TY * lk);
-      if (FWD) {
-        const int q = (li + 1) + BX * ((lj + 1) + BY * (lk + 1));
-        C2<T> y = rb[p];
-        const int qn[3] = {q - 1, q - BX, q - BX * BY};
+  const C2<T> al = ccast<T>(sc.alpha);
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const T c = T(cf[s][p]);
-          const C2<T> w = yb[qn[s]];
-          y.x -= c * w.x;
-          y.y -= c * w.y;
-        }
-        rb[p] = y;
-        yb[q] = cmul(dv[p], y);
-      } else {
-        const int q = li + BX * (lj + BY * lk);
-        C2<T> y = yb[q];
-        const int qn[3] = {q + 1, q + BX, q + BX * BY};
+  for (int u = 0; u < NU; ++u) {
+    C2<T> r = rv[u];
+    if (FWD && a.in2) r = csub(r, cmul(al, r2[u]));
+    W[rb_box(li0, lj0, u)] = r;
+    DV[sl[u]] = dd[u];
+    Cpl c;
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
-          const T c = T(cf[s][p]);
-          const C2<T> z = yb[qn[s]];
-          y.x -= c * z.x;
-          y.y -= c * z.y;
-        }
-        yb[q] = cmul(dv[p], y);
-      }
-    }
-    wave_sync();
+    for (int s = 0; s < 3; ++s) c.c[s] = cc[u][s];
+    CF[sl[u]] = c;
   }
-  // ---- write the tile back
-  for (int p = lane; p < NP; p += 64) {
-    const int li = p % TX, lj = (p / TX) % TY, lk = p / (TX * TY);
-    if (li >= ex || lj >= ey || lk >= ez) continue;
-    C2<T>* __restrict__ dst = a.out + o;
-    dst[lat_e(L, gi0 + li, gj0 + lj, gk0 + lk)] = FWD ? rb[p] : yb[li + BX * (lj + BY * lk)];
+#pragma unroll
+  for (int dir = 0; dir < 3; ++dir)
+    if (fq[dir] >= 0) W[fq[dir]] = FWD ? cmul(fd[dir], fv[dir]) : fv[dir];
+  wave_sync();
+  if (FWD) {
+#pragma unroll
+    for (int h = 0; h < NLEV; ++h) {
+      const unsigned int pt = pk[h];
+      if (pt != 0xffffffffu) {
+        const int q = pt & 0x3ff, p = pt >> 10;
+        const C2<T> dvp = DV[p];
+        const Cpl c = CF[p];
+        C2<T> y = W[q];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T cs = T(c.c[s]);
+          const C2<T> w = W[q - st[s]];
+          y.x -= cs * w.x;
+          y.y -= cs * w.y;
+        }
+        DV[p] = y;
+        W[q] = cmul(dvp, y);
+      }
+      wave_sync();
+    }
+  } else {
+#pragma unroll
+    for (int h = NLEV - 1; h >= 0; --h) {
+      const unsigned int pt = pk[h];
+      if (pt != 0xffffffffu) {
+        const int q = pt & 0x3ff, p = pt >> 10;
+        const C2<T> dvp = DV[p];
+        const Cpl c = CF[p];
+        C2<T> y = W[q];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const T cs = T(c.c[s]);
+          const C2<T> z = W[q + st[s]];
+          y.x -= cs * z.x;
+          y.y -= cs * z.y;
+        }
+        W[q] = cmul(dvp, y);
+      }
+      wave_sync();
+    }
+  }
+  // ---- write the tile back: forward y, backward z
+  C2<T>* __restrict__ dst = a.out + o;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    if (!(col_ok && u < ez)) continue;
+    dst[e0 + u * kst] = FWD ? DV[sl[u]] : W[rb_box(li0, lj0, u)];
   }
 }
 

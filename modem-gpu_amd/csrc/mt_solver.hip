@@ -335,6 +335,11 @@ static int build_tile_plan(mt_ctx* ctx, TilePlan& tp, int nlat, const LatticeInf
   for (int h = 0; h < sh.nlev; ++h)
     if (sh.lvl[h + 1] - sh.lvl[h] > 64)
       return mt_fail(ctx, MT_E_STATE, "tile hyperplane wider than a wave");
+  if (tp.tx == 8 && tp.ty == 8 && tp.tz == 8)
+    for (int h = 0; h < sh.nlev; ++h)
+      for (int l = 0; l < 64; ++l)
+        sh.lpts[h][l] = sh.lvl[h] + l < sh.lvl[h + 1] ? sh.rpts[sh.lvl[h] + l] : 0xffffffffu;
+
   MT_HIP_CHECK(ctx, hipMalloc(&tp.tiles, sizeof(int4) * std::max<size_t>(1, flat.size())));
   MT_HIP_CHECK(ctx, hipMalloc(&tp.shape, sizeof(TileShape)));
   MT_HIP_CHECK(ctx, hipMemcpy(tp.tiles, flat.data(), sizeof(int4) * flat.size(), hipMemcpyHostToDevice));
@@ -590,7 +595,7 @@ static void dilu_fwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
   for (int L = 0; L < tp.nlev; ++L) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
-    k_dilu_tile<T, V, 8, 8, 8, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    k_dilu_ro<T, V, true><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
   }
 }
 
@@ -600,7 +605,7 @@ static void dilu_bwd(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
   for (int L = tp.nlev - 1; L >= 0; --L) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
-    k_dilu_tile<T, V, 8, 8, 8, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
+    k_dilu_ro<T, V, false><<<a.ntiles * a.nact, 64, 0, ctx->stream>>>(a, tp.shape);
   }
 }
 
