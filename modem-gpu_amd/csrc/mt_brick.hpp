@@ -411,18 +411,19 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
     if (BWDP) CH[sl] = h;
   }
   __syncthreads();   // s_ tables
-  // this lane's point on every hyperplane of the tile (at most 48 per plane)
+  // this lane's point on every hyperplane of the tile (at most 48 per plane),
+  // formed before the sweeps: box index | level slot << 10, or ~0u (no point,
+  // or past a partial tile's extent)
   unsigned int pk[NLEV];
-  {
-    int lv[NLEV + 1];
 #pragma unroll
-    for (int h = 0; h <= NLEV; ++h) lv[h] = shp->lvl[h];
+  for (int h = 0; h < NLEV; ++h) pk[h] = shp->lpts[h][lane];
 #pragma unroll
-    for (int h = 0; h < NLEV; ++h) {
-      const int idx = lv[h] + lane;
-      const unsigned int v = shp->rpts[min(idx, NP - 1)];
-      pk[h] = idx < lv[h + 1] ? v : 0xffffffffu;
-    }
+  for (int h = 0; h < NLEV; ++h) {
+    const unsigned int v = pk[h];
+    const int li = v & 7, lj = (v >> 3) & 7, lk = (v >> 6) & 7;
+    const bool ok = v != 0xffffffffu && li < ex && lj < ey && lk < ez;
+    pk[h] = ok ? (unsigned(rb_box(li, lj, lk)) | ((v >> 9) << 10)) : 0xffffffffu;
+    asm volatile("" : "+v"(pk[h]));   // formed here, not per sweep step
   }
   const bool mine = __builtin_amdgcn_readfirstlane(s_ok[wv]) != 0;   // 0 past nact
   const int sys = __builtin_amdgcn_readfirstlane(s_sys[wv]);
@@ -562,57 +563,69 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
       wave_sync();
     }
   }
+  // The sweeps take no masks: a red forward step's low neighbours outside the
+  // tile are zero cells of the box (red forward stages no faces); a black
+  // point uses every lower coupling; before the black backward sweep the
+  // low-face cells are cleared, so that every upper neighbour outside the
+  // tile (an i-high cell is the next row's i-low cell, a j-high cell the next
+  // plane's j-low cells, a k-high index wraps to the k-low plane) reads zero;
+  // red backward uses every upper coupling with the high faces in the box.
+  // A neighbour past a partial tile's extent is a pad point: zero.
   if (FWDP) {
 #pragma unroll
     for (int h = 0; h < NLEV; ++h) {
       const unsigned int pt = pk[h];
-      const int li = pt & 7, lj = (pt >> 3) & 7, lk = (pt >> 6) & 7;
-      if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
-        const int p = pt >> 9;   // level slot
-        const int q = rb_box(li, lj, lk);
+      if (pt != 0xffffffffu) {
+        const int q = pt & 0x3ff, p = pt >> 10;
         const C2<T> dvp = DV[p];
         const Cpl c4 = CL[p];
-        // red: face neighbours are upper, so only in-tile lower couplings;
-        // black: every lower-side coupling (in tile or across the face)
-        const bool in[3] = {li > 0, lj > 0, lk > 0};
         C2<T> y = W[q];
+        C2<T> v[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) v[s] = W[q - st[s]];
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-          const T c = (MODE == RB_BLACK_FB || in[s]) ? T(c4.c[s]) : T(0);
-          const C2<T> v = W[q - st[s]];
-          y.x -= c * v.x;
-          y.y -= c * v.y;
+          const T c = T(c4.c[s]);
+          y.x -= c * v[s].x;
+          y.y -= c * v[s].y;
         }
         W[q] = cmul(dvp, y);
       }
       wave_sync();
     }
   }
+  if (MODE == RB_BLACK_FB) {
+    const int fa = lane & 7, fb = lane >> 3;
+    W[rb_box(0, fa, fb) - 1] = {T(0), T(0)};
+    W[rb_box(fa, 0, fb) - kRbBX] = {T(0), T(0)};
+    W[rb_box(fa, fb, 0) - kRbPK] = {T(0), T(0)};
+    wave_sync();
+  }
   if (BWDP) {
 #pragma unroll
     for (int h = NLEV - 1; h >= 0; --h) {
       const unsigned int pt = pk[h];
-      const int li = pt & 7, lj = (pt >> 3) & 7, lk = (pt >> 6) & 7;
-      if (pt != 0xffffffffu && li < ex && lj < ey && lk < ez) {
-        const int p = pt >> 9;   // level slot
-        const int q = rb_box(li, lj, lk);
+      if (pt != 0xffffffffu) {
+        const int q = pt & 0x3ff, p = pt >> 10;
         const C2<T> dvp = DV[p];
         const Cpl h4 = CH[p];
-        // black: upper neighbours in the tile only; red: in tile or across
-        const bool up[3] = {li + 1 < ex, lj + 1 < ey, lk + 1 < ez};
+        const C2<T> b = W[q];
+        C2<T> z[3];
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          // q + st < 2 NB: the index past the box wraps to its start
+          const unsigned int qn = unsigned(q + st[s]);
+          z[s] = W[s == 0 ? qn : min(qn, qn - unsigned(NB))];
+        }
+        // y = -(sum of the upper terms); the box holds w = dv*y_forward
         C2<T> y = {T(0), T(0)};
 #pragma unroll
         for (int s = 0; s < 3; ++s) {
-          const T c = (MODE == RB_RED_BWD || up[s]) ? T(h4.c[s]) : T(0);
-          int qn = q + st[s];
-          if (qn >= NB) qn -= NB;
-          const C2<T> z = W[qn];
-          y.x -= c * z.x;
-          y.y -= c * z.y;
+          const T c = T(h4.c[s]);
+          y.x -= c * z[s].x;
+          y.y -= c * z[s].y;
         }
-        // y holds -(sum of the upper terms); the box holds w = dv*y_forward
         const C2<T> d = cmul(dvp, y);
-        const C2<T> b = W[q];
         W[q] = {b.x + d.x, b.y + d.y};
       }
       wave_sync();
