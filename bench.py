@@ -175,9 +175,11 @@ def main():
                     help="SpMV kernel of the solves: 0 row, 1 k-marching (FP32), 3 multi-system row")
     ap.add_argument("--spmv-systems", type=int, default=4,
                     help="systems per block of the multi-system row SpMV (4 or 8)")
-    ap.add_argument("--dilu-variant", type=int, default=5, choices=(1, 4, 5),
+    ap.add_argument("--dilu-variant", type=int, default=6, choices=(1, 4, 5, 6),
                     help="red-black DILU launch shape: 1 single-system persistent, 4 four systems "
-                         "per workgroup, 5 the same with the lean LDS layout")
+                         "per workgroup, 5 the same with the lean LDS layout, 6 (inner solves on "
+                         "bricks) persistent four-system workgroups loading the next item while "
+                         "they sweep")
     ap.add_argument("--drop-in-only", type=int, default=0, metavar="NSYS",
                     help="only the plain FP64 solves (one system at a time, reference-ordered "
                          "DILU: the cuBiCG drop-in) of the first NSYS forward systems, for profiling")
@@ -436,10 +438,13 @@ def main():
     nl, kms, kby = kt[dom]
     rb_kernel = {4: "k_dilu_rbm<float,float,RED_FWD|BLACK_FB|RED_BWD,4>",
                  5: "k_dilu_rbl<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (lean LDS)",
+                 6: "k_dilu_rbl<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (lean LDS)",
                  1: "k_dilu_rb2<float,float,RED_FWD|BLACK_FB|RED_BWD>"}[args.dilu_variant]
     on_bricks = bricks and args.precond == 3 and args.spmv_variant in (None, 3) and args.spmv_systems == 4
     if on_bricks:
-        rb_kernel = "k_dilu_bk<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (on bricks)"
+        rb_kernel = ("k_dilu_bkp<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (on bricks, persistent, "
+                     "next item's loads in flight during the sweeps)" if args.dilu_variant == 6 else
+                     "k_dilu_bk<float,float,RED_FWD|BLACK_FB|RED_BWD,4> (on bricks)")
     pc_kernels = {0: "k_dilu_tile<float,float,8,8,8,true|false> (block-DILU level sweeps)",
                   3: rb_kernel + " (red-black tile block-DILU, 3 launches per application)"}
     spv = 3 if args.spmv_variant is None else args.spmv_variant

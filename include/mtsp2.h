@@ -388,9 +388,12 @@ int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
 /* Launch shape of the red-black DILU (MT_PC_DILU_RB) sweeps: 1 = one system
  * per single-wave persistent workgroup with the next item prefetched into
  * registers; 4 = four systems of one tile per workgroup sharing the tile's
- * couplings in LDS; 5 (default) = the same with the lean LDS layout (three
- * workgroups per CU).  The same operator; 5 rounds the black tiles' backward
- * sweep differently.  No counterpart in the reference. */
+ * couplings in LDS; 5 = the same with the lean LDS layout (three workgroups
+ * per CU); 6 (default) = for the inner solves on bricks, persistent
+ * four-system workgroups (three per CU) that load their next item while
+ * they sweep the current one (elsewhere as 5).  The same operator; 5 and 6
+ * round the black tiles' backward sweep differently from 1 and 4 (6 is
+ * bitwise 5).  No counterpart in the reference. */
 int mt_set_dilu_variant(mt_ctx *ctx, int variant);
 /* Layout of the mixed solver's inner FP32 vectors when they are
  * preconditioned by the red-black DILU with the default SpMV: 1 (default) =

@@ -645,4 +645,319 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
   }
 }
 
+// The same three launches with persistent workgroups: a workgroup walks its
+// XCD's items (tile, group of NW systems) with a stride, and while the waves
+// sweep one item out of LDS, the loads of the next item (right-hand sides,
+// pivots, couplings, faces; the system records and the tile before them) are
+// in flight into registers.  The sweeps leave the loads little to overlap
+// with otherwise: a workgroup's tile data arrive in one burst and the memory
+// pipe idles while its waves sweep.  The arithmetic is k_dilu_bk's.
+template <class T, class V, int MODE, int NW>
+__global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const TileShape* shp) {
+  constexpr int TX = 8, NP = kRbNP, NU = 8, NLEV = 22, NB = kRbNB;
+  constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
+  constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
+  constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
+  constexpr int CPT = NP / (64 * NW);         // tile points per thread for the couplings
+  static_assert(CPT * 64 * NW == NP, "couplings: whole points per thread");
+  struct Cpl { V c[3]; };
+  __shared__ C2<T> Wall[NW][NB];
+  __shared__ C2<T> DVall[NW][NP];
+  __shared__ Cpl CL[FWDP ? NP : 1];
+  __shared__ Cpl CH[BWDP ? NP : 1];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ngrp = (a.nact + NW - 1) / NW;
+  const int nwork = a.ntiles * ngrp;
+  const int xcd = blockIdx.x & 7, nb8 = gridDim.x >> 3;
+  const int chunk = (nwork + 7) / 8;
+  const int wend = min(nwork, (xcd + 1) * chunk);
+  int w = xcd * chunk + (blockIdx.x >> 3);
+  if (w >= wend) return;   // uniform over the workgroup
+  C2<T>* __restrict__ W = Wall[wv];
+  C2<T>* __restrict__ DV = DVall[wv];
+  const V* __restrict__ cfs = a.coef;
+  const int64_t cst = a.cstride;
+  const int st[3] = {1, kRbBX, kRbPK};
+  const int fa = lane & 7, fb = lane >> 3;
+  // constant over the items: the lane's point of every hyperplane (box index
+  // | level slot << 10 | li << 19 | lj << 22 | lk << 25, or ~0u), the level
+  // slots of its staging points and of its coupling points
+  unsigned int pk[NLEV];
+#pragma unroll
+  for (int h = 0; h < NLEV; ++h) {
+    const unsigned int v = shp->lpts[h][lane];
+    const unsigned int li = v & 7, lj = (v >> 3) & 7, lk = (v >> 6) & 7;
+    pk[h] = v == 0xffffffffu ? v
+                             : (unsigned(rb_box(li, lj, lk)) | ((v >> 9) << 10) | (li << 19) | (lj << 22) |
+                                (lk << 25));
+  }
+  int sl[NU], csl[CPT];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) sl[u] = shp->rslot[lane + 64 * u];
+#pragma unroll
+  for (int c = 0; c < CPT; ++c) csl[c] = shp->rslot[tid + 64 * NW * c];
+  // ---- the prefetched item
+  struct Geo {
+    int ok, sys, fidx, tself, ex, ey, ez;
+    C2<T> al;
+  };
+  Geo nx;
+  C2<T> rv[NU], v2[NU], dd[NU], fv[6];
+  V cp[CPT][6];
+  T fc[3];
+  auto load = [&](int wi) {
+    const int tix = wi / ngrp, grp = wi - tix * ngrp;
+    const int si = grp * NW + wv;
+    nx.ok = 0; nx.sys = 0; nx.fidx = 0;
+    nx.al = {T(0), T(0)};
+    if (si < a.nact) {
+      nx.sys = a.act[si];
+      const SysScal& sc = a.sc[nx.sys];
+      nx.ok = sc.status == a.status_req;
+      nx.fidx = sc.fidx;
+      nx.al = ccast<T>(sc.alpha);
+    }
+    const int4 tl = a.tiles[tix];
+    const LatticeInfo L = a.lat[tl.x];
+    const int gi0 = tl.y * TX, gj0 = tl.z * TX, gk0 = tl.w * TX;
+    nx.ex = min(TX, L.A - gi0);
+    nx.ey = min(TX, L.B - gj0);
+    nx.ez = min(TX, L.C - gk0);
+    nx.tself = L.tbase + tl.y + L.ntx * (tl.z + L.nty * tl.w);
+    const int tb = nx.tself * kBkPts;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const int p = tid + 64 * NW * c;
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        cp[c][s] = FWDP ? cfs[(int64_t)(2 * s) * cst + tb + p] : V(0);
+        cp[c][3 + s] = BWDP ? cfs[(int64_t)(2 * s + 1) * cst + tb + p] : V(0);
+      }
+    }
+    if (!nx.ok) return;
+    const int64_t o = (int64_t)nx.sys * a.vstride;
+    const C2<T>* __restrict__ pin = (FWDP ? a.in : a.out) + o + tb;
+    const C2<T>* __restrict__ pin2 = a.in2 ? a.in2 + o + tb : nullptr;
+    const C2<T>* __restrict__ dinv = a.dinv + (int64_t)nx.fidx * cst + tb;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = lane + 64 * u;
+      rv[u] = pin[p];   // points past a partial tile's extent hold zeros
+      v2[u] = (FWDP && pin2) ? pin2[p] : C2<T>{T(0), T(0)};
+      dd[u] = dinv[p];
+    }
+    if (HALO) {
+      const C2<T>* __restrict__ pout = a.out + o;
+      const C2<T>* __restrict__ xfi =
+          a.xf + (MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)nx.sys * a.xfsys;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int dir = f >> 1;
+        const bool hi = f & 1;
+        int ti, tj, tk;   // tile point next to the face
+        if (dir == 0) { ti = hi ? 7 : 0; tj = fa; tk = fb; }
+        else if (dir == 1) { ti = fa; tj = hi ? 7 : 0; tk = fb; }
+        else { ti = fa; tj = fb; tk = hi ? 7 : 0; }
+        const int ni = gi0 + ti + (dir == 0 ? (hi ? 1 : -1) : 0);
+        const int nj = gj0 + tj + (dir == 1 ? (hi ? 1 : -1) : 0);
+        const int nk = gk0 + tk + (dir == 2 ? (hi ? 1 : -1) : 0);
+        fv[f] = {T(0), T(0)};
+        const bool pin_ok = ti < nx.ex && tj < nx.ey && tk < nx.ez;
+        const bool ok = pin_ok && ni >= 0 && nj >= 0 && nk >= 0 && ni < L.A && nj < L.B && nk < L.C;
+        if (ok) {
+          const int d = hi ? 1 : -1;
+          const int tn = L.tbase + (tl.y + (dir == 0 ? d : 0)) +
+                         L.ntx * ((tl.z + (dir == 1 ? d : 0)) + L.nty * (tl.w + (dir == 2 ? d : 0)));
+          if (dir == 0) {
+            fv[f] = xfi[(int64_t)tn * kRbFaces + (hi ? 0 : 1) * 64 + lane];
+          } else {
+            const int fl = hi ? 0 : TX - 1;   // the neighbour's facing layer
+            const int pn = dir == 1 ? ti + TX * (fl + TX * tk) : ti + TX * (tj + TX * fl);
+            fv[f] = pout[(int64_t)tn * kBkPts + pn];
+          }
+        }
+        if (MODE == RB_RED_BWD && !hi) {
+          fc[dir] = T(0);
+          if (pin_ok) fc[dir] = T(cfs[(int64_t)(2 * dir) * cst + tb + ti + TX * (tj + TX * tk)]);
+        }
+      }
+    }
+  };
+  load(w);
+  while (true) {
+    // the point table is loop-invariant: kept opaque, so that what the sweeps
+    // derive from it (addresses of 22 levels) is not hoisted out of the loop
+    // into registers
+#pragma unroll
+    for (int h = 0; h < NLEV; ++h) asm volatile("" : "+v"(pk[h]));
+    const Geo cu = nx;
+    const int ex = cu.ex, ey = cu.ey, ez = cu.ez;
+    const bool mine = __builtin_amdgcn_readfirstlane(cu.ok) != 0;
+    __syncthreads();   // the previous item's sweeps and write-back are done
+    // ---- stage: the shared couplings, per wave the box and the pivots
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      if (FWDP) CL[csl[c]] = Cpl{{cp[c][0], cp[c][1], cp[c][2]}};
+      if (BWDP) CH[csl[c]] = Cpl{{cp[c][3], cp[c][4], cp[c][5]}};
+    }
+    C2<T> fold_v[3];
+    T fold_c[3];
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      fold_v[s] = {T(0), T(0)};
+      fold_c[s] = MODE == RB_RED_BWD ? fc[s] : T(0);
+    }
+    if (mine) {
+      for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
+      wave_sync();
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        C2<T> r = rv[u];
+        if (FWDP && a.in2) r = csub(r, cmul(cu.al, v2[u]));
+        W[rb_box(fa, fb & 7, u)] = r;
+        DV[sl[u]] = dd[u];
+      }
+      if (HALO) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          const int dir = f >> 1;
+          const bool hi = f & 1;
+          int ti, tj, tk;
+          if (dir == 0) { ti = hi ? 7 : 0; tj = fa; tk = fb; }
+          else if (dir == 1) { ti = fa; tj = hi ? 7 : 0; tk = fb; }
+          else { ti = fa; tj = fb; tk = hi ? 7 : 0; }
+          const bool inbox = (MODE == RB_BLACK_FB) ? !hi : hi;
+          if (!inbox) {
+            fold_v[dir] = fv[f];
+            continue;
+          }
+          if (!(ti < ex && tj < ey && tk < ez)) continue;
+          const int qp = rb_box(ti, tj, tk);
+          int qs = hi ? qp + st[dir] : qp - st[dir];
+          if (qs >= NB) qs -= NB;
+          W[qs] = fv[f];
+        }
+      }
+    }
+    // ---- the next item's loads, in flight during this item's sweeps
+    const int wn = w + nb8;
+    const bool more = wn < wend;
+    if (more) load(wn);
+    __syncthreads();   // the shared couplings
+    if (mine) {
+      const int64_t o = (int64_t)cu.sys * a.vstride;
+      // fold the face terms the sweeps do not read (k_dilu_bk)
+      if (HALO) {
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+          const int t0 = (MODE == RB_BLACK_FB) ? TX - 1 : 0;
+          int ti, tj, tk;
+          if (s == 0) { ti = t0; tj = fa; tk = fb; }
+          else if (s == 1) { ti = fa; tj = t0; tk = fb; }
+          else { ti = fa; tj = fb; tk = t0; }
+          if (ti < ex && tj < ey && tk < ez) {
+            const int q = rb_box(ti, tj, tk);
+            const int sq = shp->rslot[ti + TX * (tj + TX * tk)];
+            const T c = (MODE == RB_BLACK_FB) ? T(CH[sq].c[s]) : fold_c[s];
+            const C2<T> v = fold_v[s];
+            C2<T> r = W[q];
+            if (MODE == RB_BLACK_FB) {
+              r.x -= c * v.x;
+              r.y -= c * v.y;
+            } else {
+              const C2<T> d = cmul(DV[sq], C2<T>{c * v.x, c * v.y});
+              r.x -= d.x;
+              r.y -= d.y;
+            }
+            W[q] = r;
+          }
+          wave_sync();
+        }
+      }
+      const bool full = ex == TX && ey == TX && ez == TX;
+      auto live = [&](unsigned int pt) {
+        return pt != 0xffffffffu &&
+               (full || (int((pt >> 19) & 7) < ex && int((pt >> 22) & 7) < ey && int((pt >> 25) & 7) < ez));
+      };
+      if (FWDP) {
+#pragma unroll
+        for (int h = 0; h < NLEV; ++h) {
+          const unsigned int pt = pk[h];
+          if (live(pt)) {
+            const int q = pt & 0x3ff, p = (pt >> 10) & 0x1ff;
+            const C2<T> dvp = DV[p];
+            const Cpl c4 = CL[p];
+            C2<T> y = W[q];
+            C2<T> v[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) v[s] = W[q - st[s]];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+              const T c = T(c4.c[s]);
+              y.x -= c * v[s].x;
+              y.y -= c * v[s].y;
+            }
+            W[q] = cmul(dvp, y);
+          }
+          wave_sync();
+        }
+      }
+      if (MODE == RB_BLACK_FB) {
+        W[rb_box(0, fa, fb) - 1] = {T(0), T(0)};
+        W[rb_box(fa, 0, fb) - kRbBX] = {T(0), T(0)};
+        W[rb_box(fa, fb, 0) - kRbPK] = {T(0), T(0)};
+        wave_sync();
+        // nothing the forward sweep derived from the table is kept for the
+        // backward one (registers for the prefetched item)
+#pragma unroll
+        for (int h = 0; h < NLEV; ++h) asm volatile("" : "+v"(pk[h]));
+      }
+      if (BWDP) {
+#pragma unroll
+        for (int h = NLEV - 1; h >= 0; --h) {
+          const unsigned int pt = pk[h];
+          if (live(pt)) {
+            const int q = pt & 0x3ff, p = (pt >> 10) & 0x1ff;
+            const C2<T> dvp = DV[p];
+            const Cpl h4 = CH[p];
+            const C2<T> b = W[q];
+            C2<T> z[3];
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+              const unsigned int qn = unsigned(q + st[s]);
+              z[s] = W[s == 0 ? qn : min(qn, qn - unsigned(NB))];
+            }
+            C2<T> y = {T(0), T(0)};
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+              const T c = T(h4.c[s]);
+              y.x -= c * z[s].x;
+              y.y -= c * z[s].y;
+            }
+            const C2<T> d = cmul(dvp, y);
+            W[q] = {b.x + d.x, b.y + d.y};
+          }
+          wave_sync();
+        }
+      }
+      // ---- write the item back (red forward: w; otherwise z) and the
+      // packed i-faces the other colour reads next
+      C2<T>* __restrict__ dst = a.out + o + (int64_t)cu.tself * kBkPts;
+      C2<T>* __restrict__ xfo = a.xf + (MODE == RB_RED_FWD ? 0 : a.xfhalf) +
+                                (int64_t)cu.sys * a.xfsys + (int64_t)cu.tself * kRbFaces;
+#pragma unroll
+      for (int u = 0; u < NU; ++u) {
+        const int p = lane + 64 * u;
+        if (!(fa < ex && fb < ey && u < ez)) continue;
+        const C2<T> zv = W[rb_box(fa, fb, u)];
+        dst[p] = zv;
+        if (MODE != RB_RED_BWD && (fa == 0 || fa == TX - 1)) xfo[(fa == 0 ? 0 : 64) + fb + 8 * u] = zv;
+      }
+    }
+    if (!more) break;
+    w = wn;
+  }
+}
+
 }  // namespace mt

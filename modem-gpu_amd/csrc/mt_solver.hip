@@ -154,9 +154,11 @@ struct mt_ctx {
   // the sens_* workspaces
   bool sens_fields = false;
   bool sens_geom = false;
-  int rb_variant = 5;           // red-black DILU: 1 persistent single-system
+  int rb_variant = 6;           // red-black DILU: 1 persistent single-system
                                 // workgroups; 4 four systems per workgroup;
-                                // 5 the same in the lean LDS layout
+                                // 5 the same in the lean LDS layout; 6 (on
+                                // bricks) persistent four-system workgroups
+                                // loading the next item while they sweep
   // kernel timing (mt_timing_enable): HIP events on `stream` around every
   // SpMV launch and every DILU sweep of the BiCGStab engine
   bool timing = false;
@@ -631,11 +633,11 @@ static void dilu_rb_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
     a.tiles = tp.tiles + tp.lvl_off[L];
     a.ntiles = tp.lvl_off[L + 1] - tp.lvl_off[L];
     if (a.ntiles == 0) continue;
-    if (ctx->rb_variant == 4 || ctx->rb_variant == 5) {
+    if (ctx->rb_variant >= 4) {
       constexpr int NW = 4;
       const int64_t nwork = int64_t(a.ntiles) * ((a.nact + NW - 1) / NW);
       const int g = int(8 * ((nwork + 7) / 8));
-      if (ctx->rb_variant == 5) {
+      if (ctx->rb_variant >= 5) {
         if (step == 0) k_dilu_rbl<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
         else if (step == 1) k_dilu_rbl<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
         else k_dilu_rbl<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
@@ -672,6 +674,14 @@ static void dilu_bk_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
     if (a.ntiles == 0) continue;
     const int64_t nwork = int64_t(a.ntiles) * ((a.nact + NW - 1) / NW);
     const int g = int(8 * ((nwork + 7) / 8));
+    if (ctx->rb_variant == 6) {
+      // persistent: three workgroups per CU (their LDS), a multiple of 8 XCDs
+      const int gp = std::min(g, std::max(8, ctx->num_cu * 3 / 8 * 8));
+      if (step == 0) k_dilu_bkp<T, V, RB_RED_FWD, NW><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      else if (step == 1) k_dilu_bkp<T, V, RB_BLACK_FB, NW><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      else k_dilu_bkp<T, V, RB_RED_BWD, NW><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      continue;
+    }
     if (step == 0) k_dilu_bk<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
     else if (step == 1) k_dilu_bk<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
     else k_dilu_bk<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
@@ -2292,8 +2302,8 @@ int mt_bench_precond(mt_ctx* ctx, int nsys, int precision, int precond, int reps
 }
 
 int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
-  if (!ctx || (variant != 1 && variant != 4 && variant != 5))
-    return mt_fail(ctx, MT_E_ARG, "mt_set_dilu_variant: variant is 1, 4 or 5");
+  if (!ctx || (variant != 1 && variant != 4 && variant != 5 && variant != 6))
+    return mt_fail(ctx, MT_E_ARG, "mt_set_dilu_variant: variant is 1, 4, 5 or 6");
   ctx->rb_variant = variant;
   return MT_OK;
 }

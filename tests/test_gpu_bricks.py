@@ -69,6 +69,32 @@ def test_brick_dilu_inverts_its_factorisation(gpu_ctx, transpose):
             assert rel(z, z64) <= 2e-5, f
 
 
+def test_brick_dilu_persistent_matches(gpu_ctx):
+    """The persistent brick DILU launches (variant 6: a workgroup walks its
+    items and loads the next one while it sweeps) do k_dilu_bk's arithmetic:
+    one application and batched solves are bitwise equal."""
+    print("case brick dilu persistent", flush=True)
+    o, g = _grid(gpu_ctx, 44)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal(o.nInt) + 1j * rng.standard_normal(o.nInt)
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [1e-2, 0.5, 20.0])
+    res = {}
+    try:
+        for v in (5, 6):
+            gpu_ctx.set_dilu_variant(v)
+            z = [gpu_ctx.inner_apply(2 * np.pi * 0.7, x, what=1, transpose=t) for t in (False, True)]
+            X, st = gpu_ctx.solve_batch(om, B, X0, tol=1e-10, maxit=4000, precision=mtsp2.PREC_MIXED,
+                                        precond=mtsp2.PC_DILU_RB)
+            res[v] = (z, X, [(s["niter"], s["nouter"]) for s in st])
+    finally:
+        gpu_ctx.set_dilu_variant(6)   # the library's default
+    for t in range(2):
+        assert np.array_equal(res[5][0][t], res[6][0][t]), t
+    assert res[5][2] == res[6][2]
+    assert np.array_equal(res[5][1], res[6][1])
+
+
 @pytest.mark.parametrize("transpose", [False, True])
 def test_brick_solves_match_all_edge_layout(gpu_ctx, transpose):
     """Mixed batched solves (red-black DILU) with the inner vectors on

@@ -183,7 +183,7 @@ def dilu_variant(gpu_ctx, request):
     """Launch shape of the red-black DILU for one test (restored after)."""
     gpu_ctx.set_dilu_variant(request.param)
     yield request.param
-    gpu_ctx.set_dilu_variant(5)   # the library's default
+    gpu_ctx.set_dilu_variant(6)   # the library's default
 
 
 @pytest.mark.parametrize("dilu_variant", [4, 5, 1], indirect=True)
@@ -224,7 +224,7 @@ def test_dilu_rb_lean_layout_matches(gpu_ctx):
             gpu_ctx.set_dilu_variant(v)
             zs.append(gpu_ctx.precond_apply(2 * np.pi * 0.3, x, mtsp2.PC_DILU_RB))
     finally:
-        gpu_ctx.set_dilu_variant(5)   # the library's default
+        gpu_ctx.set_dilu_variant(6)   # the library's default
     assert rel(zs[1], zs[0]) <= 1e-13
 
 
