@@ -221,7 +221,8 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
   const int s0 = grp * sg, nact = min(a.nact, s0 + sg);
   for (int q = tid; q < kBtWaves * kBtMaxSys * NQ; q += kBtThreads) (&red[0][0][0])[q] = 0.0;
   __syncthreads();
-  const bool fz = WHICH == 2 && a.fused;
+  // the <t,s> partner: s stored by the preconditioner, else r - alpha v
+  const bool fz = WHICH == 2 && a.fused && !a.s_ok;
   int t0, t1;
   bk_tile_range3(bg, a.G, g, t0, t1);
   for (int t = t0; t < t1; ++t) {
@@ -501,7 +502,10 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
       const int p = lane + 64 * u;
       const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
       C2<T> r = rv[u];
-      if (FWDP && pin2) r = csub(r, cmul(al, v2[u]));
+      if (FWDP && pin2) {
+        r = csub(r, cmul(al, v2[u]));
+        if (a.sout) a.sout[o + tb + p] = r;
+      }
       W[rb_box(li, lj, lk)] = r;
       DV[shp->rslot[p]] = dd[u];
     }
@@ -811,10 +815,15 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
     if (mine) {
       for (int q = lane; q < NB; q += 64) W[q] = {T(0), T(0)};
       wave_sync();
+      C2<T>* __restrict__ so =
+          a.sout ? a.sout + (int64_t)cu.sys * a.vstride + (int64_t)cu.tself * kBkPts : nullptr;
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
         C2<T> r = rv[u];
-        if (FWDP && a.in2) r = csub(r, cmul(cu.al, v2[u]));
+        if (FWDP && a.in2) {
+          r = csub(r, cmul(cu.al, v2[u]));
+          if (so) so[lane + 64 * u] = r;
+        }
         W[rb_box(fa, fb & 7, u)] = r;
         DV[sl[u]] = dd[u];
       }

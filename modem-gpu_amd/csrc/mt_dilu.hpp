@@ -91,6 +91,9 @@ struct DiluArgs {
   const SysScal* sc;
   const C2<T>* in;         // forward: right-hand side
   const C2<T>* in2;        // forward, when set: right-hand side in - alpha*in2
+  // brick red-black sweeps with in2: in - alpha*in2 of every point also stored
+  // here (the BiCGStab s, which the SpMV's <t,s> and the r update then read)
+  C2<T>* sout = nullptr;
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed

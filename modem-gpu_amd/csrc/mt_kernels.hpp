@@ -49,6 +49,9 @@ struct IterArgs {
   // best iterate): s = r - alpha v is never stored; the preconditioner, the
   // <t,s> dot and k_rupdate_fused recompute it
   int fused = 0;
+  // s_ok = 1 (fused, brick red-black DILU): the preconditioner's first touch
+  // stores s = r - alpha v in s, which the <t,s> dot and k_rupdate_fused read
+  int s_ok = 0;
 };
 
 template <class T, class V>
@@ -345,7 +348,8 @@ __global__ __launch_bounds__(BS) void k_rupdate(IterArgs<T, V> a) {
 
 // Fused form for the mixed inner solves: x += alpha ph + omega sh,
 // r = (r - alpha v) - omega t, partials of <rt,r>, <r,r>.  The same
-// operations, in the same order, as k_supdate followed by k_rupdate.
+// operations, in the same order, as k_supdate followed by k_rupdate; with
+// s_ok, s = r - alpha v is the one the preconditioner stored.
 template <class T, class V>
 __global__ __launch_bounds__(BS) void k_rupdate_fused(IterArgs<T, V> a) {
   __shared__ Z lds[2 * BS / 64];
@@ -364,12 +368,15 @@ __global__ __launch_bounds__(BS) void k_rupdate_fused(IterArgs<T, V> a) {
     const C2<T>* __restrict__ v = a.v + o;
     const C2<T>* __restrict__ t = a.t + o;
     const C2<T>* __restrict__ rt = a.rt + o;
+    const C2<T>* __restrict__ sv = a.s + o;
+    const bool sk = a.s_ok;
     int64_t lo, hi;
     chunk_range(a.nE, a.G, g, lo, hi);
 #pragma unroll 2
     for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
       x[e] = cadd(cadd(x[e], cmul(al, ph[e])), cmul(om, sh[e]));
-      const C2<T> rn = csub(csub(r[e], cmul(al, v[e])), cmul(om, t[e]));
+      const C2<T> se = sk ? sv[e] : csub(r[e], cmul(al, v[e]));
+      const C2<T> rn = csub(se, cmul(om, t[e]));
       r[e] = rn;
       const C2<T> w = rt[e];
       acc[0].x += double(w.x) * rn.x + double(w.y) * rn.y;

@@ -810,6 +810,7 @@ struct Engine {
               : dilu_args<T, V>(tp, view<V>(ctx, transpose).coef, g.nE, g.nE, a.act, a.nact, a.sc,
                                 in, out, a.dinv, ST_RUN);
     d.in2 = in2;
+    d.sout = (brick && in2) ? a.s : nullptr;
     d.xf = xf;
     d.xfsys = xfsys;
     d.xfhalf = xfhalf;
@@ -948,10 +949,11 @@ struct Engine {
     if (cnt[MT_KT_SPMV])
       ctx->kt[MT_KT_SPMV].bytes += cnt[MT_KT_SPMV] * (double(stencil_nnz(ctx)) * sv + n * sv) +
                                    2.0 * units * 3.0 * n * sc2;
-    // (the second application of an iteration reads r and v when fused)
+    // (the second application of an iteration reads r and v when fused, and
+    // on bricks writes s = r - alpha v as well)
     if (cnt[MT_KT_PRECOND])
       ctx->kt[MT_KT_PRECOND].bytes += cnt[MT_KT_PRECOND] * 6.0 * n * sv +
-                                      units * (3.0 + (a.fused ? 4.0 : 3.0)) * n * sc2;
+                                      units * (3.0 + (a.fused ? (a.s_ok ? 5.0 : 4.0) : 3.0)) * n * sc2;
     return MT_OK;
   }
   ~Engine() {
@@ -1165,6 +1167,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   a.xbest = nullptr;
   a.b = nullptr;
   a.fused = (o.precond == PCK_DILU || o.precond == PCK_DILU_RB) ? 1 : 0;
+  a.s_ok = (bricks && a.fused) ? 1 : 0;   // the brick sweeps store s
   C2<float>* dinv = nullptr;
   if ((rc = build_dinv<float>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   if (bricks) {
