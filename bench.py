@@ -185,9 +185,11 @@ def main():
                          "DILU: the cuBiCG drop-in) of the first NSYS forward systems, for profiling")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch kernels eagerly instead of replaying hipGraphs")
-    ap.add_argument("--inner-layout", default="bricks", choices=("bricks", "all-edge"),
-                    help="inner FP32 vectors of the mixed solves: 8x8x8 tiles contiguous "
-                         "(bricks) or the all-edge layout of the outer solve")
+    ap.add_argument("--inner-layout", default="bricks", choices=("bricks", "bricks-unpaired", "all-edge"),
+                    help="inner FP32 vectors of the mixed solves: 8x8x8 tiles contiguous with "
+                         "p-hat and s-hat interleaved by system pair (bricks), the same with "
+                         "every vector system-major (bricks-unpaired), or the all-edge layout "
+                         "of the outer solve")
     ap.add_argument("--brick-systems", type=int, default=8,
                     help="systems per workgroup of the brick SpMV (0: all)")
     ap.add_argument("--autotune", action="store_true",
@@ -214,8 +216,9 @@ def main():
     adjoint = not args.forward_only and cfg != "cfg3"
     ctx = mtsp2.Context(0)
     ctx.set_dilu_variant(args.dilu_variant)
-    bricks = args.inner_layout == "bricks"
-    ctx.set_inner_layout(bricks, args.brick_systems)
+    bricks = args.inner_layout != "all-edge"
+    layout = {"bricks": 2, "bricks-unpaired": 1, "all-edge": 0}[args.inner_layout]
+    ctx.set_inner_layout(layout, args.brick_systems)
     t0 = time.time()
     ctx.assemble(grid["dx"], grid["dy"], grid["dz"], grid["nzAir"], lc)
     log(f"assembled n_int={ctx.n_int} nnz={ctx.nnz} in {time.time()-t0:.1f}s")
@@ -387,7 +390,7 @@ def main():
         t0 = time.time()
         step()
         ablation["mixed_batched_red_black_all_edge_layout"] = per_step / (time.time() - t0)
-        ctx.set_inner_layout(True, args.brick_systems)
+        ctx.set_inner_layout(layout, args.brick_systems)
         log(f"mixed, all-edge inner layout: {ablation['mixed_batched_red_black_all_edge_layout']:.2f} solves/s")
     if not args.no_fp64:
         sets = [(d_b, d_x0, 0)] + ([(d_ba, d_zero, 1)] if adjoint else [])
@@ -456,6 +459,10 @@ def main():
     if on_bricks:
         spmv_kernels[3] = (f"k_spmv_bt<float,float,1|2> (v = A x on bricks, "
                            f"{args.brick_systems or 'all'} systems per block, with the dot products)")
+        if layout == 2:
+            spmv_kernels[3] = (f"k_spmv_btp<1|2> (v = A x on bricks, x of system pairs "
+                               f"interleaved: one 16-byte gather per neighbour for both, "
+                               f"{args.brick_systems or 'all'} systems per block, with the dot products)")
     kernels = {"spmv": spmv_kernels[spv], "precond": pc_kernels.get(args.precond, "preconditioner")}
     achieved = kby / (kms * 1e-3) / 1e9
     traffic = counted_traffic(dom, args)

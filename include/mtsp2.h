@@ -396,12 +396,15 @@ int mt_set_spmv_variant(mt_ctx *ctx, int variant, int layers_per_segment);
  * bitwise 5).  No counterpart in the reference. */
 int mt_set_dilu_variant(mt_ctx *ctx, int variant);
 /* Layout of the mixed solver's inner FP32 vectors when they are
- * preconditioned by the red-black DILU with the default SpMV: 1 (default) =
- * bricks (each 8x8x8 tile of the red-black plan contiguous), 0 = the all-edge
- * layout of the outer solve.  systems_per_block: systems per workgroup of the
- * brick SpMV (0 = all active systems; default 8).  The same operator and
- * preconditioner; the inner sums differ in rounding.  No counterpart in the
- * reference. */
+ * preconditioned by the red-black DILU with the default SpMV: 2 (default) =
+ * bricks (each 8x8x8 tile of the red-black plan contiguous) with the
+ * preconditioned vectors p-hat and s-hat stored pair-interleaved (systems
+ * 2m and 2m+1 side by side, so the SpMV reads both with one 16-byte gather;
+ * bitwise equal to 1); 1 = bricks, every vector system-major; 0 = the
+ * all-edge layout of the outer solve.  systems_per_block: systems per
+ * workgroup of the brick SpMV (0 = all active systems; default 8).  The same
+ * operator and preconditioner; bricks and the all-edge layout differ in the
+ * rounding of the inner sums.  No counterpart in the reference. */
 int mt_set_inner_layout(mt_ctx *ctx, int bricks, int systems_per_block);
 /* The inner solves' brick kernels on one vector, for checking them: y = A x
  * (what = 0) or y = M^-1 x (what = 1, the red-black DILU of A(omega)), of A

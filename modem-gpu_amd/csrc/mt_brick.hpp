@@ -194,6 +194,51 @@ __global__ __launch_bounds__(BS) void k_bk_widen(const C2<float>* __restrict__ x
     y[q] = {double(x[q].x), double(x[q].y)};
 }
 
+// Point p of tile t for the brick SpMV: whether it is an interior edge, its
+// brick index e, its 14 neighbours' brick indices (a neighbour that is not an
+// interior edge has a zero coupling and the row's own index) and its 16
+// couplings (the brick-ordered row-major copy: 15 slots and the omega-free
+// diagonal).
+template <class V>
+__device__ __forceinline__ void bt_point(const BrickGeom& bg, const V* __restrict__ rowc, int t,
+                                         int p, bool& valid, int& e, V (&cf)[16], int (&nb)[15]) {
+  int c, ti, tj, tk;
+  bk_tile(bg, t, c, ti, tj, tk);
+  const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
+  const int i = ti * 8 + li, j = tj * 8 + lj, k = tk * 8 + lk;
+  valid = i < bg.A[c] && j < bg.B[c] && k < bg.C[c];
+  e = t * kBkPts + p;
+  const int ee = e;
+  auto setup = [&](auto cc) {
+    constexpr int C = decltype(cc)::value;
+    const int I = i + bg.oi[C], J = j + bg.oj[C], K = k + bg.ok[C];
+#pragma unroll
+    for (int q = 1; q < 15; ++q) {
+      const NbS d = bk_nb(C, q);
+      int r;
+      if (d.cb == 0) r = bk_at<0>(bg, I + d.di, J + d.dj, K + d.dk);
+      else if (d.cb == 1) r = bk_at<1>(bg, I + d.di, J + d.dj, K + d.dk);
+      else r = bk_at<2>(bg, I + d.di, J + d.dj, K + d.dk);
+      nb[q] = r < 0 ? ee : r;
+    }
+  };
+  if (c == 0) setup(Comp<0>());
+  else if (c == 1) setup(Comp<1>());
+  else setup(Comp<2>());
+  using V4 = typename std::conditional<std::is_same<V, float>::value, float4, double2>::type;
+  constexpr int PER = sizeof(V4) / sizeof(V);
+  const V4* __restrict__ rp = reinterpret_cast<const V4*>(rowc + int64_t(e) * 16);
+#pragma unroll
+  for (int q = 0; q < 16 / PER; ++q) {
+    const V4 v4 = rp[q];   // pads hold zero couplings
+    if constexpr (PER == 4) {
+      cf[4 * q] = v4.x; cf[4 * q + 1] = v4.y; cf[4 * q + 2] = v4.z; cf[4 * q + 3] = v4.w;
+    } else {
+      cf[2 * q] = v4.x; cf[2 * q + 1] = v4.y;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ SpMV
 // y_s = A x_s on bricks with the dot products of k_spmv_ms (WHICH=1:
 // v = A ph, <rt,v>; WHICH=2: t = A sh, <t,t>, <t,s>), one block per chunk of
@@ -226,46 +271,12 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
   int t0, t1;
   bk_tile_range3(bg, a.G, g, t0, t1);
   for (int t = t0; t < t1; ++t) {
-    int c, ti, tj, tk;
-    bk_tile(bg, t, c, ti, tj, tk);
     const int p = tid;
-    const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
-    const int i = ti * 8 + li, j = tj * 8 + lj, k = tk * 8 + lk;
-    const bool valid = i < bg.A[c] && j < bg.B[c] && k < bg.C[c];
-    const int e = t * kBkPts + p;
+    bool valid;
+    int e;
     V cf[16];
     int nb[15];
-    auto setup = [&](auto cc) {
-      constexpr int C = decltype(cc)::value;
-      const int I = i + bg.oi[C], J = j + bg.oj[C], K = k + bg.ok[C];
-#pragma unroll
-      for (int q = 1; q < 15; ++q) {
-        const NbS d = bk_nb(C, q);
-        int r;
-        if (d.cb == 0) r = bk_at<0>(bg, I + d.di, J + d.dj, K + d.dk);
-        else if (d.cb == 1) r = bk_at<1>(bg, I + d.di, J + d.dj, K + d.dk);
-        else r = bk_at<2>(bg, I + d.di, J + d.dj, K + d.dk);
-        // not an interior edge: zero coupling, own address
-        nb[q] = r < 0 ? e : r;
-      }
-    };
-    if (c == 0) setup(Comp<0>());
-    else if (c == 1) setup(Comp<1>());
-    else setup(Comp<2>());
-    {
-      using V4 = typename std::conditional<std::is_same<V, float>::value, float4, double2>::type;
-      constexpr int PER = sizeof(V4) / sizeof(V);
-      const V4* __restrict__ rp = reinterpret_cast<const V4*>(rowc + int64_t(e) * 16);
-#pragma unroll
-      for (int q = 0; q < 16 / PER; ++q) {
-        const V4 v4 = rp[q];   // pads hold zero couplings
-        if constexpr (PER == 4) {
-          cf[4 * q] = v4.x; cf[4 * q + 1] = v4.y; cf[4 * q + 2] = v4.z; cf[4 * q + 3] = v4.w;
-        } else {
-          cf[2 * q] = v4.x; cf[2 * q + 1] = v4.y;
-        }
-      }
-    }
+    bt_point(bg, rowc, t, p, valid, e, cf, nb);
     // one system: y = A x at this point, its dot contributions
     auto one = [&](int si, double* dq) {
       const int sys = a.act[si];
@@ -338,6 +349,137 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
   }
 }
 
+// k_spmv_bt for the FP32 inner solves with paired x: ph (WHICH=1) or sh
+// (WHICH=2) holds the two systems of a pair interleaved, so one 16-byte
+// gather per neighbour serves both (the addressers, which bound the
+// single-system kernel, process half the gather instructions).  A block
+// takes a chunk of tiles and a group of spg entries of the pair list
+// {system 2m, system 2m+1, active index of each or -1}; a pair with one
+// system stopped reads both halves and keeps only the running one.  Per
+// system the arithmetic, its order and the partial sums (per active index,
+// over the block's tiles and waves in order) are k_spmv_bt's.
+template <int WHICH>
+__global__ __launch_bounds__(kBtThreads, 4) void k_spmv_btp(IterArgs<float, float> a, BrickGeom bg,
+                                                         const float* __restrict__ rowc,
+                                                         const int4* __restrict__ pairs, int npair,
+                                                         int spg) {
+  constexpr int NQ = WHICH == 1 ? 2 : 3;
+  __shared__ double red[kBtWaves][kBtMaxSys][NQ];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int ngrp = (npair + spg - 1) / spg;
+  const int w = blockIdx.x >> 3, grp = w % ngrp;
+  const int g = (blockIdx.x & 7) * (a.G >> 3) + w / ngrp;
+  const int p0 = grp * spg, p1 = min(npair, p0 + spg);
+  for (int q = tid; q < kBtWaves * kBtMaxSys * NQ; q += kBtThreads) (&red[0][0][0])[q] = 0.0;
+  __syncthreads();
+  // the <t,s> partner: s stored by the preconditioner, else r - alpha v
+  const bool fz = WHICH == 2 && a.fused && !a.s_ok;
+  const float4* __restrict__ xi = reinterpret_cast<const float4*>(WHICH == 1 ? a.phi : a.shi);
+  const int64_t xpair = a.ppair >> 1;   // float4 entries per pair
+  int t0, t1;
+  bk_tile_range3(bg, a.G, g, t0, t1);
+  for (int t = t0; t < t1; ++t) {
+    bool valid;
+    int e;
+    float cf[16];
+    int nb[15];
+    bt_point(bg, rowc, t, tid, valid, e, cf, nb);
+    for (int pi = p0; pi < p1; ++pi) {
+      const int4 pr = pairs[pi];   // {2m, 2m+1, active index of 2m, of 2m+1}
+      // both systems' records up front (status, omega, alpha)
+      bool ra = false, rb = false;
+      float wa = 0.f, wb = 0.f;
+      C2<float> ala = {0.f, 0.f}, alb = {0.f, 0.f};
+      if (pr.z >= 0) {
+        const SysScal& sc0 = a.sc[pr.x];
+        ra = sc0.status == ST_RUN;
+        wa = float(kIsignD * sc0.freq_omega);
+        ala = ccast<float>(sc0.alpha);
+      }
+      if (pr.w >= 0) {
+        const SysScal& sc1 = a.sc[pr.y];
+        rb = sc1.status == ST_RUN;
+        wb = float(kIsignD * sc1.freq_omega);
+        alb = ccast<float>(sc1.alpha);
+      }
+      if (!ra && !rb) continue;   // uniform over the block
+      const float4* __restrict__ x = xi + (int64_t)(pr.x >> 1) * xpair;
+      const float4 xe = x[e];
+      C2<float> y0 = cscale(cf[0], C2<float>{xe.x, xe.y});
+      C2<float> y1 = cscale(cf[0], C2<float>{xe.z, xe.w});
+#pragma unroll
+      for (int q = 1; q < 15; ++q) {
+        const float4 xv = x[nb[q]];
+        const float cq = cf[q];
+        y0.x += cq * xv.x;
+        y0.y += cq * xv.y;
+        y1.x += cq * xv.z;
+        y1.y += cq * xv.w;
+      }
+      // one system: the diagonal term, y stored (on: running), its dot
+      // contributions.  Both systems of the pair are finished whether running
+      // or not (a stopped or absent second one reads the first's partner):
+      // were the second one's work in a branch, the compiler would sink its
+      // half of every gather into it, splitting each 16-byte gather in two.
+      auto fin = [&](int sys, bool on, float wsc, C2<float> al, C2<float> y, float xr, float xim,
+                     double* dq) {
+        const float d = wsc * cf[15];
+        y.x += -d * xim;
+        y.y += d * xr;
+        const int64_t o = (int64_t)sys * a.nE;
+        if (valid && on) (WHICH == 1 ? a.v : a.t)[o + e] = y;
+        if (!valid) y = {0.f, 0.f};
+        C2<float> pw = (WHICH == 1 ? a.rt : (fz ? a.r : a.s))[o + e];
+        if (fz) pw = csub(pw, cmul(al, a.v[o + e]));
+        if (WHICH == 1) {
+          dq[0] = double(pw.x) * y.x + double(pw.y) * y.y;
+          dq[1] = double(pw.x) * y.y - double(pw.y) * y.x;
+        } else {
+          dq[0] = double(y.x) * y.x + double(y.y) * y.y;
+          dq[1] = double(y.x) * pw.x + double(y.y) * pw.y;
+          dq[2] = double(y.x) * pw.y - double(y.y) * pw.x;
+        }
+      };
+      double d0[NQ], d1[NQ];
+      fin(pr.x, ra, wa, ala, y0, xe.x, xe.y, d0);
+      fin(rb ? pr.y : pr.x, rb, wb, alb, y1, xe.z, xe.w, d1);
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        d0[q] = wave_sum63(d0[q]);
+        d1[q] = wave_sum63(d1[q]);
+      }
+      if (lane == 63) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          if (ra) red[wv][pr.z][q] += d0[q];
+          if (rb) red[wv][pr.w][q] += d1[q];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // the per-(system, chunk) partials of every active index of the group
+  for (int k = tid; k < 2 * (p1 - p0); k += kBtThreads) {
+    const int4 pr = pairs[p0 + (k >> 1)];
+    const int si = (k & 1) ? pr.w : pr.z;
+    if (si < 0) continue;
+    double v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      v[q] = 0.0;
+      for (int w2 = 0; w2 < kBtWaves; ++w2) v[q] += red[w2][si][q];
+    }
+    Z* out = a.partials + ((int64_t)si * a.G + g) * 2;
+    if (WHICH == 1) {
+      out[0] = {v[0], v[1]};
+      out[1] = {0.0, 0.0};
+    } else {
+      out[0] = {v[0], 0.0};
+      out[1] = {v[1], v[2]};
+    }
+  }
+}
+
 // ------------------------------------------------- red-black DILU on bricks
 // k_dilu_rbl's sweeps (the same LDS layout and level order) on bricked data:
 // a tile's right-hand sides, pivots and the six couplings it uses are
@@ -352,7 +494,7 @@ __global__ __launch_bounds__(kBtThreads) void k_spmv_bt(IterArgs<T, V> a, BrickG
 // z of the black tiles.
 // DiluArgs on bricks: coef = the [6][nB] couplings (slots 1..6), cstride =
 // nB, vstride = nB, dinv = [nfreq][nB].
-template <class T, class V, int MODE, int NW>
+template <class T, class V, int MODE, int NW, bool ZP = false>
 __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const TileShape* shp) {
   constexpr int TX = 8, NP = kRbNP, NU = 8, NLEV = 22, NB = kRbNB;
   constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
@@ -432,7 +574,13 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
   C2<T>* __restrict__ W = Wall[wv];
   C2<T>* __restrict__ DV = DVall[wv];
   const int64_t o = (int64_t)sys * a.vstride;
-  const C2<T>* __restrict__ pout = a.out + o;   // the other colour's results
+  // the other colour's results (red backward with zout: the black tiles' z,
+  // pair-interleaved)
+  // (ZP: the launch has a.zout)
+  constexpr bool zp = ZP && MODE != RB_RED_FWD;
+  constexpr int zs = zp ? 2 : 1;
+  const C2<T>* __restrict__ pout =
+      (MODE == RB_RED_BWD && zp) ? a.zout + (int64_t)(sys >> 1) * a.zpair + (sys & 1) : a.out + o;
   const C2<T>* __restrict__ xfi =
       a.xf + (MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)sys * a.xfsys;   // faces read
   C2<T>* __restrict__ xfw =
@@ -489,7 +637,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
           } else {
             const int fl = hi ? 0 : TX - 1;   // the neighbour's facing layer
             const int pn = dir == 1 ? ti + TX * (fl + TX * tk) : ti + TX * (tj + TX * fl);
-            fv[f] = pout[(int64_t)tn * kBkPts + pn];
+            fv[f] = pout[(MODE == RB_RED_BWD ? zs : 1) * ((int64_t)tn * kBkPts + pn)];
           }
         }
         if (MODE == RB_RED_BWD && !hi && pin_ok)
@@ -637,14 +785,15 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
   }
   // ---- write the tile back (red forward: y; otherwise z) and the packed
   // faces the other colour reads next (red forward: dinv*y; black: z)
-  C2<T>* __restrict__ dst = a.out + o + tb;
+  C2<T>* __restrict__ dst =
+      zp ? a.zout + (int64_t)(sys >> 1) * a.zpair + (sys & 1) + 2 * (int64_t)tb : a.out + o + tb;
   C2<T>* __restrict__ xfo = xfw + (int64_t)tself * kRbFaces;
   for (int p = lane; p < NP; p += 64) {
     const int li = p & 7, lj = (p >> 3) & 7, lk = p >> 6;
     if (!(li < ex && lj < ey && lk < ez)) continue;
     const int q = rb_box(li, lj, lk);
     const C2<T> zv = W[q];
-    dst[p] = zv;
+    dst[zs * p] = zv;
     if (MODE != RB_RED_BWD && (li == 0 || li == TX - 1)) xfo[(li == 0 ? 0 : 64) + lj + 8 * lk] = zv;
   }
 }
@@ -656,7 +805,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
 // in flight into registers.  The sweeps leave the loads little to overlap
 // with otherwise: a workgroup's tile data arrive in one burst and the memory
 // pipe idles while its waves sweep.  The arithmetic is k_dilu_bk's.
-template <class T, class V, int MODE, int NW>
+template <class T, class V, int MODE, int NW, bool ZP = false>
 __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const TileShape* shp) {
   constexpr int TX = 8, NP = kRbNP, NU = 8, NLEV = 22, NB = kRbNB;
   constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
@@ -752,7 +901,12 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
       dd[u] = dinv[p];
     }
     if (HALO) {
-      const C2<T>* __restrict__ pout = a.out + o;
+      // the other colour's results (red backward with zout: the black
+      // tiles' z, pair-interleaved)
+      constexpr bool zr = ZP && MODE == RB_RED_BWD;   // (ZP: the launch has a.zout)
+      const C2<T>* __restrict__ pout =
+          zr ? a.zout + (int64_t)(nx.sys >> 1) * a.zpair + (nx.sys & 1) : a.out + o;
+      constexpr int zs = zr ? 2 : 1;
       const C2<T>* __restrict__ xfi =
           a.xf + (MODE == RB_BLACK_FB ? 0 : a.xfhalf) + (int64_t)nx.sys * a.xfsys;
 #pragma unroll
@@ -778,7 +932,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
           } else {
             const int fl = hi ? 0 : TX - 1;   // the neighbour's facing layer
             const int pn = dir == 1 ? ti + TX * (fl + TX * tk) : ti + TX * (tj + TX * fl);
-            fv[f] = pout[(int64_t)tn * kBkPts + pn];
+            fv[f] = pout[zs * ((int64_t)tn * kBkPts + pn)];
           }
         }
         if (MODE == RB_RED_BWD && !hi) {
@@ -952,7 +1106,11 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
       }
       // ---- write the item back (red forward: w; otherwise z) and the
       // packed i-faces the other colour reads next
-      C2<T>* __restrict__ dst = a.out + o + (int64_t)cu.tself * kBkPts;
+      constexpr bool zp = ZP && MODE != RB_RED_FWD;
+      constexpr int zs = zp ? 2 : 1;
+      C2<T>* __restrict__ dst =
+          zp ? a.zout + (int64_t)(cu.sys >> 1) * a.zpair + (cu.sys & 1) + 2 * (int64_t)cu.tself * kBkPts
+             : a.out + o + (int64_t)cu.tself * kBkPts;
       C2<T>* __restrict__ xfo = a.xf + (MODE == RB_RED_FWD ? 0 : a.xfhalf) +
                                 (int64_t)cu.sys * a.xfsys + (int64_t)cu.tself * kRbFaces;
 #pragma unroll
@@ -960,7 +1118,7 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
         const int p = lane + 64 * u;
         if (!(fa < ex && fb < ey && u < ez)) continue;
         const C2<T> zv = W[rb_box(fa, fb, u)];
-        dst[p] = zv;
+        dst[zs * p] = zv;
         if (MODE != RB_RED_BWD && (fa == 0 || fa == TX - 1)) xfo[(fa == 0 ? 0 : 64) + fb + 8 * u] = zv;
       }
     }

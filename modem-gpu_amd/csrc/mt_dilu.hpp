@@ -94,6 +94,13 @@ struct DiluArgs {
   // brick red-black sweeps with in2: in - alpha*in2 of every point also stored
   // here (the BiCGStab s, which the SpMV's <t,s> and the r update then read)
   C2<T>* sout = nullptr;
+  // brick red-black sweeps, when set: the result z (black and red backward
+  // launches) goes to zout with the two systems of a pair interleaved, entry
+  // (sys/2)*zpair + 2*point + sys%2 (zpair = twice the points per system);
+  // the red forward sweep's w stays in out, and the red backward sweep reads
+  // the black tiles' j- and k-faces from zout
+  C2<T>* zout = nullptr;
+  int64_t zpair = 0;
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed

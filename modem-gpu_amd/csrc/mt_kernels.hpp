@@ -52,6 +52,12 @@ struct IterArgs {
   // s_ok = 1 (fused, brick red-black DILU): the preconditioner's first touch
   // stores s = r - alpha v in s, which the <t,s> dot and k_rupdate_fused read
   int s_ok = 0;
+  // inner solves on bricks with paired x: ph and sh stored with the two
+  // systems of a pair interleaved, entry (sys/2)*ppair + 2*point + sys%2
+  // (ppair = twice the points per system; 0: ph and sh system-major)
+  C2<T>* phi = nullptr;
+  C2<T>* shi = nullptr;
+  int64_t ppair = 0;
 };
 
 template <class T, class V>
@@ -363,8 +369,11 @@ __global__ __launch_bounds__(BS) void k_rupdate_fused(IterArgs<T, V> a) {
     const C2<T> al = ccast<T>(sc0.alpha), om = ccast<T>(sc0.omg);
     C2<T>* __restrict__ x = a.x + o;
     C2<T>* __restrict__ r = a.r + o;
-    const C2<T>* __restrict__ ph = a.ph + o;
-    const C2<T>* __restrict__ sh = a.sh + o;
+    // ph, sh system-major, or pair-interleaved (every second entry)
+    const int64_t po = a.ppair ? (int64_t)(sys >> 1) * a.ppair + (sys & 1) : o;
+    const int ps = a.ppair ? 2 : 1;
+    const C2<T>* __restrict__ ph = (a.ppair ? a.phi : a.ph) + po;
+    const C2<T>* __restrict__ sh = (a.ppair ? a.shi : a.sh) + po;
     const C2<T>* __restrict__ v = a.v + o;
     const C2<T>* __restrict__ t = a.t + o;
     const C2<T>* __restrict__ rt = a.rt + o;
@@ -374,7 +383,7 @@ __global__ __launch_bounds__(BS) void k_rupdate_fused(IterArgs<T, V> a) {
     chunk_range(a.nE, a.G, g, lo, hi);
 #pragma unroll 2
     for (int64_t e = lo + threadIdx.x; e < hi; e += BS) {
-      x[e] = cadd(cadd(x[e], cmul(al, ph[e])), cmul(om, sh[e]));
+      x[e] = cadd(cadd(x[e], cmul(al, ph[ps * e])), cmul(om, sh[ps * e]));
       const C2<T> se = sk ? sv[e] : csub(r[e], cmul(al, v[e]));
       const C2<T> rn = csub(se, cmul(om, t[e]));
       r[e] = rn;

@@ -130,6 +130,7 @@ struct mt_ctx {
   BrickData bk;       // bricks of the mixed solver's inner solves
   bool brick = true;  // inner solves on bricks (mixed, red-black DILU, 4-system SpMV)
   int bk_sg = 8;      // brick SpMV: systems per block (0: all active systems)
+  bool bk_pairs = true;  // bricks: p-hat, s-hat interleaved by system pair (pair SpMV)
   DivcorData dc;
   SiteData sites;
   std::map<std::string, std::pair<void*, size_t>> ws;  // named device workspaces
@@ -674,17 +675,28 @@ static void dilu_bk_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
     if (a.ntiles == 0) continue;
     const int64_t nwork = int64_t(a.ntiles) * ((a.nact + NW - 1) / NW);
     const int g = int(8 * ((nwork + 7) / 8));
-    if (ctx->rb_variant == 6) {
-      // persistent: three workgroups per CU (their LDS), a multiple of 8 XCDs
-      const int gp = std::min(g, std::max(8, ctx->num_cu * 3 / 8 * 8));
-      if (step == 0) k_dilu_bkp<T, V, RB_RED_FWD, NW><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-      else if (step == 1) k_dilu_bkp<T, V, RB_BLACK_FB, NW><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-      else k_dilu_bkp<T, V, RB_RED_BWD, NW><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-      continue;
+    // the result pair-interleaved (zout: FP32 inner solves with paired x)
+    auto launch = [&](auto zp) {
+      constexpr bool ZP = decltype(zp)::value;
+      if (ctx->rb_variant == 6) {
+        // persistent: three workgroups per CU (their LDS), a multiple of 8 XCDs
+        const int gp = std::min(g, std::max(8, ctx->num_cu * 3 / 8 * 8));
+        if (step == 0) k_dilu_bkp<T, V, RB_RED_FWD, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else if (step == 1) k_dilu_bkp<T, V, RB_BLACK_FB, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else k_dilu_bkp<T, V, RB_RED_BWD, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        return;
+      }
+      if (step == 0) k_dilu_bk<T, V, RB_RED_FWD, NW, ZP><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      else if (step == 1) k_dilu_bk<T, V, RB_BLACK_FB, NW, ZP><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+      else k_dilu_bk<T, V, RB_RED_BWD, NW, ZP><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+    };
+    if constexpr (sizeof(T) == 4) {
+      if (a.zout) {
+        launch(std::true_type());
+        continue;
+      }
     }
-    if (step == 0) k_dilu_bk<T, V, RB_RED_FWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-    else if (step == 1) k_dilu_bk<T, V, RB_BLACK_FB, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
-    else k_dilu_bk<T, V, RB_RED_BWD, NW><<<g, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+    launch(std::false_type());
   }
 }
 
@@ -781,6 +793,24 @@ struct Engine {
   BrickGeom bg{};
   const V* bk_rows = nullptr;
   const V* bk_slots = nullptr;
+  // paired x (a.ppair > 0, FP32 on bricks): the active systems' pairs
+  // {2m, 2m+1, active index of each or -1}, rebuilt with the active list
+  int4* d_pairs = nullptr;
+  std::vector<int4> h_pairs;
+  int npair = 0, g_npair = -1;
+  void set_pairs(const std::vector<int>& act) {
+    h_pairs.clear();
+    for (int si = 0; si < int(act.size()); ++si) {
+      const int s = act[si], m = s >> 1;
+      if (h_pairs.empty() || h_pairs.back().x != 2 * m) h_pairs.push_back({2 * m, 2 * m + 1, -1, -1});
+      if (s & 1) h_pairs.back().w = si;
+      else h_pairs.back().z = si;
+    }
+    npair = int(h_pairs.size());
+    // the source outlives the copy: callers synchronise before the next set
+    if (npair)
+      hipMemcpyAsync(d_pairs, h_pairs.data(), npair * sizeof(int4), hipMemcpyHostToDevice, ctx->stream);
+  }
   // timing: event pairs (family, first, second) recorded by one chunk; with a
   // graph they are recorded by the graph's event nodes on every replay
   struct TPair { int fam, e0, e1; };
@@ -800,8 +830,10 @@ struct Engine {
       dilu_bwd<T, V>(ctx, ctx->tiles, d);
     }
   }
-  // out = M^-1 (in - alpha*in2) (in2 may be null)
-  void precond(const C2<T>* in, C2<T>* out, const C2<T>* in2 = nullptr) {
+  // out = M^-1 (in - alpha*in2) (in2 may be null); with zout (bricks, paired
+  // x) the result goes to zout pair-interleaved and out holds the red tiles'
+  // intermediate w
+  void precond(const C2<T>* in, C2<T>* out, const C2<T>* in2 = nullptr, C2<T>* zout = nullptr) {
     if (a.pc != PCK_DILU && a.pc != PCK_DILU_RB) return;
     const Grid& g = ctx->hop.g;
     const TilePlan& tp = a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles;
@@ -811,6 +843,8 @@ struct Engine {
                                 in, out, a.dinv, ST_RUN);
     d.in2 = in2;
     d.sout = (brick && in2) ? a.s : nullptr;
+    d.zout = brick ? zout : nullptr;
+    d.zpair = a.ppair;
     d.xf = xf;
     d.xfsys = xfsys;
     d.xfhalf = xfhalf;
@@ -830,6 +864,16 @@ struct Engine {
       IterArgs<T, V> b = a;
       b.G = brick_chunks(bg);
       const int sg = ctx->bk_sg > 0 ? ctx->bk_sg : a.nact;
+      if constexpr (std::is_same<T, float>::value && std::is_same<V, float>::value) {
+        if (a.ppair) {
+          const int spg = std::max(1, sg / 2);
+          const int ngrp = (npair + spg - 1) / spg;
+          k_spmv_btp<WHICH><<<b.G * ngrp, kBtThreads, 0, ctx->stream>>>(b, bg, bk_rows, d_pairs,
+                                                                       npair, spg);
+          if (ctx->timing) tpairs.push_back({MT_KT_SPMV, e0, trec()});
+          return b.G;
+        }
+      }
       const int ngrp = (a.nact + sg - 1) / sg;
       k_spmv_bt<T, V, WHICH><<<b.G * ngrp, kBtThreads, 0, ctx->stream>>>(b, bg, bk_rows, sg);
       G = b.G;
@@ -853,11 +897,12 @@ struct Engine {
     const int nblk = a.nact * a.G;
     hipStream_t s = ctx->stream;
     k_pupdate<T, V><<<nblk, BS, 0, s>>>(a);
-    precond(a.p, a.ph);
+    // (paired x: p-hat to phi, the sweeps' w in ph)
+    precond(a.p, a.ph, nullptr, a.ppair ? a.phi : nullptr);
     const int g1 = spmv<1>(nblk);
     k_finalize<FIN_ALPHA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, g1, a.partials, a.rerr, a.rerr_len, a.tol);
     if (a.fused) {
-      precond(a.r, a.sh, a.v);  // sh = M^-1 (r - alpha v)
+      precond(a.r, a.ppair ? a.ph : a.sh, a.v, a.ppair ? a.shi : nullptr);  // sh = M^-1 (r - alpha v)
     } else {
       k_supdate<T, V><<<nblk, BS, 0, s>>>(a);
       precond(a.s, a.sh);
@@ -894,7 +939,7 @@ struct Engine {
       MT_HIP_CHECK(ctx, hipGetLastError());
       return MT_OK;
     }
-    if (gexec && (g_nact != a.nact || g_chunk != chunk)) {
+    if (gexec && (g_nact != a.nact || g_chunk != chunk || g_npair != npair)) {
       hipGraphExecDestroy(gexec);
       gexec = nullptr;
     }
@@ -909,6 +954,7 @@ struct Engine {
       hipGraphDestroy(graph);
       g_nact = a.nact;
       g_chunk = chunk;
+      g_npair = npair;
     }
     MT_HIP_CHECK(ctx, hipGraphLaunch(gexec, ctx->stream));
     return MT_OK;
@@ -1168,6 +1214,14 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
   a.b = nullptr;
   a.fused = (o.precond == PCK_DILU || o.precond == PCK_DILU_RB) ? 1 : 0;
   a.s_ok = (bricks && a.fused) ? 1 : 0;   // the brick sweeps store s
+  if (bricks && ctx->bk_pairs) {
+    // p-hat and s-hat interleaved by system pair, read by the pair SpMV
+    const size_t npv = size_t(nI) * 2 * ((nsys + 1) / 2);
+    if ((rc = ws(ctx, "bk_phi", npv, &a.phi)) || (rc = ws(ctx, "bk_shi", npv, &a.shi)) ||
+        (rc = ws(ctx, "bk_pairs", size_t((nsys + 1) / 2), &E.d_pairs)))
+      return rc;
+    a.ppair = 2 * nI;
+  }
   C2<float>* dinv = nullptr;
   if ((rc = build_dinv<float>(ctx, o.precond, tr, freqs, &dinv))) return rc;
   if (bricks) {
@@ -1266,6 +1320,7 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
     MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, iact.data(), iact.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     a.nact = int(iact.size());
     a.G = pick_G(ctx, a.nact);
+    if (a.ppair) E.set_pairs(iact);
     if (bricks) {
       k_bk_to_brick<<<4096, BS, 0, ctx->stream>>>(d_act, a.nact, ctx->bk.bmap, nI, nE, r_nat, a.r, a.x);
       MT_HIP_CHECK(ctx, hipGetLastError());
@@ -1276,7 +1331,11 @@ static int solve_mixed(mt_ctx* ctx, int nsys, const double* omegas, Z* b, Z* x,
       if ((rc = E.account(hi))) return rc;
       const int ni = compact(ctx, hi, d_act, iact, keep_running);
       if (ni == 0) break;
-      if (ni != a.nact) { a.nact = ni; a.G = pick_G(ctx, ni); }
+      if (ni != a.nact) {
+        a.nact = ni;
+        a.G = pick_G(ctx, ni);
+        if (a.ppair) E.set_pairs(iact);
+      }
     }
     // x += d / scale on the systems that refined
     MT_HIP_CHECK(ctx, hipMemcpyAsync(d_act, outer_act.data(), outer_act.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
@@ -2182,18 +2241,38 @@ int mt_bench_spmv(mt_ctx* ctx, int nsys, int precision, int reps, double* ms_per
     a.partials = partials;
     C2<T>* vin; C2<T>* vout; C2<T>* vrt;
     int r2;
-    const size_t nvb = bricks ? size_t(ctx->bk.g.nB) * nsys : nv;
+    // (bricks: an even number of systems' room, the pair kernel's x)
+    const size_t nvb = bricks ? size_t(ctx->bk.g.nB) * (nsys + (nsys & 1)) : nv;
     if ((r2 = ws(ctx, sizeof(T) == 8 ? "bn_in64" : (bricks ? "bn_inb" : "bn_in32"), nvb, &vin)) ||
         (r2 = ws(ctx, sizeof(T) == 8 ? "bn_out64" : (bricks ? "bn_outb" : "bn_out32"), nvb, &vout)) ||
         (r2 = ws(ctx, sizeof(T) == 8 ? "bn_rt64" : (bricks ? "bn_rtb" : "bn_rt32"), nvb, &vrt)))
       return r2;
     a.ph = vin; a.v = vout; a.rt = vrt;
+    int4* d_pairs = nullptr;
+    const int npair = (nsys + 1) / 2;
+    const bool paired = bricks && ctx->bk_pairs && sizeof(T) == 4;
     if (bricks) {
       a.nE = ctx->bk.g.nB;
       a.G = brick_chunks(ctx->bk.g);
     }
+    if (paired) {
+      // x of every pair interleaved in vin, as the inner solves keep p-hat
+      std::vector<int4> hp(npair);
+      for (int m = 0; m < npair; ++m) hp[m] = {2 * m, 2 * m + 1, 2 * m, 2 * m + 1 < nsys ? 2 * m + 1 : -1};
+      if ((r2 = ws(ctx, "bn_pairs", size_t(npair), &d_pairs))) return r2;
+      MT_HIP_CHECK(ctx, hipMemcpy(d_pairs, hp.data(), npair * sizeof(int4), hipMemcpyHostToDevice));
+      a.phi = vin;
+      a.ppair = 2 * ctx->bk.g.nB;
+    }
     auto launch = [&]() {
-      if (bricks) {
+      if (paired) {
+        if constexpr (sizeof(T) == 4) {
+          const int sg = ctx->bk_sg > 0 ? ctx->bk_sg : nsys;
+          const int spg = std::max(1, sg / 2);
+          k_spmv_btp<1><<<a.G * ((npair + spg - 1) / spg), kBtThreads, 0, ctx->stream>>>(
+              a, ctx->bk.g, reinterpret_cast<const float*>(ctx->bk.rows[0]), d_pairs, npair, spg);
+        }
+      } else if (bricks) {
         const int sg = ctx->bk_sg > 0 ? ctx->bk_sg : nsys;
         k_spmv_bt<T, T, 1><<<a.G * ((nsys + sg - 1) / sg), kBtThreads, 0, ctx->stream>>>(
             a, ctx->bk.g, reinterpret_cast<const T*>(ctx->bk.rows[0]), sg);
@@ -2312,9 +2391,10 @@ int mt_set_dilu_variant(mt_ctx* ctx, int variant) {
 }
 
 int mt_set_inner_layout(mt_ctx* ctx, int bricks, int systems_per_block) {
-  if (!ctx || (bricks != 0 && bricks != 1) || systems_per_block < 0 || systems_per_block > 64)
-    return mt_fail(ctx, MT_E_ARG, "mt_set_inner_layout: bricks is 0 or 1, systems_per_block 0..64");
+  if (!ctx || bricks < 0 || bricks > 2 || systems_per_block < 0 || systems_per_block > 64)
+    return mt_fail(ctx, MT_E_ARG, "mt_set_inner_layout: bricks is 0, 1 or 2, systems_per_block 0..64");
   ctx->brick = bricks != 0;
+  ctx->bk_pairs = bricks == 2;
   ctx->bk_sg = systems_per_block;
   return MT_OK;
 }

@@ -471,10 +471,12 @@ class Context:
 
     def set_inner_layout(self, bricks, systems_per_block=8):
         """Inner FP32 vectors of the mixed solver (red-black DILU, default
-        SpMV) on bricks (True, default) or in the all-edge layout (False);
-        systems_per_block: workgroup size in systems of the brick SpMV (0:
-        all active systems; default 8)."""
-        self._chk(lib().mt_set_inner_layout(self._h, int(bool(bricks)), int(systems_per_block)))
+        SpMV): True or 2 (default) = bricks with p-hat and s-hat interleaved
+        by system pair (the pair SpMV); 1 = bricks, system-major; False or 0 =
+        the all-edge layout.  systems_per_block: workgroup size in systems of
+        the brick SpMV (0: all active systems; default 8)."""
+        b = (2 if bricks else 0) if isinstance(bricks, bool) else int(bricks)
+        self._chk(lib().mt_set_inner_layout(self._h, b, int(systems_per_block)))
 
     def set_tuning(self, groups):
         self._chk(lib().mt_set_tuning(self._h, int(groups)))

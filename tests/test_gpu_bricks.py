@@ -96,6 +96,33 @@ def test_brick_dilu_persistent_matches(gpu_ctx):
 
 
 @pytest.mark.parametrize("transpose", [False, True])
+def test_brick_paired_x_matches_unpaired(gpu_ctx, transpose):
+    """Paired x (p-hat and s-hat interleaved by system pair, written so by the
+    DILU and gathered 16 bytes at a time for both systems by the pair SpMV)
+    does the unpaired kernels' arithmetic: batched mixed solves are bitwise
+    equal.  Five systems: a pair with one member, and systems that stop at
+    different iterations (pairs whose other member has stopped)."""
+    print(f"case brick paired x transpose={transpose}", flush=True)
+    o, g = _grid(gpu_ctx, 45)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [1e-2, 0.5, 20.0])
+    om, B, X0 = om[:5], B[:5], X0[:5]
+    res = {}
+    try:
+        for layout in (2, 1):
+            gpu_ctx.set_inner_layout(layout)
+            X, st = gpu_ctx.solve_batch(om, B, X0, tol=1e-10, maxit=4000, precision=mtsp2.PREC_MIXED,
+                                        precond=mtsp2.PC_DILU_RB, op_transpose=int(transpose))
+            res[layout] = (X, [(s["status"], s["niter"], s["nouter"]) for s in st])
+    finally:
+        gpu_ctx.set_inner_layout(True)   # the library's default (paired)
+    assert all(s == 0 for s, _, _ in res[2][1]), res[2][1]
+    assert res[2][1] == res[1][1]
+    assert len({n for _, n, _ in res[2][1]}) > 1, res[2][1]
+    assert np.array_equal(res[2][0], res[1][0])
+
+
+@pytest.mark.parametrize("transpose", [False, True])
 def test_brick_solves_match_all_edge_layout(gpu_ctx, transpose):
     """Mixed batched solves (red-black DILU) with the inner vectors on
     bricks and in the all-edge layout: both reach the FP64 tolerance and the
