@@ -122,6 +122,26 @@ def test_brick_paired_x_matches_unpaired(gpu_ctx, transpose):
     assert np.array_equal(res[2][0], res[1][0])
 
 
+def test_brick_paired_graph_and_eager_agree(gpu_ctx):
+    """The mixed solver with paired x replayed as hipGraphs gives the eager
+    launches' iterates: short chunks, so the graphs are captured again as
+    systems stop (the pair list and its length change with the active
+    list)."""
+    print("case brick paired graph-vs-eager", flush=True)
+    o, g = _grid(gpu_ctx, 46)
+    grid = {"dx": g[0], "dy": g[1], "dz": g[2], "nzAir": g[3]}
+    om, B, X0 = P.forward_systems(gpu_ctx, grid, [1e-2, 0.5, 20.0])
+    om, B, X0 = om[:5], B[:5], X0[:5]
+    res = {}
+    for graph in (1, 0):
+        X, st = gpu_ctx.solve_batch(om, B, X0, tol=1e-10, maxit=4000, precision=mtsp2.PREC_MIXED,
+                                    precond=mtsp2.PC_DILU_RB, use_graph=graph, chunk=3)
+        res[graph] = (X, [(s["status"], s["niter"], s["nouter"]) for s in st])
+    assert all(s == 0 for s, _, _ in res[1][1]), res[1][1]
+    assert res[1][1] == res[0][1]
+    assert np.array_equal(res[1][0], res[0][0])
+
+
 @pytest.mark.parametrize("transpose", [False, True])
 def test_brick_solves_match_all_edge_layout(gpu_ctx, transpose):
     """Mixed batched solves (red-black DILU) with the inner vectors on
