@@ -194,6 +194,42 @@ __global__ __launch_bounds__(BS) void k_bk_widen(const C2<float>* __restrict__ x
     y[q] = {double(x[q].x), double(x[q].y)};
 }
 
+// k_pupdate on the tiles of one colour of the red-black plan (tiles[0,
+// ntiles): {lattice, ti, tj, tk}): the black tiles' p, when the red forward
+// sweep forms the red tiles' itself.  Block (system, chunk g of the tiles),
+// as k_pupdate's; the same arithmetic per point.
+template <class T, class V>
+__global__ __launch_bounds__(BS) void k_pupdate_tiles(IterArgs<T, V> a, const int4* __restrict__ tiles,
+                                                      int ntiles, LatticeInfo l0, LatticeInfo l1,
+                                                      LatticeInfo l2) {
+  int si, g;
+  map_sys_block(a.nact, a.G, si, g);
+  const int sys = a.act[si];
+  const SysScal& sc = a.sc[sys];
+  if (sc.status != ST_RUN) return;
+  const int64_t o = (int64_t)sys * a.nE;
+  const bool rs = sc.restart_due;
+  const C2<T> beta = ccast<T>(sc.beta), om = ccast<T>(sc.omg);
+  const int t0 = int(int64_t(ntiles) * g / a.G), t1 = int(int64_t(ntiles) * (g + 1) / a.G);
+  for (int t = t0; t < t1; ++t) {
+    const int4 tl = tiles[t];
+    const LatticeInfo L = tl.x == 0 ? l0 : (tl.x == 1 ? l1 : l2);
+    const int64_t b0 = o + (int64_t)(L.tbase + tl.y + L.ntx * (tl.z + L.nty * tl.w)) * kBkPts;
+    for (int q = threadIdx.x; q < kBkPts; q += BS) {
+      const int64_t e = b0 + q;
+      const C2<T> r = a.r[e];
+      C2<T> p;
+      if (rs) {
+        a.rt[e] = r;
+        p = r;
+      } else {
+        p = cadd(r, cmul(beta, csub(a.p[e], cmul(om, a.v[e]))));
+      }
+      a.p[e] = p;
+    }
+  }
+}
+
 // Point p of tile t for the brick SpMV: whether it is an interior edge, its
 // brick index e, its 14 neighbours' brick indices (a neighbour that is not an
 // interior edge has a zero coupling and the row's own index) and its 16
@@ -805,9 +841,10 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bk(DiluArgs<T, V> a, const Til
 // in flight into registers.  The sweeps leave the loads little to overlap
 // with otherwise: a workgroup's tile data arrive in one burst and the memory
 // pipe idles while its waves sweep.  The arithmetic is k_dilu_bk's.
-template <class T, class V, int MODE, int NW, bool ZP = false>
+template <class T, class V, int MODE, int NW, bool ZP = false, bool PU = false>
 __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const TileShape* shp) {
   constexpr int TX = 8, NP = kRbNP, NU = 8, NLEV = 22, NB = kRbNB;
+  static_assert(!PU || MODE == RB_RED_FWD, "the p-update is folded into the red forward sweep");
   constexpr bool FWDP = MODE != RB_RED_BWD;   // has a forward sweep
   constexpr bool BWDP = MODE != RB_RED_FWD;   // has a backward sweep
   constexpr bool HALO = MODE != RB_RED_FWD;   // reads face neighbours
@@ -854,9 +891,12 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
   struct Geo {
     int ok, sys, fidx, tself, ex, ey, ez;
     C2<T> al;
+    int rs;           // PU: restart due
+    C2<T> beta, om;   // PU: beta, omega
   };
   Geo nx;
   C2<T> rv[NU], v2[NU], dd[NU], fv[6];
+  C2<T> r3[PU ? NU : 1];   // PU: r
   V cp[CPT][6];
   T fc[3];
   auto load = [&](int wi) {
@@ -870,6 +910,11 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
       nx.ok = sc.status == a.status_req;
       nx.fidx = sc.fidx;
       nx.al = ccast<T>(sc.alpha);
+      if constexpr (PU) {
+        nx.rs = sc.restart_due;
+        nx.beta = ccast<T>(sc.beta);
+        nx.om = ccast<T>(sc.omg);
+      }
     }
     const int4 tl = a.tiles[tix];
     const LatticeInfo L = a.lat[tl.x];
@@ -897,7 +942,12 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
     for (int u = 0; u < NU; ++u) {
       const int p = lane + 64 * u;
       rv[u] = pin[p];   // points past a partial tile's extent hold zeros
-      v2[u] = (FWDP && pin2) ? pin2[p] : C2<T>{T(0), T(0)};
+      if constexpr (PU) {
+        v2[u] = a.pu_v[o + tb + p];
+        r3[u] = a.pu_r[o + tb + p];
+      } else {
+        v2[u] = (FWDP && pin2) ? pin2[p] : C2<T>{T(0), T(0)};
+      }
       dd[u] = dinv[p];
     }
     if (HALO) {
@@ -974,7 +1024,17 @@ __global__ __launch_bounds__(64 * NW) void k_dilu_bkp(DiluArgs<T, V> a, const Ti
 #pragma unroll
       for (int u = 0; u < NU; ++u) {
         C2<T> r = rv[u];
-        if (FWDP && a.in2) {
+        if constexpr (PU) {
+          // k_pupdate's p, stored for the red tiles
+          const int64_t pe = (int64_t)cu.sys * a.vstride + (int64_t)cu.tself * kBkPts + lane + 64 * u;
+          if (cu.rs) {
+            a.pu_rt[pe] = r3[u];
+            r = r3[u];
+          } else {
+            r = cadd(r3[u], cmul(cu.beta, csub(r, cmul(cu.om, v2[u]))));
+          }
+          a.pu_p[pe] = r;
+        } else if (FWDP && a.in2) {
           r = csub(r, cmul(cu.al, v2[u]));
           if (so) so[lane + 64 * u] = r;
         }

@@ -101,6 +101,15 @@ struct DiluArgs {
   // the black tiles' j- and k-faces from zout
   C2<T>* zout = nullptr;
   int64_t zpair = 0;
+  // persistent red forward sweep of the first application, when pu_r is set:
+  // its right-hand side is the BiCGStab p = r + beta (p - omega v) (on a
+  // restart p = r and rt = r), formed from r (pu_r), the previous p (in) and
+  // v (pu_v), and stored (pu_p, pu_rt) for the red tiles (the black tiles'
+  // by k_pupdate_tiles before the launch)
+  const C2<T>* pu_r = nullptr;
+  const C2<T>* pu_v = nullptr;
+  C2<T>* pu_p = nullptr;
+  C2<T>* pu_rt = nullptr;
   C2<T>* out;              // forward: y; backward: in place y -> z
   const C2<T>* dinv;       // [nfreq][cstride]
   int status_req;          // only systems with this status are processed

@@ -681,7 +681,9 @@ static void dilu_bk_apply(mt_ctx* ctx, const TilePlan& tp, DiluArgs<T, V> a) {
       if (ctx->rb_variant == 6) {
         // persistent: three workgroups per CU (their LDS), a multiple of 8 XCDs
         const int gp = std::min(g, std::max(8, ctx->num_cu * 3 / 8 * 8));
-        if (step == 0) k_dilu_bkp<T, V, RB_RED_FWD, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        if (step == 0 && a.pu_r)   // (the p-update folded into the red forward sweep)
+          k_dilu_bkp<T, V, RB_RED_FWD, NW, ZP, true><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
+        else if (step == 0) k_dilu_bkp<T, V, RB_RED_FWD, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
         else if (step == 1) k_dilu_bkp<T, V, RB_BLACK_FB, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
         else k_dilu_bkp<T, V, RB_RED_BWD, NW, ZP><<<gp, 64 * NW, 0, ctx->stream>>>(a, tp.shape);
         return;
@@ -833,7 +835,9 @@ struct Engine {
   // out = M^-1 (in - alpha*in2) (in2 may be null); with zout (bricks, paired
   // x) the result goes to zout pair-interleaved and out holds the red tiles'
   // intermediate w
-  void precond(const C2<T>* in, C2<T>* out, const C2<T>* in2 = nullptr, C2<T>* zout = nullptr) {
+  // pu: the red forward sweep also forms p (k_pupdate) for the red tiles
+  void precond(const C2<T>* in, C2<T>* out, const C2<T>* in2 = nullptr, C2<T>* zout = nullptr,
+               bool pu = false) {
     if (a.pc != PCK_DILU && a.pc != PCK_DILU_RB) return;
     const Grid& g = ctx->hop.g;
     const TilePlan& tp = a.pc == PCK_DILU_RB ? ctx->tiles_rb : ctx->tiles;
@@ -845,6 +849,12 @@ struct Engine {
     d.sout = (brick && in2) ? a.s : nullptr;
     d.zout = brick ? zout : nullptr;
     d.zpair = a.ppair;
+    if (pu) {
+      d.pu_r = a.r;
+      d.pu_v = a.v;
+      d.pu_p = a.p;
+      d.pu_rt = a.rt;
+    }
     d.xf = xf;
     d.xfsys = xfsys;
     d.xfhalf = xfhalf;
@@ -893,12 +903,25 @@ struct Engine {
     return G;
   }
 
+  // the p-update folded into the first application's red forward sweep:
+  // inner solves on bricks with the persistent red-black sweeps
+  bool pfold() const {
+    return brick && a.fused && a.pc == PCK_DILU_RB && ctx->rb_variant == 6 && ctx->tiles_rb.nlev == 2;
+  }
   void launch_iteration() {
     const int nblk = a.nact * a.G;
     hipStream_t s = ctx->stream;
-    k_pupdate<T, V><<<nblk, BS, 0, s>>>(a);
     // (paired x: p-hat to phi, the sweeps' w in ph)
-    precond(a.p, a.ph, nullptr, a.ppair ? a.phi : nullptr);
+    if (pfold()) {
+      // p of the black tiles here, of the red ones in the red forward sweep
+      const TilePlan& tp = ctx->tiles_rb;
+      k_pupdate_tiles<T, V><<<nblk, BS, 0, s>>>(a, tp.tiles + tp.lvl_off[1], tp.lvl_off[2] - tp.lvl_off[1],
+                                                tp.lat[0], tp.lat[1], tp.lat[2]);
+      precond(a.p, a.ph, nullptr, a.ppair ? a.phi : nullptr, true);
+    } else {
+      k_pupdate<T, V><<<nblk, BS, 0, s>>>(a);
+      precond(a.p, a.ph, nullptr, a.ppair ? a.phi : nullptr);
+    }
     const int g1 = spmv<1>(nblk);
     k_finalize<FIN_ALPHA><<<a.nact, BS, 0, s>>>(a.sc, a.act, a.nact, g1, a.partials, a.rerr, a.rerr_len, a.tol);
     if (a.fused) {
@@ -996,10 +1019,13 @@ struct Engine {
       ctx->kt[MT_KT_SPMV].bytes += cnt[MT_KT_SPMV] * (double(stencil_nnz(ctx)) * sv + n * sv) +
                                    2.0 * units * 3.0 * n * sc2;
     // (the second application of an iteration reads r and v when fused, and
-    // on bricks writes s = r - alpha v as well)
+    // on bricks writes s = r - alpha v as well; with the p-update folded, the
+    // first one's red forward sweep reads r and v and writes p for the red
+    // tiles, half of the points)
     if (cnt[MT_KT_PRECOND])
       ctx->kt[MT_KT_PRECOND].bytes += cnt[MT_KT_PRECOND] * 6.0 * n * sv +
-                                      units * (3.0 + (a.fused ? (a.s_ok ? 5.0 : 4.0) : 3.0)) * n * sc2;
+                                      units * (3.0 + (pfold() ? 1.5 : 0.0) +
+                                               (a.fused ? (a.s_ok ? 5.0 : 4.0) : 3.0)) * n * sc2;
     return MT_OK;
   }
   ~Engine() {
