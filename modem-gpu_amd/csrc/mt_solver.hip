@@ -129,7 +129,7 @@ struct mt_ctx {
   TilePlan tiles_rb;  // red-black tile ordering of the same lattices
   BrickData bk;       // bricks of the mixed solver's inner solves
   bool brick = true;  // inner solves on bricks (mixed, red-black DILU, 4-system SpMV)
-  int bk_sg = 8;      // brick SpMV: systems per block (0: all active systems)
+  int bk_sg = 16;     // brick SpMV: systems per block (0: all active systems)
   bool bk_pairs = true;  // bricks: p-hat, s-hat interleaved by system pair (pair SpMV)
   DivcorData dc;
   SiteData sites;
