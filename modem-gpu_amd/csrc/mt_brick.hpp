@@ -12,6 +12,12 @@
 // contiguous per system, and the SpMV's k-1 and k+1 neighbours of 7 of a
 // tile's 8 planes lie in the tile itself.
 //
+// The preconditioned vectors p-hat and s-hat, which the SpMV gathers, can be
+// kept with the systems of a pair (2m, 2m+1) side by side, entry
+// (s/2)*2nB + 2*point + s%2: the SpMV then reads both systems of a pair with
+// one 16-byte gather per neighbour (k_spmv_btp), the DILU writes them so
+// (DiluArgs::zout), and every other inner vector stays system-major.
+//
 // The outer FP64 defect correction stays in the all-edge layout: its scaled
 // residual is permuted into bricks before an inner solve and the inner
 // correction permuted back after it (k_bk_to_brick / k_bk_from_brick, once
