@@ -190,7 +190,7 @@ def main():
                          "p-hat and s-hat interleaved by system pair (bricks), the same with "
                          "every vector system-major (bricks-unpaired), or the all-edge layout "
                          "of the outer solve")
-    ap.add_argument("--brick-systems", type=int, default=16,
+    ap.add_argument("--brick-systems", type=int, default=0,
                     help="systems per workgroup of the brick SpMV (0: all)")
     ap.add_argument("--autotune", action="store_true",
                     help="time and pick the brick SpMV's systems per workgroup and the vector "

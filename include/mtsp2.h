@@ -402,7 +402,7 @@ int mt_set_dilu_variant(mt_ctx *ctx, int variant);
  * 2m and 2m+1 side by side, so the SpMV reads both with one 16-byte gather;
  * bitwise equal to 1); 1 = bricks, every vector system-major; 0 = the
  * all-edge layout of the outer solve.  systems_per_block: systems per
- * workgroup of the brick SpMV (0 = all active systems; default 16).  The same
+ * workgroup of the brick SpMV (0, the default: all active systems).  The same
  * operator and preconditioner; bricks and the all-edge layout differ in the
  * rounding of the inner sums.  No counterpart in the reference. */
 int mt_set_inner_layout(mt_ctx *ctx, int bricks, int systems_per_block);

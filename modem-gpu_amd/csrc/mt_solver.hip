@@ -129,7 +129,7 @@ struct mt_ctx {
   TilePlan tiles_rb;  // red-black tile ordering of the same lattices
   BrickData bk;       // bricks of the mixed solver's inner solves
   bool brick = true;  // inner solves on bricks (mixed, red-black DILU, 4-system SpMV)
-  int bk_sg = 16;     // brick SpMV: systems per block (0: all active systems)
+  int bk_sg = 0;      // brick SpMV: systems per block (0: all active systems)
   bool bk_pairs = true;  // bricks: p-hat, s-hat interleaved by system pair (pair SpMV)
   DivcorData dc;
   SiteData sites;
@@ -876,7 +876,8 @@ struct Engine {
       const int sg = ctx->bk_sg > 0 ? ctx->bk_sg : a.nact;
       if constexpr (std::is_same<T, float>::value && std::is_same<V, float>::value) {
         if (a.ppair) {
-          const int spg = std::max(1, sg / 2);
+          // pairs per block (0: every active pair in one group)
+          const int spg = ctx->bk_sg > 0 ? std::max(1, ctx->bk_sg / 2) : std::max(1, npair);
           const int ngrp = (npair + spg - 1) / spg;
           k_spmv_btp<WHICH><<<b.G * ngrp, kBtThreads, 0, ctx->stream>>>(b, bg, bk_rows, d_pairs,
                                                                        npair, spg);

@@ -469,12 +469,12 @@ class Context:
         systems per workgroup in the lean LDS layout."""
         self._chk(lib().mt_set_dilu_variant(self._h, int(variant)))
 
-    def set_inner_layout(self, bricks, systems_per_block=16):
+    def set_inner_layout(self, bricks, systems_per_block=0):
         """Inner FP32 vectors of the mixed solver (red-black DILU, default
         SpMV): True or 2 (default) = bricks with p-hat and s-hat interleaved
         by system pair (the pair SpMV); 1 = bricks, system-major; False or 0 =
         the all-edge layout.  systems_per_block: workgroup size in systems of
-        the brick SpMV (0: all active systems; default 16)."""
+        the brick SpMV (0, the default: all active systems)."""
         b = (2 if bricks else 0) if isinstance(bricks, bool) else int(bricks)
         self._chk(lib().mt_set_inner_layout(self._h, b, int(systems_per_block)))
 
